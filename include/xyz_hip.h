@@ -1,0 +1,213 @@
+/*
+ * xyz_hip.h -- C ABI of the MI355X (gfx950) kd-tree closest-hit and
+ * path-tracing library.
+ *
+ * Conventions
+ *   - Every function returns XYZ_OK (0) or a negative XYZ_E_* code; the
+ *     message of the last failure on the calling thread is xyz_last_error().
+ *   - No exception crosses this boundary; no torch or HIP type appears in it.
+ *     `stream` arguments are hipStream_t values passed as void* (NULL = the
+ *     null stream).
+ *   - Primitive ids are the caller's triangle order (the index in the vectors
+ *     given to xyz_scene_create), i.e. the index a caller would have in the
+ *     vector handed to KDTree::Build (reference xyz/geom/kdtree.cpp:207).
+ *   - Scene descriptors and tree exports are HOST memory.  Ray, hit, film and
+ *     gradient buffers of the compute entry points are DEVICE memory (any
+ *     pointer the GPU can dereference), so the inputs of a hot loop stay in HBM.
+ *   - A scene handle is immutable after creation; concurrent calls with one
+ *     handle on different streams are allowed.
+ */
+#ifndef XYZ_HIP_H_
+#define XYZ_HIP_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  XYZ_OK = 0,
+  XYZ_E_INVALID = -1,     /* bad argument (NULL, size, non-finite vertex, ...) */
+  XYZ_E_NOMEM = -2,       /* host or device allocation failed */
+  XYZ_E_DEVICE = -3,      /* a HIP call failed */
+  XYZ_E_UNSUPPORTED = -4  /* a size beyond a documented limit */
+};
+
+/* BSDF kinds, same order as Bsdf::type (reference xyz/bsdf/bsdf.hpp:21-28). */
+enum { XYZ_LAMBERT = 0, XYZ_MIRROR = 1, XYZ_GLASS = 2, XYZ_METAL = 3, XYZ_LIGHT = 4 };
+
+typedef struct xyz_kdtree xyz_kdtree;
+typedef struct xyz_scene xyz_scene;
+
+/* One material.  Replaces the Bsdf subclasses' constructor arguments
+ * (reference xyz/bsdf/lambert.hpp:12, metal.hpp:15, glass.cpp:119, light.hpp).
+ *   rgb: albedo (LAMBERT, METAL) or emitted radiance L_e (LIGHT); the light
+ *        power the integrator uses is Phi = L_e * pi * A_lights, so equal L_e on
+ *        every light reproduces the reference's single global power
+ *        (xyz/core/config.hpp:91-94).
+ *   shininess: METAL lobe exponent.   ior: GLASS index (reference: 1.5).
+ *   texture: -1, or the index of an RGB texture that replaces a LAMBERT albedo
+ *        (nearest texel at the hit's uv; the reference stores uv but never
+ *        reads it, xyz/geom/triangle.hpp:111-113). */
+typedef struct {
+  uint32_t type;
+  float rgb[3];
+  float shininess;
+  float ior;
+  int32_t texture;
+} xyz_bsdf_desc;
+
+/* Scene description (host memory).  Replaces Scene::Load + Scene::Build
+ * (reference xyz/core/scene_mqo.cpp:547-777, xyz/core/scene.cpp:35).
+ *   v0,v1,v2: 3*n_tris floats each.   n0,n1,n2: optional per-vertex shading
+ *   normals (NULL = flat, reference triangle.cpp:10-33).  uv: optional
+ *   6*n_tris.  bsdf: per-triangle material index (NULL = all 0).
+ *   textures: tex_wh[2*i] = width, tex_wh[2*i+1] = height, texels are the
+ *   textures' RGB floats concatenated, row 0 first.
+ *   kd_max_bucket / kd_max_depth: KDTree(max_bucket=16, max_depth=16)
+ *   (reference xyz/geom/kdtree.hpp:19-20); 0 selects the default 16. */
+typedef struct {
+  uint32_t n_tris;
+  const float *v0, *v1, *v2;
+  const float *n0, *n1, *n2;
+  const float *uv;
+  const uint32_t *bsdf;
+  uint32_t n_bsdfs;
+  const xyz_bsdf_desc *bsdfs;
+  uint32_t n_textures;
+  const uint32_t *tex_wh;
+  const float *texels;
+  uint32_t kd_max_bucket;
+  uint32_t kd_max_depth;
+} xyz_scene_desc;
+
+typedef struct {
+  uint64_t n_tris;
+  uint64_t n_nodes;       /* interior + leaf nodes, 8 bytes each on the device */
+  uint64_t n_leaves;      /* including empty leaves (KDNull) */
+  uint64_t n_leaf_refs;   /* triangle references summed over leaves */
+  uint32_t depth;         /* deepest interior level + 1 */
+  uint32_t max_leaf;      /* largest leaf */
+  float bmin[3], bmax[3]; /* root box (bmax is the builder's upper bound) */
+  double build_seconds;   /* host SAH build */
+  uint64_t device_bytes;  /* HBM held by the scene (0 for a host tree) */
+} xyz_scene_info;
+
+/* ---- host-only kd-tree (no GPU needed) ------------------------------------
+ * SAH build with the reference's exact split rules (xyz/geom/kdtree.cpp:207-357)
+ * flattened to the device layout:
+ *   nodes[2*i], nodes[2*i+1], pre-order, left child = i + 1;
+ *     interior: nodes[2*i] = (right_child << 2) | axis, nodes[2*i+1] = split bits
+ *     leaf:     nodes[2*i] = (count << 2) | 3,         nodes[2*i+1] = first ref
+ *   leaf_prims[ref] = primitive id, each leaf's ids ascending (the reference
+ *   keeps caller order in leaves). */
+int xyz_kdtree_build(uint32_t n_tris, const float *v0, const float *v1, const float *v2,
+                     uint32_t max_bucket, uint32_t max_depth, xyz_kdtree **out);
+int xyz_kdtree_info(const xyz_kdtree *tree, xyz_scene_info *info);
+int xyz_kdtree_export(const xyz_kdtree *tree, uint32_t *nodes, uint32_t *leaf_prims);
+void xyz_kdtree_destroy(xyz_kdtree *tree);
+
+/* ---- scenes ---------------------------------------------------------------- */
+int xyz_scene_create(const xyz_scene_desc *desc, xyz_scene **out);
+int xyz_scene_get_info(const xyz_scene *scene, xyz_scene_info *info);
+void xyz_scene_destroy(xyz_scene *scene);
+
+/* ---- closest hit -------------------------------------------------------------
+ * Replaces Scene::FindIntersection / KDTree::FindIntersection
+ * (reference xyz/core/scene.hpp:236-238,277-295; xyz/geom/kdtree.cpp:366-372)
+ * for a batch of n rays.  org/dir: 3*n floats (xyz per ray).  Outputs (n each):
+ *   prim: primitive id or -1 (the reference's nullptr),
+ *   t:    Intersection::t_max() at the hit, +inf on a miss,
+ *   f, g, back: barycentrics and back-face flag (valid on a hit; may be NULL),
+ *   counters: optional 2*n uint32: kd nodes visited and triangles tested per
+ *   ray (replaces Scene::n_FindIntersection, xyz/core/scene.hpp:232-234).
+ * Interval [1e-5, +inf) as Intersection::Init (xyz/geom/intersection.hpp:11-14).
+ * Results are bit-identical to the reference traversal in IEEE binary32. */
+int xyz_closest_hit(xyz_scene *scene, uint32_t n, const float *org, const float *dir,
+                    int32_t *prim, float *t, float *f, float *g, uint8_t *back,
+                    uint32_t *counters, void *stream);
+
+/* ---- path tracing -------------------------------------------------------------
+ * Pinhole camera (reference ThinLensCamera pinhole mode,
+ * xyz/core/thinlenscamera.cpp:71-124) with an explicit vertical field of view:
+ * the lens-film distance is V = 0.0125 / tan(fov_y / 2). */
+typedef struct {
+  float eye[3], at[3], up[3];
+  float fov_y_deg;
+  uint32_t width, height;
+} xyz_camera;
+
+/* Integrator settings (reference constants xyz/core/config.hpp:30,60-61 made
+ * runtime):  max_depth D caps paths at D segments (reference
+ * XYZ_CONFIG_kMaxRandomWalkDepth = D + 1; the reference's 16 is D = 15);
+ * Russian roulette (threshold rr_threshold, reference 0.2) runs at vertices
+ * j > rr_start (0 = every vertex, as the reference).  Samples
+ * [sample_offset, sample_offset + spp) of every pixel are rendered; a sample's
+ * random numbers depend only on (seed, pixel, sample index), so progressive
+ * passes and resumes are exact. */
+typedef struct {
+  uint32_t spp;
+  uint32_t sample_offset;
+  uint32_t max_depth;
+  uint32_t rr_start;
+  float rr_threshold;
+  uint32_t reserved;
+  uint64_t seed;
+} xyz_render_params;
+
+typedef struct {
+  uint64_t paths;          /* camera paths traced */
+  uint64_t queries;        /* closest-hit queries (extension + shadow) */
+  double seconds;          /* device time of the call (HIP events) */
+} xyz_render_stats;
+
+/* Replaces PathTracer::run's pixel loop around Scene::EvalutePathTracing
+ * (reference xyz/integrator/integrator.cpp:119-128,
+ * xyz/core/scene_pathtracing.cpp:11-144).  film: device float[width*height*3],
+ * row 0 = bottom image row (as the reference film), receives += the SUM of the
+ * samples' radiance per pixel (divide by the total spp for the mean, as
+ * ImageFilm::SaveAsPfm does, xyz/core/imagefilm.cpp:153-206).
+ * Wavefront: camera, extend, shade/NEE, shadow and scatter kernels with
+ * per-bounce compaction.  stats may be NULL. */
+int xyz_render(xyz_scene *scene, const xyz_camera *camera, const xyz_render_params *params,
+               float *film, xyz_render_stats *stats, void *stream);
+
+/* Path-replay adjoint of xyz_render (no counterpart in the reference).  Given
+ * d_film = dLoss/d(film SUM) (device float[w*h*3]), adds dLoss/d(texel) to
+ * d_texels (device, laid out like the scene's texels) and dLoss/dL_e to
+ * d_emission (device float[3*n_bsdfs]; lights only).  Paths are re-traced from
+ * their counter-based seeds, so nothing is stored between the passes. */
+int xyz_render_backward(xyz_scene *scene, const xyz_camera *camera,
+                        const xyz_render_params *params, const float *d_film, float *d_texels,
+                        float *d_emission, void *stream);
+
+/* ---- batched rendering (no counterpart in the reference) ---------------------
+ * Renders n independent (scene, camera, params, film) jobs in one persistent
+ * launch over a global work queue of pixel tiles.  tuning may be NULL (defaults)
+ * or come from xyz_batch_autotune. */
+typedef struct {
+  uint32_t tile_pixels;    /* pixels per work item */
+  uint32_t blocks_per_cu;  /* persistent workgroups per compute unit */
+  uint32_t block_threads;  /* threads per workgroup */
+  uint32_t use_graph;      /* capture the launch in a hipGraph and replay it */
+} xyz_batch_tuning;
+
+int xyz_render_batch(uint32_t n, xyz_scene *const *scenes, const xyz_camera *cameras,
+                     const xyz_render_params *params, float *const *films,
+                     const xyz_batch_tuning *tuning, xyz_render_stats *stats, void *stream);
+
+/* Times candidate tunings on the given jobs (films are overwritten) and writes
+ * the fastest to *best. */
+int xyz_batch_autotune(uint32_t n, xyz_scene *const *scenes, const xyz_camera *cameras,
+                       const xyz_render_params *params, float *const *films,
+                       xyz_batch_tuning *best, void *stream);
+
+/* ---- misc ---------------------------------------------------------------------- */
+const char *xyz_last_error(void);
+const char *xyz_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* XYZ_HIP_H_ */

@@ -1,0 +1,1117 @@
+// xyz_oracle.cpp -- CPU restatement of the reference renderer's hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker that the HIP product is
+// compared against; only the test suite, the smoke check and the CPU-baseline
+// leg of the benchmark load it.  Nothing in the product links or calls it.
+//
+// It restates, operation for operation, what the reference computes:
+//   * Triangle precompute and double-faced ray/triangle test
+//       xyz/geom/triangle.cpp:10-60, 75-85, 127-165; xyz/geom/triangle.hpp:122-156
+//   * SAH kd-tree build (recursive, per-node event sort)
+//       xyz/geom/kdtree.cpp:207-357
+//   * recursive kd traversal, first-hit-leaf early exit, last-accept-wins leaves
+//       xyz/geom/kdtree.cpp:82-122, 140-177, 366-372; xyz/geom/intersection.hpp:11-14
+//   * brute-force closest hit                          xyz/core/scene.hpp:283-293
+//   * shading basis / area sampling / light CDF        xyz/geom/triangle.cpp:194-263,
+//                                                      xyz/core/scene.cpp:42-78
+//   * pinhole camera                                   xyz/core/thinlenscamera.cpp:71-124
+//   * path tracing with NEE + power-heuristic MIS + RR xyz/core/scene_pathtracing.cpp:11-144
+//   * BSDFs: Lambert  xyz/bsdf/lambert.cpp:125-187, Mirror xyz/bsdf/mirror.cpp:50-107,
+//            Glass    xyz/bsdf/glass.cpp:96-165,    Metal  xyz/bsdf/metal.cpp:68-153,
+//            Light    xyz/bsdf/light.cpp:18-33, helpers xyz/stdafx.hpp:44-168
+//   * progressive driver + film                        xyz/integrator/integrator.cpp:88-162
+//
+// Arithmetic contract: IEEE binary32, expression order exactly as the reference
+// sources (dot = (a0*b0 + a1*b1) + a2*b2, libhi/hi/math/vector3.hpp:121-122),
+// no FMA contraction (built with -ffp-contract=off), correctly rounded / and sqrt.
+// Where the reference mixes in double (M_1_PI, 0.5 literals) the same promotions
+// are made here.
+//
+// Documented departures from the reference (each reduces to the reference in
+// the special case named):
+//   * RNG: the reference's mt19937 "next<T>" header is absent from its tree, so
+//     samples come from Philox4x32-10 keyed by (pixel, seed) with counter
+//     (sample, bounce, seed_hi, 0); slot use per call is fixed (see pt_path()).
+//   * cos/sin of the sampled azimuth use a double-precision Taylor evaluation
+//     (xyz_cos/xyz_sin below) instead of libm so that CPU and GPU agree bit for
+//     bit; it is within 1 ulp of libm's float result (tested).
+//   * per-light emission L_e (Phi = L_e * pi * A_total; equal L_e == reference),
+//     pinhole with explicit vertical FOV, runtime depth cap kMax = D + 1,
+//     RR enabled at vertices j > rr_start, optional per-texel Lambert albedo.
+//   * a light-CDF upper_bound that lands on end() (undefined behaviour in the
+//     reference) is clamped to the last light.
+//
+// Parity status of each part and the checks that pin it are listed in the
+// tests (tests/test_oracle_*.py).
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+typedef float real;
+const real EPSILON = real(1e-6);  // xyz/stdafx.hpp:35
+
+// ---------------------------------------------------------------- vectors
+struct V3 {
+  real x[3];
+  V3() { x[0] = x[1] = x[2] = 0; }
+  V3(real a, real b, real c) { x[0] = a; x[1] = b; x[2] = c; }
+  real& operator[](int i) { return x[i]; }
+  real operator[](int i) const { return x[i]; }
+};
+inline V3 operator+(V3 const& a, V3 const& b) { return V3(a[0] + b[0], a[1] + b[1], a[2] + b[2]); }
+inline V3 operator-(V3 const& a, V3 const& b) { return V3(a[0] - b[0], a[1] - b[1], a[2] - b[2]); }
+inline V3 operator-(V3 const& a) { return V3(-a[0], -a[1], -a[2]); }
+inline V3 operator*(V3 const& a, V3 const& b) { return V3(a[0] * b[0], a[1] * b[1], a[2] * b[2]); }
+inline V3 operator*(V3 const& a, real s) { return V3(a[0] * s, a[1] * s, a[2] * s); }
+inline V3 operator*(real s, V3 const& a) { return V3(s * a[0], s * a[1], s * a[2]); }
+inline V3 operator/(V3 const& a, real s) { return V3(a[0] / s, a[1] / s, a[2] / s); }
+// libhi/hi/math/vector3.hpp:119-131
+inline real dot(V3 const& a, V3 const& b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+inline V3 cross(V3 const& a, V3 const& b) {
+  return V3(a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]);
+}
+// libhi/hi/math/math.hpp:146-157,235-257
+inline real rcp(real x) { return real(1) / x; }
+inline real rsqrt(real x) { return real(1) / std::sqrt(x); }
+inline real length_squared(V3 const& a) { return dot(a, a); }
+inline real length(V3 const& a) { return std::sqrt(length_squared(a)); }
+inline V3 normalize(V3 const& a) { return a * rsqrt(length_squared(a)); }
+// libhi/hi/math/math.hpp min/max: (x<y)?x:y and (x<y)?y:x
+inline real hmin(real x, real y) { return (x < y) ? x : y; }
+inline real hmax(real x, real y) { return (x < y) ? y : x; }
+
+// --------------------------------------------------------- deterministic cos/sin
+// Double-precision evaluation of cos/sin of a float angle: quadrant reduction by
+// a two-part pi/2 (P1 has 33 significant bits, so k*P1 is exact for the |k| <= 8
+// that float angles in [0, 4pi) give), then Taylor series to degree 21/22 on
+// |r| <= pi/4.
+double taylor_sin(double r) {
+  double r2 = r * r, term = r, sum = r;
+  for (int k = 1; k <= 10; ++k) {
+    term = term * (-r2) / double((2 * k) * (2 * k + 1));
+    sum = sum + term;
+  }
+  return sum;
+}
+double taylor_cos(double r) {
+  double r2 = r * r, term = 1.0, sum = 1.0;
+  for (int k = 1; k <= 11; ++k) {
+    term = term * (-r2) / double((2 * k - 1) * (2 * k));
+    sum = sum + term;
+  }
+  return sum;
+}
+void reduce_quadrant(float xf, double* r, int* q) {
+  double x = double(xf);
+  double k = std::floor(x * 0.63661977236758134308 + 0.5);  // 2/pi
+  // pi/2 = P1 + P1t to about 4e-27
+  const double P1 = 1.5707963267341256141662597656;    // 0x1.921fb544p+0
+  const double P1t = 6.07710050650619224932e-11;
+  double rr = (x - k * P1) - k * P1t;
+  *r = rr;
+  long long ki = (long long)k;
+  *q = int(((ki % 4) + 4) % 4);
+}
+float xyz_cos(float a) {
+  double r; int q;
+  reduce_quadrant(a, &r, &q);
+  double v;
+  switch (q) {
+    case 0: v = taylor_cos(r); break;
+    case 1: v = -taylor_sin(r); break;
+    case 2: v = -taylor_cos(r); break;
+    default: v = taylor_sin(r); break;
+  }
+  return float(v);
+}
+float xyz_sin(float a) {
+  double r; int q;
+  reduce_quadrant(a, &r, &q);
+  double v;
+  switch (q) {
+    case 0: v = taylor_sin(r); break;
+    case 1: v = taylor_cos(r); break;
+    case 2: v = -taylor_sin(r); break;
+    default: v = -taylor_cos(r); break;
+  }
+  return float(v);
+}
+
+// --------------------------------------------------------------- Philox4x32-10
+// Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3", SC'11.
+struct U4 { uint32_t v[4]; };
+U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+  const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+  for (int round = 0; round < 10; ++round) {
+    if (round > 0) { k0 += W0; k1 += W1; }
+    uint64_t p0 = uint64_t(M0) * c.v[0];
+    uint64_t p1 = uint64_t(M1) * c.v[2];
+    uint32_t hi0 = uint32_t(p0 >> 32), lo0 = uint32_t(p0);
+    uint32_t hi1 = uint32_t(p1 >> 32), lo1 = uint32_t(p1);
+    U4 n;
+    n.v[0] = hi1 ^ c.v[1] ^ k0;
+    n.v[1] = lo1;
+    n.v[2] = hi0 ^ c.v[3] ^ k1;
+    n.v[3] = lo0;
+    c = n;
+  }
+  return c;
+}
+inline real u01(uint32_t x) { return real(x >> 8) * real(1.0f / 16777216.0f); }
+struct Draw { real u[4]; };
+Draw draw(uint32_t pixel, uint64_t seed, uint32_t sample, uint32_t bounce) {
+  U4 c; c.v[0] = sample; c.v[1] = bounce; c.v[2] = uint32_t(seed >> 32); c.v[3] = 0;
+  U4 r = philox4x32_10(c, pixel, uint32_t(seed));
+  Draw d;
+  for (int i = 0; i < 4; ++i) d.u[i] = u01(r.v[i]);
+  return d;
+}
+
+// --------------------------------------------------------------- triangle
+// xyz/geom/triangle.hpp:93-119, xyz/geom/triangle.cpp:10-60
+struct Tri {
+  uint32_t bsdf;
+  V3 v0, e1, e2, ng, n0, s1, s2;
+  real uv0[2], uv1[2], uv2[2];
+  real lo(int k) const { return hmin(hmin(v0[k], v0[k] + e1[k]), v0[k] + e2[k]); }
+  real hi(int k) const { return hmax(hmax(v0[k], v0[k] + e1[k]), v0[k] + e2[k]); }
+  // xyz/geom/triangle.hpp:122-140 (accumulating form used for the root box)
+  void acc_min(V3& v) const {
+    for (int k = 0; k < 3; ++k)
+      v[k] = hmin(hmin(v[k], v0[k]), hmin(v0[k] + e1[k], v0[k] + e2[k]));
+  }
+  void acc_max(V3& v) const {
+    for (int k = 0; k < 3; ++k)
+      v[k] = hmax(hmax(v[k], v0[k]), hmax(v0[k] + e1[k], v0[k] + e2[k]));
+  }
+  // xyz/geom/triangle.cpp:75-85
+  int overlap(int axis, real value) const {
+    if (hi(axis) < value) return 1;
+    if (value <= lo(axis)) return 2;
+    return 3;
+  }
+  real area() const { return length(cross(e2, e1)) * real(0.5); }
+};
+
+Tri make_tri(uint32_t bsdf, V3 p0, V3 p1, V3 p2, V3 const* ns, real const* uv) {
+  Tri t;
+  t.bsdf = bsdf;
+  t.v0 = p0;
+  t.e1 = p1 - p0;
+  t.e2 = p2 - p0;
+  t.ng = normalize(cross(t.e2, t.e1));
+  if (ns) {
+    t.n0 = ns[0];
+    t.s1 = ns[1] - ns[0];
+    t.s2 = ns[2] - ns[0];
+  } else {
+    t.n0 = t.ng;
+    t.s1 = V3(0, 0, 0);
+    t.s2 = V3(0, 0, 0);
+  }
+  for (int i = 0; i < 2; ++i) {
+    t.uv0[i] = uv ? uv[i] : 0;
+    t.uv1[i] = uv ? uv[2 + i] : 0;
+    t.uv2[i] = uv ? uv[4 + i] : 0;
+  }
+  return t;
+}
+
+// xyz/stdafx.hpp:44-47 with libhi/hi/math/math.hpp:22-44 power helpers
+inline real fake_fresnel(real hk) {
+  real const fresnel0 = real(0.9);
+  real const x = 1 - hk;
+  real const x2 = x * x;
+  return fresnel0 + (1 - fresnel0) * (x * (x2 * x2));
+}
+
+struct Hit {
+  real t_min, t_max, f, g;
+  bool back;
+  void init() { t_min = real(1e-5); t_max = std::numeric_limits<real>::infinity(); }
+};
+
+// xyz/geom/triangle.cpp:127-165
+bool intersect_df(Tri const& tr, V3 const& o, V3 const& d, Hit* p) {
+  V3 const n = cross(tr.e2, tr.e1);
+  real const det = dot(d, n);
+  if (std::fabs(det) < EPSILON) return false;
+  real const inv = rcp(det);
+  V3 const vmo = tr.v0 - o;
+  real const t = inv * dot(vmo, n);
+  if ((t < p->t_min) || (p->t_max < t)) return false;
+  V3 const q = cross(d, vmo);
+  real const f = inv * dot(tr.e2, q);
+  if ((f < 0) || (f > 1)) return false;
+  real const g = inv * -dot(tr.e1, q);
+  if ((g < 0) || ((f + g) > 1)) return false;
+  p->t_max = t;
+  p->f = f;
+  p->g = g;
+  p->back = det > 0;
+  return true;
+}
+
+// --------------------------------------------------------------- kd-tree
+// Pointer-free node pool; children[0] = [min,value), children[1] = [value,max).
+struct Node {
+  int kind;          // 0,1,2 = interior on that axis; 3 = leaf; 4 = null leaf
+  real value;
+  int child[2];
+  std::vector<uint32_t> set;  // leaf triangle indices in stored order
+};
+
+struct Tree {
+  std::vector<Node> nodes;
+  int root = -1;
+  V3 bmin, bmax;
+  size_t max_bucket = 16, max_depth = 16;
+  std::vector<Tri> const* tris = nullptr;
+};
+
+struct Event {
+  bool type;  // false: open, true: close
+  real value;
+  bool operator<(Event const& e) const {
+    return (value < e.value) ? true : (value > e.value) ? false : type < e.type;
+  }
+};
+
+// xyz/geom/kdtree.cpp:226-357
+int build_rec(Tree& tr, std::vector<uint32_t> const& set, V3 const& mn, V3 const& mx, size_t depth) {
+  std::vector<Tri> const& T = *tr.tris;
+  if ((set.size() <= tr.max_bucket) || (depth > tr.max_depth)) {
+    Node nd;
+    nd.kind = set.empty() ? 4 : 3;
+    nd.value = 0;
+    nd.child[0] = nd.child[1] = -1;
+    nd.set = set;
+    tr.nodes.push_back(std::move(nd));
+    return int(tr.nodes.size()) - 1;
+  }
+  depth++;
+  V3 const range = mx - mn;
+  size_t axis = ~size_t(0);
+  real value = std::numeric_limits<real>::signaling_NaN();
+  real min_cost = std::numeric_limits<real>::max();
+  std::vector<Event> ev(set.size() * 2);
+  for (size_t k = 0; k < 3; ++k) {
+    size_t const size = set.size();
+    for (size_t i = 0; i < size; ++i) {
+      ev[i].type = false;
+      ev[i].value = T[set[i]].lo(int(k));
+      ev[i + size].type = true;
+      ev[i + size].value = T[set[i]].hi(int(k)) + EPSILON;
+    }
+    std::sort(ev.begin(), ev.end());
+    real const a = range[int((k + 1) % 3)] * range[int((k + 2) % 3)];
+    real const b = range[int((k + 1) % 3)] + range[int((k + 2) % 3)];
+    size_t n_left = 0, n_right = set.size();
+    auto update = [&](real v) {
+      real const r = v - mn[int(k)];
+      real const sa_left = a + b * r;
+      real const sa_right = a + b * (range[int(k)] - r);
+      real const cost = real(n_left) * sa_left + real(n_right) * sa_right;
+      if (min_cost > cost) {
+        min_cost = cost;
+        axis = k;
+        value = v;
+      }
+    };
+    update(ev[0].value - EPSILON);
+    for (size_t i = 0, n = ev.size(); i < n; ++i) {
+      if (ev[i].type) n_right--; else n_left++;
+      real v = ev[i].value;
+      if ((i + 1) < n) {
+        if (v == ev[i + 1].value) continue;
+      } else {
+        v += EPSILON;
+      }
+      update(v);
+    }
+  }
+  std::vector<uint32_t> sl, sr;
+  for (uint32_t id : set) {
+    switch (T[id].overlap(int(axis), value)) {
+      case 1: sl.push_back(id); break;
+      case 2: sr.push_back(id); break;
+      default: sl.push_back(id); sr.push_back(id); break;
+    }
+  }
+  int const self = int(tr.nodes.size());
+  {
+    Node nd;
+    nd.kind = int(axis);
+    nd.value = value;
+    nd.child[0] = nd.child[1] = -1;
+    tr.nodes.push_back(std::move(nd));
+  }
+  V3 max_left(mx); max_left[int(axis)] = value;
+  int const l = build_rec(tr, sl, mn, max_left, depth);
+  tr.nodes[self].child[0] = l;
+  V3 min_right(mn); min_right[int(axis)] = value;
+  int const r = build_rec(tr, sr, min_right, mx, depth);
+  tr.nodes[self].child[1] = r;
+  return self;
+}
+
+// xyz/geom/kdtree.cpp:207-224
+void build_tree(Tree& tr) {
+  std::vector<Tri> const& T = *tr.tris;
+  V3 mn(std::numeric_limits<real>::infinity(), std::numeric_limits<real>::infinity(),
+        std::numeric_limits<real>::infinity());
+  V3 mx(-std::numeric_limits<real>::infinity(), -std::numeric_limits<real>::infinity(),
+        -std::numeric_limits<real>::infinity());
+  for (Tri const& t : T) { t.acc_min(mn); t.acc_max(mx); }
+  mx = mx + (mx - mn) * real(1e-5);
+  tr.bmin = mn; tr.bmax = mx;
+  tr.nodes.clear();
+  std::vector<uint32_t> all(T.size());
+  for (size_t i = 0; i < T.size(); ++i) all[i] = uint32_t(i);
+  tr.root = build_rec(tr, all, mn, mx, 0);
+}
+
+struct Counters { uint32_t nodes = 0, tris = 0; };
+
+// xyz/geom/kdtree.cpp:82-122 and 140-177 (recursive, as the reference)
+int find_rec(Tree const& tr, int ni, V3 const& o, V3 const& d, Hit* p, Counters* c) {
+  Node const& nd = tr.nodes[size_t(ni)];
+  c->nodes++;
+  if (nd.kind == 4) return -1;
+  if (nd.kind == 3) {
+    int s = -1;
+    for (uint32_t id : nd.set) {
+      c->tris++;
+      if (intersect_df((*tr.tris)[id], o, d, p)) s = int(id);
+    }
+    return s;
+  }
+  int const axis = nd.kind;
+  real const e = (nd.value - o[axis]) / d[axis];
+  int const i = o[axis] < nd.value;
+  int const nearc = nd.child[1 - i], farc = nd.child[i];
+  if ((e > p->t_max) || (e < 0)) {
+    return find_rec(tr, nearc, o, d, p, c);
+  } else if (e < p->t_min) {
+    return find_rec(tr, farc, o, d, p, c);
+  } else {
+    real const tmax = p->t_max;
+    p->t_max = e;
+    int const s = find_rec(tr, nearc, o, d, p, c);
+    if (s >= 0) return s;
+    p->t_min = e;
+    p->t_max = tmax;
+    return find_rec(tr, farc, o, d, p, c);
+  }
+}
+
+// xyz/geom/kdtree.cpp:366-372
+int find_intersection(Tree const& tr, V3 const& o, V3 const& d, Hit* p, Counters* c) {
+  p->init();
+  if (tr.root < 0) return -1;
+  return find_rec(tr, tr.root, o, d, p, c);
+}
+
+// xyz/core/scene.hpp:283-293
+int brute_force(std::vector<Tri> const& T, V3 const& o, V3 const& d, Hit* p) {
+  p->init();
+  int s = -1;
+  for (size_t i = 0; i < T.size(); ++i)
+    if (intersect_df(T[i], o, d, p)) s = int(i);
+  return s;
+}
+
+// --------------------------------------------------------------- scene
+enum { LAMBERT = 0, MIRROR = 1, GLASS = 2, METAL = 3, LIGHT = 4 };  // xyz/bsdf/bsdf.hpp:21-28
+
+struct Bsdf {
+  uint32_t type;
+  V3 rgb;         // albedo (Lambert/Metal) or L_e (Light)
+  real shininess;
+  real ior;
+  int32_t tex;
+  V3 phi;         // light power Phi = L_e * pi * A_total (Light only)
+};
+
+struct Texture { uint32_t w, h; std::vector<real> texels; };
+
+struct Scene {
+  std::vector<Tri> tris;
+  std::vector<Bsdf> bsdfs;
+  std::vector<Texture> textures;
+  std::vector<uint32_t> lights;   // triangle ids in caller order
+  std::vector<real> cdf;          // cdf[0] = 0; cdf[i+1] = cdf[i] + area
+  Tree tree;
+  bool use_tree = true;
+  real light_area() const { return cdf.back(); }
+};
+
+struct Vertex {
+  int tri = -1;
+  bool back = false, specular = false;
+  V3 pos, dir, ng, ns, t, b;
+  V3 power;
+  real f_next = 0;
+};
+
+int scene_find(Scene const& sc, V3 const& o, V3 const& d, Hit* p, Counters* c) {
+  if (sc.use_tree) return find_intersection(sc.tree, o, d, p, c);
+  return brute_force(sc.tris, o, d, p);
+}
+
+// xyz/geom/triangle.cpp:194-219
+void shading_basis(Tri const& tr, Hit const& h, Vertex* v) {
+  V3 ns = normalize((tr.n0 + h.f * tr.s1) + h.g * tr.s2);
+  V3 ng = tr.ng;
+  if (h.back) { ng = -ng; ns = -ns; }
+  V3 t(-ns[2], 0, ns[0]);
+  real const l2 = length_squared(t);
+  V3 b;
+  if (l2 > 0) {
+    t = t * rsqrt(l2);
+    b = cross(t, ns);
+  } else {
+    b = V3(0, 0, (ns[1] > 0) ? 1 : -1);
+    t = cross(ns, b);
+  }
+  v->ns = ns; v->ng = ng; v->t = t; v->b = b;
+}
+
+// xyz/geom/triangle.cpp:232-263 + xyz/core/scene.cpp:42-78
+void sample_light(Scene const& sc, real s, real t, Vertex* lv, int* light_index) {
+  if (sc.lights.empty()) { lv->tri = -1; *light_index = -1; return; }
+  real const u = s * sc.cdf.back();
+  size_t it = size_t(std::upper_bound(sc.cdf.begin(), sc.cdf.end(), u) - sc.cdf.begin());
+  size_t n = it - 1;
+  if (n >= sc.lights.size()) n = sc.lights.size() - 1;  // clamp (reference: UB)
+  real const v = (u - sc.cdf[n]) / (sc.cdf[n + 1] - sc.cdf[n]);
+  Tri const& tr = sc.tris[sc.lights[n]];
+  real const ss = std::sqrt(v);
+  real const tt = t;
+  real const uu = ss * (1 - tt);
+  real const vv = ss * tt;
+  lv->tri = int(sc.lights[n]);
+  lv->pos = (tr.v0 + uu * tr.e1) + vv * tr.e2;
+  lv->ns = normalize((tr.n0 + uu * tr.s1) + vv * tr.s2);
+  lv->ng = tr.ng;
+  lv->f_next = rcp(sc.light_area());
+  lv->specular = false;
+  *light_index = int(n);
+}
+
+struct Camera {
+  V3 eye, back, right, up;
+  real film, aspect, v;  // film size F, aspect A, lens-film distance V
+};
+
+// xyz/core/thinlenscamera.cpp:71-90 with an explicit vertical field of view
+Camera make_camera(float const* cam, uint32_t W, uint32_t H) {
+  Camera c;
+  V3 eye(cam[0], cam[1], cam[2]), at(cam[3], cam[4], cam[5]), up(cam[6], cam[7], cam[8]);
+  real const fov = cam[9];
+  c.eye = eye;
+  c.film = real(0.025);
+  c.aspect = static_cast<real>(int(W)) / int(H);
+  c.v = real(double(c.film) * 0.5 / std::tan(double(fov) * 3.14159265358979323846 / 360.0));
+  V3 const view = eye - at;
+  c.back = normalize(view);
+  c.right = normalize(cross(up, c.back));
+  c.up = cross(c.back, c.right);
+  return c;
+}
+
+// xyz/core/thinlenscamera.cpp:111-124
+V3 primary_dir(Camera const& c, real s, real t) {
+  real const lx = real((double(s) - 0.5) * double(c.film) * double(c.aspect));
+  real const ly = real((double(t) - 0.5) * double(c.film));
+  real const lz = -c.v;
+  return normalize((lx * c.right + ly * c.up) + lz * c.back);
+}
+
+V3 albedo_at(Scene const& sc, Bsdf const& b, Tri const& tr, Hit const& h, int* texel) {
+  *texel = -1;
+  if (b.tex < 0) return b.rgb;
+  Texture const& tx = sc.textures[size_t(b.tex)];
+  // uv = uv0 + f (uv1-uv0) + g (uv2-uv0), same barycentrics as triangle.hpp:194-199
+  real u = (tr.uv0[0] + h.f * (tr.uv1[0] - tr.uv0[0])) + h.g * (tr.uv2[0] - tr.uv0[0]);
+  real v = (tr.uv0[1] + h.f * (tr.uv1[1] - tr.uv0[1])) + h.g * (tr.uv2[1] - tr.uv0[1]);
+  int ix = int(u * real(tx.w));
+  int iy = int(v * real(tx.h));
+  ix = ix < 0 ? 0 : (ix >= int(tx.w) ? int(tx.w) - 1 : ix);
+  iy = iy < 0 ? 0 : (iy >= int(tx.h) ? int(tx.h) - 1 : iy);
+  int const idx = iy * int(tx.w) + ix;
+  *texel = idx;
+  return V3(tx.texels[size_t(idx) * 3 + 0], tx.texels[size_t(idx) * 3 + 1], tx.texels[size_t(idx) * 3 + 2]);
+}
+
+struct PTParams {
+  uint32_t kmax;        // reference XYZ_CONFIG_kMaxRandomWalkDepth = max depth D + 1
+  uint32_t rr_start;    // RR applies at vertices j > rr_start (0 = reference)
+  real rr_threshold;    // CONFIG_TERMINATE_THRESHOLD = 0.2
+  uint64_t seed;
+};
+
+// Per-vertex record used by the adjoint replay.
+struct VertexRecord {
+  int texture;  // -1 if albedo is constant
+  int texel;
+  V3 albedo;
+  V3 prefix;    // color accumulated before this vertex's albedo was applied
+};
+
+struct PathStats { uint32_t queries = 0; Counters c; };
+
+// xyz/core/scene_pathtracing.cpp:11-144.  Returns the sample's colour.
+// Optional: records Lambert vertices and per-light emission contributions.
+V3 pt_path(Scene const& sc, Camera const& cam, PTParams const& pp, uint32_t W, uint32_t H,
+           uint32_t px, uint32_t py, uint32_t sample, PathStats* st,
+           std::vector<VertexRecord>* rec, std::vector<V3>* light_contrib) {
+  uint32_t const pixel = py * W + px;
+  Draw d0 = draw(pixel, pp.seed, sample, 0);
+  // xyz/integrator/integrator.cpp:124-125
+  real const sx = (real(int(px)) + d0.u[0]) / real(int(W));
+  real const sy = (real(int(py)) + d0.u[1]) / real(int(H));
+
+  Vertex lv;
+  int light_n = -1;
+  sample_light(sc, d0.u[2], d0.u[3], &lv, &light_n);
+  if (light_n >= 0) lv.power = sc.bsdfs[sc.tris[size_t(lv.tri)].bsdf].phi;
+
+  Vertex ev;
+  ev.pos = cam.eye;
+  ev.power = V3(1, 1, 1);
+  ev.f_next = 0;
+  ev.dir = primary_dir(cam, sx, sy);
+  ev.specular = true;
+
+  V3 color(0, 0, 0);
+  real const A = sc.cdf.back();
+  for (uint32_t k = 1;;) {
+    Hit h;
+    if (st) st->queries++;
+    Counters c;
+    ev.tri = scene_find(sc, ev.pos, ev.dir, &h, &c);
+    if (st) { st->c.nodes += c.nodes; st->c.tris += c.tris; }
+    if (ev.tri < 0) break;
+    Tri const& tr = sc.tris[size_t(ev.tri)];
+    shading_basis(tr, h, &ev);
+    ev.back = h.back;
+    if (-dot(ev.dir, ev.ns) <= 0.0) break;
+    ev.pos = ev.pos + ev.dir * h.t_max;
+
+    Bsdf const& b = sc.bsdfs[tr.bsdf];
+    if ((b.type == LIGHT) && !ev.back) {
+      V3 const lp = b.phi;
+      V3 add;
+      if (ev.specular) {
+        double const w = M_1_PI / double(A);
+        V3 const pe = lp * ev.power;
+        add = V3(real(double(pe[0]) * w), real(double(pe[1]) * w), real(double(pe[2]) * w));
+      } else {
+        real const cos_t = std::fabs(dot(ev.dir, ev.ns));
+        real const dist2 = h.t_max * h.t_max;
+        real const p_imp = ev.f_next * cos_t / dist2;
+        real const p_exp = rcp(A);
+        real const dens = p_exp / p_imp;
+        double const w1 = M_1_PI / double(A);
+        double const w2 = double(rcp(1 + dens * dens));
+        V3 const pe = lp * ev.power;
+        add = V3(real(double(pe[0]) * w1 * w2), real(double(pe[1]) * w1 * w2),
+                 real(double(pe[2]) * w1 * w2));
+      }
+      if (light_contrib) {
+        // emission of the light that was hit
+        (*light_contrib)[size_t(tr.bsdf)] = (*light_contrib)[size_t(tr.bsdf)] + add;
+      }
+      color = color + add;
+      break;
+    }
+    if (++k >= pp.kmax) break;
+
+    uint32_t const j = k - 1;  // vertex index, 1-based
+    Draw dj = draw(pixel, pp.seed, sample, j);
+
+    // ---- NEE: Bsdf::CalculateWeightedDirectLighting
+    if (b.type == LAMBERT || b.type == METAL) {
+      int texel = -1;
+      V3 const alb = albedo_at(sc, b, tr, h, &texel);
+      if (rec) {
+        VertexRecord r;
+        r.texture = (b.type == LAMBERT) ? b.tex : -1;
+        r.texel = texel;
+        r.albedo = alb;
+        r.prefix = color;
+        rec->push_back(r);
+      }
+      ev.power = ev.power * alb;
+      if (lv.tri >= 0) {
+        V3 const w = normalize(ev.pos - lv.pos);
+        real const nk3 = dot(w, lv.ng);
+        bool ok = !((nk3 <= 0) || (dot(w, lv.ns) <= 0));
+        real nk2 = 0;
+        if (ok) {
+          nk2 = -dot(w, ev.ns);
+          ok = !((nk2 <= 0) || (-dot(w, ev.ng) <= 0));
+        }
+        if (ok) {
+          Hit sh;
+          if (st) st->queries++;
+          Counters c2;
+          int const hit = scene_find(sc, lv.pos, w, &sh, &c2);
+          if (st) { st->c.nodes += c2.nodes; st->c.tris += c2.tris; }
+          if (hit == ev.tri) {
+            V3 add;
+            if (b.type == LAMBERT) {
+              // xyz/bsdf/lambert.cpp:157-172
+              real const G = nk2 * nk3 / (sh.t_max * sh.t_max);
+              real const p_imp = real(M_1_PI * double(G));
+              real const p_exp = rcp(A);
+              real const dens = p_imp / p_exp;
+              double const wgt = (double(G) * (M_1_PI * M_1_PI)) * double(rcp(1 + dens * dens));
+              V3 const pe = lv.power * ev.power;
+              add = V3(real(double(pe[0]) * wgt), real(double(pe[1]) * wgt), real(double(pe[2]) * wgt));
+            } else {
+              // xyz/bsdf/metal.cpp:100-116
+              real const wgt = nk3 / (sh.t_max * sh.t_max);
+              real const G = nk2 * wgt;
+              V3 const hv = -normalize(w + ev.dir);
+              real const hk = -dot(hv, w);
+              real const nh = dot(ev.ns, hv);
+              real const nk1 = -dot(ev.ns, ev.dir);
+              real const fsn = real(double(b.shininess + 1) * (M_1_PI / 8) *
+                                    double(std::pow(nh, b.shininess)) / double(hk));
+              real const dens = fsn * wgt * A;
+              real const fres = fake_fresnel(hk);
+              V3 const pe = lv.power * ev.power;
+              double const s1 = double(G) * M_1_PI;
+              double const mx = double(hmax(nk1, nk2));
+              double const rw = double(rcp(1 + dens * dens));
+              for (int ch = 0; ch < 3; ++ch)
+                add[ch] = real(((((double(pe[ch]) * s1) * double(fsn)) / mx) * double(fres)) * rw);
+            }
+            if (light_contrib && light_n >= 0) {
+              uint32_t const lb = sc.tris[size_t(lv.tri)].bsdf;
+              (*light_contrib)[lb] = (*light_contrib)[lb] + add;
+            }
+            color = color + add;
+          }
+        }
+      }
+    }
+
+    // ---- Russian roulette: xyz/core/scene_pathtracing.cpp:128-137
+    if (j > pp.rr_start) {
+      real const t = ev.power[1];
+      if (t < pp.rr_threshold) {
+        real const q = t / pp.rr_threshold;
+        if (dj.u[0] >= q) break;
+        real const inv = rcp(q);
+        ev.power = ev.power * inv;
+      }
+    }
+
+    // ---- Bsdf::NextScatteringDirection
+    if (b.type == LAMBERT) {
+      // xyz/bsdf/lambert.cpp:176-187, xyz/stdafx.hpp:50-63
+      real const phi = real(2 * M_PI) * dj.u[2];
+      real const cos_t = std::sqrt(dj.u[1]);
+      real const sin_t = std::sqrt(1 - dj.u[1]);
+      ev.dir = (ev.t * (sin_t * xyz_cos(phi)) + ev.b * (sin_t * xyz_sin(phi))) + ev.ns * cos_t;
+      ev.specular = false;
+      ev.f_next = real(M_1_PI * double(cos_t));
+      if (!(dot(ev.dir, ev.ng) > 0)) break;
+    } else if (b.type == MIRROR) {
+      // xyz/bsdf/mirror.cpp:56-105
+      real const c = -dot(ev.dir, ev.ns);
+      ev.dir = ev.dir + (real(2) * c) * ev.ns;
+      if (dot(ev.dir, ev.ng) <= 0) break;
+      ev.specular = true;
+      ev.f_next = 1;
+    } else if (b.type == GLASS) {
+      // xyz/bsdf/glass.cpp:104-165
+      real const ior = b.ior;
+      real const r = ev.back ? rcp(ior) : ior;
+      real const ior_sq = r * r;
+      real const nk = -dot(ev.dir, ev.ns);
+      real const bb = ior_sq + nk * nk - 1;
+      if (bb > 0) {
+        real const g = std::sqrt(bb);
+        real const gpc = g + nk;
+        real const gmc = g - nk;
+        real const alpha = gmc / gpc;
+        real const beta = (nk * gpc - 1) / (nk * gmc + 1);
+        real const fr = real(0.5) * alpha * alpha * (beta * beta + 1);
+        if (dj.u[1] < fr) {
+          ev.dir = ev.dir + (real(2) * nk) * ev.ns;
+          if (dot(ev.dir, ev.ng) <= 0) break;
+        } else {
+          ev.dir = (ev.dir - gmc * ev.ns) / std::sqrt(ior_sq);
+          if (dot(ev.dir, ev.ng) >= 0) break;
+        }
+      } else {
+        ev.dir = ev.dir + (real(2) * nk) * ev.ns;
+        if (dot(ev.dir, ev.ng) <= 0) break;
+      }
+      ev.specular = true;
+      ev.f_next = 1;
+    } else if (b.type == METAL) {
+      // xyz/bsdf/metal.cpp:122-153, xyz/stdafx.hpp:154-168
+      real const phi = real(2 * M_PI) * dj.u[2];
+      real const cos_h = std::pow(dj.u[1], rcp(b.shininess));
+      real const sin_h = std::sqrt(real(1) - cos_h * cos_h);
+      V3 const hv = (ev.t * (sin_h * xyz_cos(phi)) + ev.b * (sin_h * xyz_sin(phi))) + ev.ns * cos_h;
+      real const nk1 = -dot(ev.ns, ev.dir);
+      real const hk = -dot(hv, ev.dir);
+      ev.dir = ev.dir + (2 * hk) * hv;
+      real const nk2 = dot(ev.ns, ev.dir);
+      if ((nk2 <= 0) || (dot(ev.ng, ev.dir) <= 0)) break;
+      real const fres = fake_fresnel(hk);
+      real const sc1 = fres / hmax(nk1, nk2) * nk2;
+      ev.power = ev.power * sc1;
+      ev.specular = false;
+      ev.f_next = real(double(b.shininess + 1) * (M_1_PI / 8) * double(std::pow(cos_h, b.shininess)) / double(hk));
+    } else {
+      break;  // Light seen from behind: xyz/bsdf/light.cpp:27-31
+    }
+  }
+  return color;
+}
+
+int n_threads(int req) {
+#ifdef _OPENMP
+  return req > 0 ? req : omp_get_max_threads();
+#else
+  (void)req;
+  return 1;
+#endif
+}
+
+}  // namespace
+
+// ======================================================================= C ABI
+extern "C" {
+
+// Scene with triangles in caller order (caller order defines primitive ids).
+void* orc_scene_create(uint32_t n_tris, const float* v0, const float* v1, const float* v2,
+                       const float* n0, const float* n1, const float* n2, const float* uv,
+                       const uint32_t* tri_bsdf, uint32_t n_bsdfs, const uint32_t* bsdf_type,
+                       const float* bsdf_rgb, const float* bsdf_shininess, const float* bsdf_ior,
+                       const int32_t* bsdf_tex, uint32_t n_tex, const uint32_t* tex_wh,
+                       const float* texels, uint32_t max_bucket, uint32_t max_depth, int use_tree) {
+  Scene* sc = new Scene();
+  sc->tris.reserve(n_tris);
+  for (uint32_t i = 0; i < n_tris; ++i) {
+    V3 p0(v0[3 * i], v0[3 * i + 1], v0[3 * i + 2]);
+    V3 p1(v1[3 * i], v1[3 * i + 1], v1[3 * i + 2]);
+    V3 p2(v2[3 * i], v2[3 * i + 1], v2[3 * i + 2]);
+    V3 ns[3];
+    if (n0) {
+      ns[0] = V3(n0[3 * i], n0[3 * i + 1], n0[3 * i + 2]);
+      ns[1] = V3(n1[3 * i], n1[3 * i + 1], n1[3 * i + 2]);
+      ns[2] = V3(n2[3 * i], n2[3 * i + 1], n2[3 * i + 2]);
+    }
+    sc->tris.push_back(make_tri(tri_bsdf ? tri_bsdf[i] : 0, p0, p1, p2, n0 ? ns : nullptr,
+                                uv ? uv + 6 * size_t(i) : nullptr));
+  }
+  sc->bsdfs.resize(n_bsdfs);
+  for (uint32_t b = 0; b < n_bsdfs; ++b) {
+    Bsdf& bs = sc->bsdfs[b];
+    bs.type = bsdf_type[b];
+    bs.rgb = V3(bsdf_rgb[3 * b], bsdf_rgb[3 * b + 1], bsdf_rgb[3 * b + 2]);
+    bs.shininess = bsdf_shininess ? bsdf_shininess[b] : real(0);
+    bs.ior = bsdf_ior ? bsdf_ior[b] : real(1.5);
+    bs.tex = bsdf_tex ? bsdf_tex[b] : -1;
+  }
+  size_t off = 0;
+  for (uint32_t t = 0; t < n_tex; ++t) {
+    Texture tx;
+    tx.w = tex_wh[2 * t];
+    tx.h = tex_wh[2 * t + 1];
+    tx.texels.assign(texels + off, texels + off + size_t(tx.w) * tx.h * 3);
+    off += size_t(tx.w) * tx.h * 3;
+    sc->textures.push_back(std::move(tx));
+  }
+  // light list and area CDF in caller order: xyz/core/scene_mqo.cpp:509-518
+  sc->cdf.push_back(0);
+  for (uint32_t i = 0; i < n_tris; ++i) {
+    if (sc->bsdfs[sc->tris[i].bsdf].type == LIGHT) {
+      sc->lights.push_back(i);
+      sc->cdf.push_back(sc->cdf.back() + sc->tris[i].area());
+    }
+  }
+  real const A = sc->cdf.back();
+  for (Bsdf& bs : sc->bsdfs) {
+    if (bs.type == LIGHT) {
+      double const s = 3.14159265358979323846 * double(A);
+      bs.phi = V3(real(double(bs.rgb[0]) * s), real(double(bs.rgb[1]) * s), real(double(bs.rgb[2]) * s));
+    }
+  }
+  sc->use_tree = use_tree != 0;
+  sc->tree.tris = &sc->tris;
+  sc->tree.max_bucket = max_bucket;
+  sc->tree.max_depth = max_depth;
+  if (sc->use_tree) build_tree(sc->tree);
+  return sc;
+}
+
+void orc_scene_destroy(void* s) { delete static_cast<Scene*>(s); }
+
+float orc_light_area(void* s) { return static_cast<Scene*>(s)->light_area(); }
+
+// Tree structure: counts[0] = nodes, counts[1] = leaves (incl. null), counts[2] = refs,
+// counts[3] = max depth reached.  Bounds: 6 floats (min, upper bound).
+void orc_tree_counts(void* s, uint64_t* counts, float* bounds) {
+  Tree const& tr = static_cast<Scene*>(s)->tree;
+  uint64_t leaves = 0, refs = 0;
+  for (Node const& n : tr.nodes) {
+    if (n.kind >= 3) { leaves++; refs += n.set.size(); }
+  }
+  counts[0] = tr.nodes.size();
+  counts[1] = leaves;
+  counts[2] = refs;
+  if (bounds) {
+    for (int k = 0; k < 3; ++k) { bounds[k] = tr.bmin[k]; bounds[3 + k] = tr.bmax[k]; }
+  }
+}
+
+// Pre-order dump: kind[i] (0..2 axis, 3 leaf, 4 null leaf), split[i], and for
+// leaves the range [leaf_off[i], leaf_off[i]+leaf_cnt[i]) into refs.
+static void dump_rec(Tree const& tr, int ni, uint32_t* kind, float* split, uint32_t* loff,
+                     uint32_t* lcnt, uint32_t* refs, uint64_t* pos, uint64_t* rpos) {
+  Node const& n = tr.nodes[size_t(ni)];
+  uint64_t const i = (*pos)++;
+  kind[i] = uint32_t(n.kind);
+  split[i] = n.value;
+  loff[i] = 0;
+  lcnt[i] = 0;
+  if (n.kind >= 3) {
+    loff[i] = uint32_t(*rpos);
+    lcnt[i] = uint32_t(n.set.size());
+    for (uint32_t id : n.set) refs[(*rpos)++] = id;
+    return;
+  }
+  dump_rec(tr, n.child[0], kind, split, loff, lcnt, refs, pos, rpos);
+  dump_rec(tr, n.child[1], kind, split, loff, lcnt, refs, pos, rpos);
+}
+
+void orc_tree_dump(void* s, uint32_t* kind, float* split, uint32_t* leaf_off, uint32_t* leaf_cnt,
+                   uint32_t* refs) {
+  Tree const& tr = static_cast<Scene*>(s)->tree;
+  uint64_t pos = 0, rpos = 0;
+  if (tr.root >= 0) dump_rec(tr, tr.root, kind, split, leaf_off, leaf_cnt, refs, &pos, &rpos);
+}
+
+// Closest hit for n rays (AoS float3 origin/direction).  prim = -1 on a miss,
+// t = +inf on a miss.  Optional outputs may be NULL.  Returns wall seconds.
+double orc_closest_hit(void* s, uint64_t n, const float* org, const float* dir, int32_t* prim,
+                       float* t, float* f, float* g, uint8_t* back, uint32_t* cnt_nodes,
+                       uint32_t* cnt_tris, int threads) {
+  Scene const& sc = *static_cast<Scene*>(s);
+  auto t0 = std::chrono::steady_clock::now();
+  int const nt = n_threads(threads);
+#pragma omp parallel for schedule(dynamic, 1024) num_threads(nt)
+  for (int64_t i = 0; i < int64_t(n); ++i) {
+    V3 o(org[3 * i], org[3 * i + 1], org[3 * i + 2]);
+    V3 d(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]);
+    Hit h;
+    Counters c;
+    int const r = scene_find(sc, o, d, &h, &c);
+    if (prim) prim[i] = r;
+    if (t) t[i] = r >= 0 ? h.t_max : std::numeric_limits<float>::infinity();
+    if (f) f[i] = r >= 0 ? h.f : 0.0f;
+    if (g) g[i] = r >= 0 ? h.g : 0.0f;
+    if (back) back[i] = r >= 0 ? uint8_t(h.back) : 0;
+    if (cnt_nodes) cnt_nodes[i] = c.nodes;
+    if (cnt_tris) cnt_tris[i] = c.tris;
+  }
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// Brute-force closest hit (xyz/core/scene.hpp:283-293), regardless of use_tree.
+void orc_brute_force(void* s, uint64_t n, const float* org, const float* dir, int32_t* prim,
+                     float* t, int threads) {
+  Scene const& sc = *static_cast<Scene*>(s);
+  int const nt = n_threads(threads);
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nt)
+  for (int64_t i = 0; i < int64_t(n); ++i) {
+    V3 o(org[3 * i], org[3 * i + 1], org[3 * i + 2]);
+    V3 d(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]);
+    Hit h;
+    int const r = brute_force(sc.tris, o, d, &h);
+    prim[i] = r;
+    t[i] = r >= 0 ? h.t_max : std::numeric_limits<float>::infinity();
+  }
+}
+
+// Single-triangle test (xyz/geom/triangle.cpp:127-165) with a caller interval.
+int orc_intersect_triangle(const float* v0, const float* v1, const float* v2, const float* o,
+                           const float* d, float t_min, float t_max, float* out4) {
+  Tri tr = make_tri(0, V3(v0[0], v0[1], v0[2]), V3(v1[0], v1[1], v1[2]), V3(v2[0], v2[1], v2[2]),
+                    nullptr, nullptr);
+  Hit h;
+  h.t_min = t_min;
+  h.t_max = t_max;
+  h.f = h.g = 0;
+  h.back = false;
+  bool const ok = intersect_df(tr, V3(o[0], o[1], o[2]), V3(d[0], d[1], d[2]), &h);
+  out4[0] = h.t_max; out4[1] = h.f; out4[2] = h.g; out4[3] = h.back ? 1.0f : 0.0f;
+  return ok ? 1 : 0;
+}
+
+void orc_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
+  U4 c;
+  for (int i = 0; i < 4; ++i) c.v[i] = ctr[i];
+  U4 r = philox4x32_10(c, key[0], key[1]);
+  for (int i = 0; i < 4; ++i) out[i] = r.v[i];
+}
+
+void orc_cos_sin(uint64_t n, const float* x, float* c, float* s) {
+  for (uint64_t i = 0; i < n; ++i) { c[i] = xyz_cos(x[i]); s[i] = xyz_sin(x[i]); }
+}
+
+// Shading basis (xyz/geom/triangle.cpp:194-219) of triangle `tri` at (f, g, back).
+void orc_shading_basis(void* s, uint32_t tri, float f, float g, int back, float* out12) {
+  Scene const& sc = *static_cast<Scene*>(s);
+  Hit h; h.f = f; h.g = g; h.back = back != 0;
+  Vertex v;
+  shading_basis(sc.tris[tri], h, &v);
+  for (int k = 0; k < 3; ++k) {
+    out12[k] = v.ns[k]; out12[3 + k] = v.ng[k]; out12[6 + k] = v.t[k]; out12[9 + k] = v.b[k];
+  }
+}
+
+// Light sample (xyz/core/scene.cpp:42-78).  out: pos3, ns3, ng3, light tri.
+int orc_sample_light(void* s, float u, float v, float* out9) {
+  Scene const& sc = *static_cast<Scene*>(s);
+  Vertex lv; int n = -1;
+  sample_light(sc, u, v, &lv, &n);
+  if (lv.tri < 0) return -1;
+  for (int k = 0; k < 3; ++k) { out9[k] = lv.pos[k]; out9[3 + k] = lv.ns[k]; out9[6 + k] = lv.ng[k]; }
+  return lv.tri;
+}
+
+void orc_primary_dir(const float* cam, uint32_t W, uint32_t H, float s, float t, float* out3) {
+  Camera c = make_camera(cam, W, H);
+  V3 d = primary_dir(c, s, t);
+  for (int k = 0; k < 3; ++k) out3[k] = d[k];
+}
+
+// Progressive path tracing (xyz/integrator/integrator.cpp:119-128) over rows
+// [y0, y1) for samples [sample0, sample0 + spp).  film (W*H*3) receives the
+// per-pixel SUM of sample colours, added sample pass by sample pass.
+// stats[0] = closest-hit queries, stats[1] = nodes visited, stats[2] = tris tested.
+// Returns wall seconds.
+double orc_render(void* s, const float* cam, uint32_t W, uint32_t H, uint32_t spp,
+                  uint32_t sample0, uint32_t max_depth, uint32_t rr_start, float rr_threshold,
+                  uint64_t seed, uint32_t y0, uint32_t y1, float* film, uint64_t* stats,
+                  int threads) {
+  Scene const& sc = *static_cast<Scene*>(s);
+  Camera const c = make_camera(cam, W, H);
+  PTParams pp;
+  pp.kmax = max_depth + 1;
+  pp.rr_start = rr_start;
+  pp.rr_threshold = rr_threshold;
+  pp.seed = seed;
+  int const nt = n_threads(threads);
+  std::atomic<uint64_t> q(0), nn(0), ntr(0);
+  auto t0 = std::chrono::steady_clock::now();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
+  for (int64_t y = int64_t(y0); y < int64_t(y1); ++y) {
+    PathStats ps;
+    for (uint32_t x = 0; x < W; ++x) {
+      float* px = film + (size_t(y) * W + x) * 3;
+      for (uint32_t smp = sample0; smp < sample0 + spp; ++smp) {
+        V3 col = pt_path(sc, c, pp, W, H, x, uint32_t(y), smp, stats ? &ps : nullptr, nullptr, nullptr);
+        px[0] += col[0]; px[1] += col[1]; px[2] += col[2];
+      }
+    }
+    if (stats) { q += ps.queries; nn += ps.c.nodes; ntr += ps.c.tris; }
+  }
+  if (stats) { stats[0] = q; stats[1] = nn; stats[2] = ntr; }
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// Reference gradient of the image loss sum_p w_p . I_p, where I_p is the mean of
+// `spp` samples of pixel p and w = d_film (W*H*3), with respect to every texel
+// of every texture (d_tex, concatenated like the scene's texels) and to each
+// BSDF's emission L_e (d_emit, 3 per BSDF; non-zero for lights only).
+// Each sample's colour is a sum of terms; a term added after Lambert vertex i
+// carries that vertex's albedo as a factor, so d(term)/d(albedo_i) = term/albedo_i
+// per channel.  Emission terms are linear in the emitting light's L_e.
+// Russian-roulette probabilities and discrete choices are held fixed.
+double orc_render_backward(void* s, const float* cam, uint32_t W, uint32_t H, uint32_t spp,
+                           uint32_t sample0, uint32_t max_depth, uint32_t rr_start,
+                           float rr_threshold, uint64_t seed, const float* d_film, double* d_tex,
+                           double* d_emit, int threads) {
+  Scene const& sc = *static_cast<Scene*>(s);
+  Camera const c = make_camera(cam, W, H);
+  PTParams pp;
+  pp.kmax = max_depth + 1;
+  pp.rr_start = rr_start;
+  pp.rr_threshold = rr_threshold;
+  pp.seed = seed;
+  std::vector<size_t> tex_off(sc.textures.size() + 1, 0);
+  for (size_t i = 0; i < sc.textures.size(); ++i)
+    tex_off[i + 1] = tex_off[i] + size_t(sc.textures[i].w) * sc.textures[i].h * 3;
+  int const nt = n_threads(threads);
+  auto t0 = std::chrono::steady_clock::now();
+  std::vector<std::vector<double>> gt(size_t(nt), std::vector<double>(tex_off.back(), 0.0));
+  std::vector<std::vector<double>> ge(size_t(nt), std::vector<double>(sc.bsdfs.size() * 3, 0.0));
+  double const inv_spp = 1.0 / double(spp);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
+  for (int64_t y = 0; y < int64_t(H); ++y) {
+#ifdef _OPENMP
+    int const tid = omp_get_thread_num();
+#else
+    int const tid = 0;
+#endif
+    std::vector<VertexRecord> rec;
+    std::vector<V3> lc(sc.bsdfs.size());
+    for (uint32_t x = 0; x < W; ++x) {
+      size_t const p = size_t(y) * W + x;
+      double const w[3] = {d_film[3 * p] * inv_spp, d_film[3 * p + 1] * inv_spp, d_film[3 * p + 2] * inv_spp};
+      for (uint32_t smp = sample0; smp < sample0 + spp; ++smp) {
+        rec.clear();
+        for (V3& v : lc) v = V3(0, 0, 0);
+        V3 const col = pt_path(sc, c, pp, W, H, x, uint32_t(y), smp, nullptr, &rec, &lc);
+        for (VertexRecord const& r : rec) {
+          if (r.texture < 0) continue;
+          size_t const base = tex_off[size_t(r.texture)] + size_t(r.texel) * 3;
+          for (int ch = 0; ch < 3; ++ch) {
+            double const suffix = double(col[ch]) - double(r.prefix[ch]);
+            gt[size_t(tid)][base + ch] += w[ch] * suffix / double(r.albedo[ch]);
+          }
+        }
+        for (size_t b = 0; b < sc.bsdfs.size(); ++b) {
+          if (sc.bsdfs[b].type != LIGHT) continue;
+          for (int ch = 0; ch < 3; ++ch)
+            if (sc.bsdfs[b].rgb[ch] != 0)
+              ge[size_t(tid)][3 * b + ch] += w[ch] * double(lc[b][ch]) / double(sc.bsdfs[b].rgb[ch]);
+        }
+      }
+    }
+  }
+  for (int t = 0; t < nt; ++t) {
+    for (size_t i = 0; i < tex_off.back(); ++i) d_tex[i] += gt[size_t(t)][i];
+    for (size_t i = 0; i < sc.bsdfs.size() * 3; ++i) d_emit[i] += ge[size_t(t)][i];
+  }
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+int orc_max_threads(void) { return n_threads(0); }
+
+}  // extern "C"

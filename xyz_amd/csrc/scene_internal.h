@@ -1,0 +1,31 @@
+// Host-side scene object shared by the C ABI translation units.
+#ifndef XYZ_SCENE_INTERNAL_H_
+#define XYZ_SCENE_INTERNAL_H_
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "device_common.h"
+#include "xyz_internal.h"
+
+struct xyz_scene {
+  xyzhip::DevScene dev;
+  xyz_scene_info info;
+  xyzhip::FlatTree tree;  // host copy (released after upload)
+  std::vector<void*> allocs;
+  uint64_t n_tris = 0;
+  uint64_t n_texels = 0;
+};
+
+namespace xyzhip {
+
+int hip_fail(hipError_t e, const char* what);
+
+int launch_closest_hit(const DevScene& s, uint32_t n, const float* org, const float* dir,
+                       int32_t* prim, float* t, float* f, float* g, uint8_t* back,
+                       uint32_t* counters, hipStream_t stream);
+
+}  // namespace xyzhip
+
+#endif  // XYZ_SCENE_INTERNAL_H_

@@ -1,0 +1,451 @@
+// C ABI of the gfx950 library: trees, scenes and the closest-hit entry point.
+// The path-tracing entry points live in render.hip.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "device_common.h"
+#include "scene_internal.h"
+#include "xyz_internal.h"
+
+namespace xyzhip {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int hip_fail(hipError_t e, const char* what) {
+  set_error(std::string(what) + ": " + hipGetErrorString(e));
+  return XYZ_E_DEVICE;
+}
+
+// ------------------------------------------------------------ closest hit
+constexpr uint32_t kHitBlock = 256;
+
+// One ray per lane; blocks are remapped so that each XCD (blocks are dealt to
+// the 8 XCDs round robin) walks one contiguous range of rays, keeping the
+// kd nodes and triangles that neighbouring (coherent) rays share in one L2.
+template <bool kCount>
+__global__ void __launch_bounds__(kHitBlock)
+closest_hit_kernel(DevScene s, uint32_t n, const float* __restrict__ org,
+                   const float* __restrict__ dir, int32_t* __restrict__ prim,
+                   float* __restrict__ tout, float* __restrict__ fout, float* __restrict__ gout,
+                   uint8_t* __restrict__ bout, uint32_t* __restrict__ counters,
+                   uint32_t blocks_per_xcd) {
+  extern __shared__ uint2 lds_stack[];
+  uint32_t const b = blockIdx.x;
+  uint32_t const logical = (b & 7u) * blocks_per_xcd + (b >> 3);
+  uint32_t const i = logical * kHitBlock + threadIdx.x;
+  if (i >= n) return;
+  float3 const o = make_float3(org[3 * size_t(i)], org[3 * size_t(i) + 1], org[3 * size_t(i) + 2]);
+  float3 const d = make_float3(dir[3 * size_t(i)], dir[3 * size_t(i) + 1], dir[3 * size_t(i) + 2]);
+  uint32_t cn = 0, ct = 0;
+  HitRec const h = trace_closest<kCount>(s, o, d, lds_stack + threadIdx.x, kHitBlock, &cn, &ct);
+  prim[i] = h.prim;
+  if (tout) tout[i] = h.t;
+  if (fout) fout[i] = h.f;
+  if (gout) gout[i] = h.g;
+  if (bout) bout[i] = h.back ? 1 : 0;
+  if (kCount) {
+    counters[i] = cn;
+    counters[size_t(n) + i] = ct;
+  }
+}
+
+int launch_closest_hit(const DevScene& s, uint32_t n, const float* org, const float* dir,
+                       int32_t* prim, float* t, float* f, float* g, uint8_t* back,
+                       uint32_t* counters, hipStream_t stream) {
+  if (n == 0) return XYZ_OK;
+  uint32_t const blocks = (n + kHitBlock - 1) / kHitBlock;
+  uint32_t const per_xcd = (blocks + 7) / 8;
+  uint32_t const grid = per_xcd * 8;
+  size_t const lds = size_t(s.stack_cap) * kHitBlock * sizeof(uint2);
+  if (counters) {
+    hipLaunchKernelGGL(closest_hit_kernel<true>, dim3(grid), dim3(kHitBlock), lds, stream, s, n,
+                       org, dir, prim, t, f, g, back, counters, per_xcd);
+  } else {
+    hipLaunchKernelGGL(closest_hit_kernel<false>, dim3(grid), dim3(kHitBlock), lds, stream, s, n,
+                       org, dir, prim, t, f, g, back, counters, per_xcd);
+  }
+  hipError_t const e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "closest_hit launch");
+  return XYZ_OK;
+}
+
+// ------------------------------------------------------------ scene upload
+namespace {
+
+template <typename T>
+int upload(const std::vector<T>& h, T** d, uint64_t* bytes) {
+  *d = nullptr;
+  if (h.empty()) return XYZ_OK;
+  size_t const nb = h.size() * sizeof(T);
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(d), nb);
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+  e = hipMemcpy(*d, h.data(), nb, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(e, "hipMemcpy");
+  *bytes += nb;
+  return XYZ_OK;
+}
+
+template <typename T>
+int upload_field(xyz_scene* sc, const std::vector<T>& h, const T*& field, uint64_t* bytes) {
+  T* p = nullptr;
+  int const rc = upload(h, &p, bytes);
+  if (rc != XYZ_OK) return rc;
+  if (p) sc->allocs.push_back(p);
+  field = p;
+  return XYZ_OK;
+}
+
+inline float f3x(const float* p, size_t i, int k) { return p[3 * i + size_t(k)]; }
+
+}  // namespace
+
+void scene_free(xyz_scene* s) {
+  for (void* p : s->allocs) {
+    if (p) (void)hipFree(p);
+  }
+  s->allocs.clear();
+}
+
+int scene_create(const xyz_scene_desc* desc, xyz_scene** out) {
+  if (!desc || !out) {
+    set_error("xyz_scene_create: NULL argument");
+    return XYZ_E_INVALID;
+  }
+  *out = nullptr;
+  uint32_t const n = desc->n_tris;
+  if (n > 0 && (!desc->v0 || !desc->v1 || !desc->v2)) {
+    set_error("xyz_scene_create: NULL vertex arrays");
+    return XYZ_E_INVALID;
+  }
+  if (desc->n_bsdfs > 0 && !desc->bsdfs) {
+    set_error("xyz_scene_create: NULL bsdf array");
+    return XYZ_E_INVALID;
+  }
+  uint32_t const nb = desc->n_bsdfs;
+  for (uint32_t i = 0; desc->bsdf && i < n; ++i) {
+    if (desc->bsdf[i] >= nb) {
+      set_error("xyz_scene_create: triangle " + std::to_string(i) + " has bsdf index out of range");
+      return XYZ_E_INVALID;
+    }
+  }
+  if (!desc->bsdf && n > 0 && nb == 0) {
+    set_error("xyz_scene_create: no bsdfs");
+    return XYZ_E_INVALID;
+  }
+  if (desc->kd_max_depth > 30) {
+    // the per-lane LDS traversal stack holds kd_max_depth + 1 entries of 8 bytes
+    set_error("xyz_scene_create: kd_max_depth above 30 is not supported on the device");
+    return XYZ_E_UNSUPPORTED;
+  }
+  std::unique_ptr<xyz_scene> sc(new xyz_scene());
+  std::string err = build_flat_tree(n, desc->v0, desc->v1, desc->v2, desc->kd_max_bucket,
+                                    desc->kd_max_depth, &sc->tree);
+  if (!err.empty()) {
+    set_error("xyz_scene_create: " + err);
+    return XYZ_E_INVALID;
+  }
+  FlatTree const& T = sc->tree;
+  size_t const n_refs = T.leaf_prims.size();
+
+  // triangle precompute, xyz/geom/triangle.cpp:10-60
+  std::vector<float> e1(size_t(n) * 3), e2(size_t(n) * 3), nrm(size_t(n) * 3), ng(size_t(n) * 3);
+  for (size_t i = 0; i < n; ++i) {
+    float a[3], b[3], c[3];
+    for (int k = 0; k < 3; ++k) {
+      a[k] = f3x(desc->v0, i, k);
+      b[k] = f3x(desc->v1, i, k) - a[k];
+      c[k] = f3x(desc->v2, i, k) - a[k];
+      e1[3 * i + k] = b[k];
+      e2[3 * i + k] = c[k];
+    }
+    // n = cross(e2, e1), libhi/hi/math/vector3.hpp:126-131
+    float const nx = c[1] * b[2] - c[2] * b[1];
+    float const ny = c[2] * b[0] - c[0] * b[2];
+    float const nz = c[0] * b[1] - c[1] * b[0];
+    nrm[3 * i] = nx;
+    nrm[3 * i + 1] = ny;
+    nrm[3 * i + 2] = nz;
+    // ng = normalize(n) = n * (1 / sqrt(dot(n, n)))
+    float const l2 = nx * nx + ny * ny + nz * nz;
+    float const r = 1.0f / std::sqrt(l2);
+    ng[3 * i] = nx * r;
+    ng[3 * i + 1] = ny * r;
+    ng[3 * i + 2] = nz * r;
+  }
+  std::vector<float4> A(n_refs), B(n_refs), C(n_refs);
+  std::vector<int32_t> rp(n_refs);
+  for (size_t r = 0; r < n_refs; ++r) {
+    size_t const i = T.leaf_prims[r];
+    A[r] = make_float4(f3x(desc->v0, i, 0), f3x(desc->v0, i, 1), f3x(desc->v0, i, 2), nrm[3 * i]);
+    B[r] = make_float4(e1[3 * i], e1[3 * i + 1], e1[3 * i + 2], nrm[3 * i + 1]);
+    C[r] = make_float4(e2[3 * i], e2[3 * i + 1], e2[3 * i + 2], nrm[3 * i + 2]);
+    rp[r] = int32_t(i);
+  }
+  std::vector<uint2> nodes(T.nodes.size() / 2);
+  for (size_t i = 0; i < nodes.size(); ++i) nodes[i] = make_uint2(T.nodes[2 * i], T.nodes[2 * i + 1]);
+
+  // per-primitive shading records
+  std::vector<float4> png(n), pv0(n), pe1(n), pe2(n);
+  for (size_t i = 0; i < n; ++i) {
+    uint32_t const bs = desc->bsdf ? desc->bsdf[i] : 0u;
+    float bsf;
+    std::memcpy(&bsf, &bs, 4);
+    png[i] = make_float4(ng[3 * i], ng[3 * i + 1], ng[3 * i + 2], bsf);
+    pv0[i] = make_float4(f3x(desc->v0, i, 0), f3x(desc->v0, i, 1), f3x(desc->v0, i, 2), 0.0f);
+    pe1[i] = make_float4(e1[3 * i], e1[3 * i + 1], e1[3 * i + 2], 0.0f);
+    pe2[i] = make_float4(e2[3 * i], e2[3 * i + 1], e2[3 * i + 2], 0.0f);
+  }
+  std::vector<float4> pn0, ps1, ps2;
+  if (desc->n0 && desc->n1 && desc->n2) {
+    pn0.resize(n);
+    ps1.resize(n);
+    ps2.resize(n);
+    for (size_t i = 0; i < n; ++i) {
+      float a[3], b[3], c[3];
+      for (int k = 0; k < 3; ++k) {
+        a[k] = f3x(desc->n0, i, k);
+        b[k] = f3x(desc->n1, i, k) - a[k];
+        c[k] = f3x(desc->n2, i, k) - a[k];
+      }
+      pn0[i] = make_float4(a[0], a[1], a[2], 0.0f);
+      ps1[i] = make_float4(b[0], b[1], b[2], 0.0f);
+      ps2[i] = make_float4(c[0], c[1], c[2], 0.0f);
+    }
+  }
+  std::vector<float2> puv;
+  if (desc->uv) {
+    puv.resize(size_t(n) * 3);
+    for (size_t i = 0; i < size_t(n) * 3; ++i)
+      puv[i] = make_float2(desc->uv[2 * i], desc->uv[2 * i + 1]);
+  }
+
+  // materials
+  std::vector<DevBsdf> bs(nb);
+  for (uint32_t i = 0; i < nb; ++i) {
+    xyz_bsdf_desc const& d = desc->bsdfs[i];
+    if (d.type > XYZ_LIGHT) {
+      set_error("xyz_scene_create: bsdf " + std::to_string(i) + " has unknown type");
+      return XYZ_E_INVALID;
+    }
+    if (d.texture >= 0 && uint32_t(d.texture) >= desc->n_textures) {
+      set_error("xyz_scene_create: bsdf " + std::to_string(i) + " texture out of range");
+      return XYZ_E_INVALID;
+    }
+    bs[i].type = d.type;
+    bs[i].texture = d.texture;
+    for (int k = 0; k < 3; ++k) {
+      bs[i].rgb[k] = d.rgb[k];
+      bs[i].phi[k] = 0.0f;
+    }
+    bs[i].shininess = d.shininess;
+    bs[i].ior = d.ior;
+  }
+  // lights and their area CDF, caller order (xyz/core/scene_mqo.cpp:509-518)
+  std::vector<float> cdf(1, 0.0f);
+  std::vector<uint32_t> lprim;
+  for (uint32_t i = 0; i < n; ++i) {
+    uint32_t const b = desc->bsdf ? desc->bsdf[i] : 0u;
+    if (nb > 0 && bs[b].type == XYZ_LIGHT) {
+      // area = length(cross(e2, e1)) * 0.5 (xyz/geom/triangle.hpp:57-59)
+      float const nx = nrm[3 * size_t(i)], ny = nrm[3 * size_t(i) + 1], nz = nrm[3 * size_t(i) + 2];
+      float const area = std::sqrt(nx * nx + ny * ny + nz * nz) * 0.5f;
+      lprim.push_back(i);
+      cdf.push_back(cdf.back() + area);
+    }
+  }
+  float const light_area = cdf.back();
+  for (uint32_t i = 0; i < nb; ++i) {
+    if (bs[i].type == XYZ_LIGHT) {
+      double const s = 3.14159265358979323846 * double(light_area);
+      for (int k = 0; k < 3; ++k) bs[i].phi[k] = float(double(bs[i].rgb[k]) * s);
+    }
+  }
+  // textures
+  std::vector<DevTex> tex(desc->n_textures);
+  std::vector<float> texels;
+  {
+    size_t off = 0;
+    for (uint32_t t = 0; t < desc->n_textures; ++t) {
+      uint32_t const w = desc->tex_wh[2 * t], h = desc->tex_wh[2 * t + 1];
+      if (w == 0 || h == 0) {
+        set_error("xyz_scene_create: empty texture");
+        return XYZ_E_INVALID;
+      }
+      tex[t].w = w;
+      tex[t].h = h;
+      tex[t].offset = uint32_t(off);
+      tex[t].pad = 0;
+      off += size_t(w) * h * 3;
+    }
+    if (off > 0) {
+      if (!desc->texels) {
+        set_error("xyz_scene_create: NULL texels");
+        return XYZ_E_INVALID;
+      }
+      texels.assign(desc->texels, desc->texels + off);
+    }
+    sc->n_texels = off;
+  }
+
+  // device upload
+  int rc = XYZ_OK;
+  uint64_t bytes = 0;
+  DevScene& D = sc->dev;
+  std::memset(&D, 0, sizeof(D));
+#define UP(vec, field)                                 \
+  do {                                                 \
+    rc = upload_field(sc.get(), vec, D.field, &bytes); \
+    if (rc != XYZ_OK) {                                \
+      scene_free(sc.get());                            \
+      return rc;                                       \
+    }                                                  \
+  } while (0)
+  UP(nodes, nodes);
+  UP(A, triA);
+  UP(B, triB);
+  UP(C, triC);
+  UP(rp, ref_prim);
+  UP(png, prim_ng);
+  UP(pn0, prim_n0);
+  UP(ps1, prim_s1);
+  UP(ps2, prim_s2);
+  UP(puv, prim_uv);
+  UP(pv0, prim_v0);
+  UP(pe1, prim_e1);
+  UP(pe2, prim_e2);
+  UP(cdf, light_cdf);
+  UP(lprim, light_prim);
+  UP(bs, bsdfs);
+  UP(texels, texels);
+  UP(tex, tex);
+#undef UP
+  D.stack_cap = T.max_depth_param + 1;
+  D.n_lights = uint32_t(lprim.size());
+  D.light_area = light_area;
+  D.n_bsdfs = nb;
+  D.n_tex = desc->n_textures;
+
+  xyz_scene_info& I = sc->info;
+  I.n_tris = n;
+  I.n_nodes = nodes.size();
+  I.n_leaves = T.n_leaves;
+  I.n_leaf_refs = n_refs;
+  I.depth = T.depth;
+  I.max_leaf = T.max_leaf;
+  for (int k = 0; k < 3; ++k) {
+    I.bmin[k] = T.bmin[k];
+    I.bmax[k] = T.bmax[k];
+  }
+  I.build_seconds = T.build_seconds;
+  I.device_bytes = bytes;
+  sc->n_tris = n;
+  // the host copy of the tree is no longer needed
+  std::vector<uint32_t>().swap(sc->tree.nodes);
+  std::vector<uint32_t>().swap(sc->tree.leaf_prims);
+  *out = sc.release();
+  return XYZ_OK;
+}
+
+}  // namespace xyzhip
+
+using namespace xyzhip;
+
+extern "C" {
+
+const char* xyz_last_error(void) { return g_last_error.c_str(); }
+
+const char* xyz_version(void) { return "xyz-hip 0.1 (gfx950)"; }
+
+int xyz_kdtree_build(uint32_t n_tris, const float* v0, const float* v1, const float* v2,
+                     uint32_t max_bucket, uint32_t max_depth, xyz_kdtree** out) {
+  if (!out) {
+    set_error("xyz_kdtree_build: NULL out");
+    return XYZ_E_INVALID;
+  }
+  *out = nullptr;
+  std::unique_ptr<xyz_kdtree> t(new xyz_kdtree());
+  std::string err = build_flat_tree(n_tris, v0, v1, v2, max_bucket, max_depth, &t->t);
+  if (!err.empty()) {
+    set_error("xyz_kdtree_build: " + err);
+    return XYZ_E_INVALID;
+  }
+  t->n_tris = n_tris;
+  *out = t.release();
+  return XYZ_OK;
+}
+
+int xyz_kdtree_info(const xyz_kdtree* tree, xyz_scene_info* info) {
+  if (!tree || !info) {
+    set_error("xyz_kdtree_info: NULL argument");
+    return XYZ_E_INVALID;
+  }
+  std::memset(info, 0, sizeof(*info));
+  info->n_tris = tree->n_tris;
+  info->n_nodes = tree->t.nodes.size() / 2;
+  info->n_leaves = tree->t.n_leaves;
+  info->n_leaf_refs = tree->t.leaf_prims.size();
+  info->depth = tree->t.depth;
+  info->max_leaf = tree->t.max_leaf;
+  for (int k = 0; k < 3; ++k) {
+    info->bmin[k] = tree->t.bmin[k];
+    info->bmax[k] = tree->t.bmax[k];
+  }
+  info->build_seconds = tree->t.build_seconds;
+  return XYZ_OK;
+}
+
+int xyz_kdtree_export(const xyz_kdtree* tree, uint32_t* nodes, uint32_t* leaf_prims) {
+  if (!tree) {
+    set_error("xyz_kdtree_export: NULL tree");
+    return XYZ_E_INVALID;
+  }
+  if (nodes) std::memcpy(nodes, tree->t.nodes.data(), tree->t.nodes.size() * 4);
+  if (leaf_prims) std::memcpy(leaf_prims, tree->t.leaf_prims.data(), tree->t.leaf_prims.size() * 4);
+  return XYZ_OK;
+}
+
+void xyz_kdtree_destroy(xyz_kdtree* tree) { delete tree; }
+
+int xyz_scene_create(const xyz_scene_desc* desc, xyz_scene** out) { return scene_create(desc, out); }
+
+int xyz_scene_get_info(const xyz_scene* scene, xyz_scene_info* info) {
+  if (!scene || !info) {
+    set_error("xyz_scene_get_info: NULL argument");
+    return XYZ_E_INVALID;
+  }
+  *info = scene->info;
+  return XYZ_OK;
+}
+
+void xyz_scene_destroy(xyz_scene* scene) {
+  if (!scene) return;
+  scene_free(scene);
+  delete scene;
+}
+
+int xyz_closest_hit(xyz_scene* scene, uint32_t n, const float* org, const float* dir,
+                    int32_t* prim, float* t, float* f, float* g, uint8_t* back,
+                    uint32_t* counters, void* stream) {
+  if (!scene) {
+    set_error("xyz_closest_hit: NULL scene");
+    return XYZ_E_INVALID;
+  }
+  if (n > 0 && (!org || !dir || !prim)) {
+    set_error("xyz_closest_hit: NULL ray or prim buffer");
+    return XYZ_E_INVALID;
+  }
+  return launch_closest_hit(scene->dev, n, org, dir, prim, t, f, g, back, counters,
+                            static_cast<hipStream_t>(stream));
+}
+
+}  // extern "C"

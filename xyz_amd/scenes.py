@@ -1,0 +1,329 @@
+"""Deterministic synthetic scenes and ray batches for the workloads this package
+is measured on.
+
+Every generator returns plain numpy float32/uint32 arrays; triangle order is the
+caller order that defines primitive ids.  Winding: the renderer's geometric
+normal is cross(v2 - v0, v1 - v0) (xyz/geom/triangle.cpp:20), so triangles whose
+outside matters (glass, lights) are emitted with that normal pointing outward.
+"""
+import math
+
+import numpy as np
+
+LAMBERT, MIRROR, GLASS, METAL, LIGHT = 0, 1, 2, 3, 4
+
+
+# --------------------------------------------------------------- helpers
+def _orient(v0, v1, v2, outward):
+    """Swap v1/v2 where cross(v2-v0, v1-v0) does not point along `outward`."""
+    n = np.cross(v2 - v0, v1 - v0)
+    flip = (n * outward).sum(axis=1) < 0
+    a, b = v1.copy(), v2.copy()
+    a[flip], b[flip] = v2[flip], v1[flip]
+    return v0, a, b
+
+
+def quad(p, u, v, outward):
+    """Two triangles spanning p, p+u, p+u+v, p+v with normal along `outward`."""
+    p, u, v = (np.asarray(x, np.float64) for x in (p, u, v))
+    a, b, c, d = p, p + u, p + u + v, p + v
+    v0 = np.array([a, a], np.float64)
+    v1 = np.array([b, c], np.float64)
+    v2 = np.array([c, d], np.float64)
+    out = np.tile(np.asarray(outward, np.float64), (2, 1))
+    return _orient(v0, v1, v2, out)
+
+
+def box(center, size, rot=None, tess=1):
+    """Axis box (size per axis) rotated by 3x3 `rot`, each face split into
+    tess x tess x 2 triangles, outward normals."""
+    center = np.asarray(center, np.float64)
+    h = np.asarray(size, np.float64) / 2
+    rot = np.eye(3) if rot is None else np.asarray(rot, np.float64)
+    V0, V1, V2 = [], [], []
+    for ax in range(3):
+        for sgn in (-1.0, 1.0):
+            n = np.zeros(3)
+            n[ax] = sgn
+            a1, a2 = (ax + 1) % 3, (ax + 2) % 3
+            e_u = np.zeros(3)
+            e_u[a1] = 2 * h[a1] / tess
+            e_v = np.zeros(3)
+            e_v[a2] = 2 * h[a2] / tess
+            base = np.zeros(3)
+            base[ax] = sgn * h[ax]
+            base[a1] = -h[a1]
+            base[a2] = -h[a2]
+            for i in range(tess):
+                for j in range(tess):
+                    p = base + i * e_u + j * e_v
+                    v0, v1, v2 = quad(p, e_u, e_v, n)
+                    V0.append(v0)
+                    V1.append(v1)
+                    V2.append(v2)
+    V0, V1, V2 = (np.concatenate(x) for x in (V0, V1, V2))
+    V0, V1, V2 = (x @ rot.T + center for x in (V0, V1, V2))
+    return V0, V1, V2
+
+
+def rot_y(deg):
+    a = math.radians(deg)
+    c, s = math.cos(a), math.sin(a)
+    return np.array([[c, 0, s], [0, 1, 0], [-s, 0, c]])
+
+
+def random_rotation(rng):
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    return np.array([
+        [1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+        [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+        [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)],
+    ])
+
+
+# -------------------------------------------------------- icosphere + noise
+def _icosahedron():
+    t = (1.0 + 5 ** 0.5) / 2.0
+    v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t],
+                  [0, -1, -t], [0, 1, -t], [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]],
+                 np.float64)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    f = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9],
+                  [5, 11, 4], [11, 10, 2], [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2],
+                  [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5], [2, 4, 11], [6, 2, 10],
+                  [8, 6, 7], [9, 8, 1]], np.int64)
+    return v, f
+
+
+def icosphere(level):
+    """Unit icosphere subdivided `level` times (20 * 4**level faces), shared vertices."""
+    v, f = _icosahedron()
+    for _ in range(level):
+        e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
+        e.sort(axis=1)
+        key = e[:, 0] * (len(v) + 1) + e[:, 1]
+        uniq, inv = np.unique(key, return_inverse=True)
+        a = uniq // (len(v) + 1)
+        b = uniq % (len(v) + 1)
+        mid = v[a] + v[b]
+        mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+        base = len(v)
+        v = np.concatenate([v, mid])
+        nf = len(f)
+        m01 = base + inv[:nf]
+        m12 = base + inv[nf:2 * nf]
+        m20 = base + inv[2 * nf:]
+        f = np.concatenate([
+            np.stack([f[:, 0], m01, m20], 1),
+            np.stack([f[:, 1], m12, m01], 1),
+            np.stack([f[:, 2], m20, m12], 1),
+            np.stack([m01, m12, m20], 1),
+        ])
+    return v, f
+
+
+def geodesic_sphere(freq):
+    """Unit class-I geodesic sphere: each icosahedron face split into freq**2 triangles."""
+    v, f = _icosahedron()
+    V, F = [], []
+    off = 0
+    for a, b, c in f:
+        A, B, C = v[a], v[b], v[c]
+        idx = {}
+        pts = []
+        for i in range(freq + 1):
+            for j in range(freq + 1 - i):
+                k = freq - i - j
+                p = (A * k + B * i + C * j) / freq
+                idx[(i, j)] = len(pts)
+                pts.append(p)
+        tris = []
+        for i in range(freq):
+            for j in range(freq - i):
+                tris.append((idx[(i, j)], idx[(i + 1, j)], idx[(i, j + 1)]))
+                if j < freq - i - 1:
+                    tris.append((idx[(i + 1, j)], idx[(i + 1, j + 1)], idx[(i, j + 1)]))
+        pts = np.array(pts)
+        pts /= np.linalg.norm(pts, axis=1, keepdims=True)
+        V.append(pts)
+        F.append(np.array(tris, np.int64) + off)
+        off += len(pts)
+    return np.concatenate(V), np.concatenate(F)
+
+
+def _lattice(ix, iy, iz, seed):
+    """Seeded integer hash of lattice points -> uniform in [-1, 1]."""
+    h = (ix.astype(np.uint64) * np.uint64(0x8DA6B343) ^ iy.astype(np.uint64) * np.uint64(0xD8163841)
+         ^ iz.astype(np.uint64) * np.uint64(0xCB1AB31F) ^ np.uint64(seed) * np.uint64(0x9E3779B9))
+    h &= np.uint64(0xFFFFFFFF)
+    h ^= h >> np.uint64(16)
+    h = (h * np.uint64(0x7FEB352D)) & np.uint64(0xFFFFFFFF)
+    h ^= h >> np.uint64(15)
+    h = (h * np.uint64(0x846CA68B)) & np.uint64(0xFFFFFFFF)
+    h ^= h >> np.uint64(16)
+    return h.astype(np.float64) / 4294967295.0 * 2.0 - 1.0
+
+
+def value_noise(p, freq, seed):
+    q = p * freq
+    i = np.floor(q)
+    fr = q - i
+    w = fr * fr * (3 - 2 * fr)
+    i = i.astype(np.int64) + (1 << 20)
+    out = np.zeros(len(p))
+    for dx in (0, 1):
+        for dy in (0, 1):
+            for dz in (0, 1):
+                c = _lattice(i[:, 0] + dx, i[:, 1] + dy, i[:, 2] + dz, seed)
+                wx = w[:, 0] if dx else 1 - w[:, 0]
+                wy = w[:, 1] if dy else 1 - w[:, 1]
+                wz = w[:, 2] if dz else 1 - w[:, 2]
+                out += c * wx * wy * wz
+    return out
+
+
+def noise3(p, seed):
+    """3-octave value noise (frequencies 4, 8, 16; amplitudes 1, 1/2, 1/4) in [-1, 1]."""
+    n = (value_noise(p, 4.0, seed) + 0.5 * value_noise(p, 8.0, seed + 1)
+         + 0.25 * value_noise(p, 16.0, seed + 2))
+    return n / 1.75
+
+
+def displaced_sphere(v, f, amplitude=0.05, seed=0, center=(0, 0, 0), radius=1.0):
+    v = v * (1.0 + amplitude * noise3(v, seed))[:, None] * radius + np.asarray(center)
+    v0, v1, v2 = v[f[:, 0]], v[f[:, 1]], v[f[:, 2]]
+    out = (v0 + v1 + v2) / 3 - np.asarray(center)
+    return _orient(v0, v1, v2, out)
+
+
+def _pack(parts):
+    v0 = np.concatenate([p[0] for p in parts]).astype(np.float32)
+    v1 = np.concatenate([p[1] for p in parts]).astype(np.float32)
+    v2 = np.concatenate([p[2] for p in parts]).astype(np.float32)
+    return v0, v1, v2
+
+
+# ------------------------------------------------------------ workloads
+def cornell_box(textured=False, tex_res=64, tex_seed=0):
+    """Cornell box: unit cube [0,1]^3 with the z=0 face open, 5 Lambert walls,
+    a 0.3^3 box rotated -18 deg and a 0.3x0.6x0.3 box rotated +18 deg about y,
+    and a 0.25x0.25 quad light at y=0.999 (L_e = 15).  36 triangles.
+
+    With `textured`, each wall gets a tex_res^2 RGB albedo texture with texels
+    uniform in [0.1, 0.9] and UVs spanning the wall."""
+    white, red, green = (0.73, 0.73, 0.73), (0.65, 0.05, 0.05), (0.12, 0.45, 0.15)
+    parts, bs = [], []
+    uvs = []
+    walls = [  # origin, u, v, inward normal, bsdf
+        ((0, 0, 0), (1, 0, 0), (0, 0, 1), (0, 1, 0), 0),   # floor
+        ((0, 1, 0), (1, 0, 0), (0, 0, 1), (0, -1, 0), 0),  # ceiling
+        ((0, 0, 1), (1, 0, 0), (0, 1, 0), (0, 0, -1), 0),  # back
+        ((0, 0, 0), (0, 1, 0), (0, 0, 1), (1, 0, 0), 1),   # left x=0 (red)
+        ((1, 0, 0), (0, 1, 0), (0, 0, 1), (-1, 0, 0), 2),  # right x=1 (green)
+    ]
+    for wi, (p, u, v, n, b) in enumerate(walls):
+        v0, v1, v2 = quad(p, u, v, n)
+        parts.append((v0, v1, v2))
+        bsdf = (3 + wi) if textured else b
+        bs += [bsdf, bsdf]
+        # uv of each vertex: position relative to the wall frame
+        for tv in (v0, v1, v2):
+            pass
+        P, U, Vv = (np.asarray(x, np.float64) for x in (p, u, v))
+        tri_uv = []
+        for t in range(2):
+            row = []
+            for vert in (v0[t], v1[t], v2[t]):
+                d = vert - P
+                row += [float(d @ U), float(d @ Vv)]
+            tri_uv.append(row)
+        uvs += tri_uv
+    for center, size, ang in (((0.30, 0.15, 0.30), (0.3, 0.3, 0.3), -18.0),
+                              ((0.68, 0.30, 0.66), (0.3, 0.6, 0.3), 18.0)):
+        parts.append(box(center, size, rot_y(ang)))
+        bs += [0] * 12
+        uvs += [[0.0] * 6] * 12
+    lv = quad((0.375, 0.999, 0.375), (0.25, 0, 0), (0, 0, 0.25), (0, -1, 0))
+    parts.append(lv)
+    light_bsdf = 3 if not textured else 8
+    bs += [light_bsdf, light_bsdf]
+    uvs += [[0.0] * 6] * 2
+    v0, v1, v2 = _pack(parts)
+    bsdfs = [dict(type=LAMBERT, rgb=white), dict(type=LAMBERT, rgb=red),
+             dict(type=LAMBERT, rgb=green)]
+    textures = None
+    if textured:
+        rng = np.random.default_rng(tex_seed)
+        textures = [rng.uniform(0.1, 0.9, size=(tex_res, tex_res, 3)).astype(np.float32)
+                    for _ in range(5)]
+        for wi in range(5):
+            bsdfs.append(dict(type=LAMBERT, rgb=(1.0, 1.0, 1.0), texture=wi))
+    bsdfs.append(dict(type=LIGHT, rgb=(15.0, 15.0, 15.0)))
+    if not textured:
+        bsdfs = bsdfs[:3] + [bsdfs[-1]]
+    camera = np.array([0.5, 0.5, -1.4, 0.5, 0.5, 0.5, 0, 1, 0, 40.0], np.float32)
+    return dict(v0=v0, v1=v1, v2=v2, tri_bsdf=np.array(bs, np.uint32), bsdfs=bsdfs,
+                uv=np.array(uvs, np.float32), textures=textures, camera=camera)
+
+
+def closest_hit_mesh(level=8, n_boxes=64, seed=1):
+    """Icosphere of `level` (20*4**level tris) radially displaced by 0.05 x
+    3-octave value noise, plus `n_boxes` random boxes (12 tris each) with
+    centres in [-2,2]^3, extents in [0.05, 0.3] and random rotations."""
+    v, f = icosphere(level)
+    parts = [displaced_sphere(v, f, 0.05, seed)]
+    rng = np.random.default_rng(seed)
+    for _ in range(n_boxes):
+        c = rng.uniform(-2, 2, 3)
+        s = rng.uniform(0.05, 0.3, 3)
+        parts.append(box(c, s, random_rotation(rng)))
+    v0, v1, v2 = _pack(parts)
+    return dict(v0=v0, v1=v1, v2=v2)
+
+
+def pinhole_rays(eye, at, fov_deg, W, H, up=(0, 1, 0)):
+    """One ray per pixel centre of a W x H pinhole camera (row 0 = bottom).
+    Directions follow the renderer's pinhole (film 0.025, V = 0.0125/tan(fov/2))."""
+    eye = np.asarray(eye, np.float32)
+    at = np.asarray(at, np.float32)
+    up = np.asarray(up, np.float32)
+
+    def nrm(a):
+        return a * (np.float32(1) / np.sqrt((a * a).sum(-1, keepdims=True, dtype=np.float32)))
+    back = nrm(eye - at)
+    right = nrm(np.cross(up, back).astype(np.float32))
+    upv = np.cross(back, right).astype(np.float32)
+    F = np.float32(0.025)
+    A = np.float32(W) / np.float32(H)
+    V = np.float32(0.0125 / math.tan(math.radians(fov_deg) / 2))
+    xs = ((np.arange(W, dtype=np.float32) + np.float32(0.5)) / np.float32(W) - np.float32(0.5)) * F * A
+    ys = ((np.arange(H, dtype=np.float32) + np.float32(0.5)) / np.float32(H) - np.float32(0.5)) * F
+    d = (xs[None, :, None] * right[None, None, :] + ys[:, None, None] * upv[None, None, :]
+         - V * back[None, None, :]).astype(np.float32)
+    d = nrm(d.reshape(-1, 3)).astype(np.float32)
+    o = np.broadcast_to(eye, d.shape).astype(np.float32)
+    return np.ascontiguousarray(o), np.ascontiguousarray(d)
+
+
+def incoherent_rays(n, bmin, bmax, seed=2):
+    """Origins uniform in [bmin, bmax], directions uniform on the sphere."""
+    rng = np.random.default_rng(seed)
+    o = rng.uniform(bmin, bmax, size=(n, 3)).astype(np.float32)
+    z = rng.uniform(-1, 1, n)
+    phi = rng.uniform(0, 2 * np.pi, n)
+    r = np.sqrt(np.maximum(0, 1 - z * z))
+    d = np.stack([r * np.cos(phi), r * np.sin(phi), z], 1).astype(np.float32)
+    return o, d
+
+
+def closest_hit_rays(mesh, W=4096, H=2048, n_incoherent=None, seed=2):
+    """Coherent primary rays from a W x H pinhole at (0,0,-3.5) looking at the
+    origin (40 deg), plus as many incoherent rays inside the mesh's AABB."""
+    oc, dc = pinhole_rays((0, 0, -3.5), (0, 0, 0), 40.0, W, H)
+    lo = np.minimum(np.minimum(mesh["v0"], mesh["v1"]), mesh["v2"]).min(0)
+    hi = np.maximum(np.maximum(mesh["v0"], mesh["v1"]), mesh["v2"]).max(0)
+    n_inc = W * H if n_incoherent is None else n_incoherent
+    oi, di = incoherent_rays(n_inc, lo, hi, seed)
+    return (oc, dc), (oi, di)
