@@ -30,6 +30,9 @@ def parse():
     ap.add_argument("--cpu-sample-stride", type=int, default=16,
                     help="the CPU baseline traces every n-th ray of each batch")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a 256 MiB device-to-device copy (the reference dispatch "
+                         "that scales rocprof FETCH_SIZE / WRITE_SIZE readings)")
     return ap.parse_args()
 
 
@@ -44,12 +47,13 @@ def main():
               "(the closest-hit batch would be split across GPUs with a replicated tree).",
               file=sys.stderr)
         sys.exit(2)
-    import torch
     from xyz_amd import build as xbuild
-    from xyz_amd import capi, scenes
+    from xyz_amd import capi, hip, scenes
     if not os.path.exists(xbuild.LIB):
         xbuild.build()
-    dev = torch.device("cuda:0")
+    if hip.device_count() < 1:
+        print("bench.py: no HIP device", file=sys.stderr)
+        sys.exit(1)
 
     t0 = time.time()
     mesh = scenes.closest_hit_mesh(8, 64, seed=1)
@@ -62,39 +66,48 @@ def main():
     log(f"[bench] kd-tree: {info['n_nodes']} nodes, {info['n_leaf_refs']} refs, "
         f"build {info['build_seconds']:.2f}s, device {info['device_bytes'] / 2**20:.1f} MiB")
 
-    o = torch.from_numpy(np.concatenate([oc, oi])).to(dev)
-    d = torch.from_numpy(np.concatenate([dc, di])).to(dev)
+    o = hip.DeviceArray.from_numpy(np.concatenate([oc, oi]))
+    d = hip.DeviceArray.from_numpy(np.concatenate([dc, di]))
     n = o.shape[0]
-    prim = torch.empty(n, dtype=torch.int32, device=dev)
-    t = torch.empty(n, dtype=torch.float32, device=dev)
-    stream = torch.cuda.current_stream()
+    prim = hip.DeviceArray(n, np.int32)
+    t = hip.DeviceArray(n, np.float32)
+    stream = None  # the null stream: kernels and events below are on the same stream
 
     # exact traversal counts of this ray set (identical to the oracle's traversal)
-    cnt = torch.empty(2 * n, dtype=torch.int32, device=dev)
+    cnt = hip.DeviceArray(2 * n, np.uint32)
     scene.closest_hit(o, d, prim, t, counters=cnt, stream=stream)
-    torch.cuda.synchronize()
-    c = cnt.cpu().numpy().view(np.uint32).astype(np.int64)
+    c = cnt.numpy().astype(np.int64)
     nodes_sum, tris_sum = int(c[:n].sum()), int(c[n:].sum())
     del cnt
     alg_bytes = 32 * n + 8 * nodes_sum + 40 * tris_sum
-    hit_rate = float((prim.cpu().numpy() >= 0).mean())
+    hit_rate = float((prim.numpy() >= 0).mean())
+
+    if args.copy_calibration:
+        import ctypes
+        src = hip.DeviceArray(64 << 20, np.float32)
+        dst = hip.DeviceArray(64 << 20, np.float32)
+        src.zero()
+        hip.synchronize()
+        hip.rt().hipMemcpy(dst.ptr, src.ptr, ctypes.c_size_t(src.nbytes), hip.D2D)
+        hip.synchronize()
+        del src, dst
 
     def one(oo, dd, pp, tt):
         scene.closest_hit(oo, dd, pp, tt, stream=stream)
 
     for _ in range(args.warmup):
         one(o, d, prim, t)
-    torch.cuda.synchronize()
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    hip.synchronize()
+    starts = [hip.Event() for _ in range(args.steps)]
+    ends = [hip.Event() for _ in range(args.steps)]
     t_wall0 = time.perf_counter()
     for k in range(args.steps):
         starts[k].record(stream)
         one(o, d, prim, t)
         ends[k].record(stream)
-    torch.cuda.synchronize()
+    hip.synchronize()
     wall = time.perf_counter() - t_wall0
-    ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    ms = [s.elapsed_ms(e) for s, e in zip(starts, ends)]
     ms_avg = float(np.mean(ms))
     mrays = n / (ms_avg * 1e-3) / 1e6
     log(f"[bench] {args.steps} steps: {ms_avg:.3f} ms/launch (min {min(ms):.3f}), "
@@ -102,25 +115,20 @@ def main():
 
     # coherent / incoherent split (same kernel, separate launches)
     split = {}
-    for name, sl in (("coherent", slice(0, n_c)), ("incoherent", slice(n_c, n))):
-        oo, dd = o[sl], d[sl]
-        pp, tt = prim[sl], t[sl]
+    for name, (a, b) in (("coherent", (0, n_c)), ("incoherent", (n_c, n))):
+        oo, dd, pp, tt = o.rows(a, b), d.rows(a, b), prim.rows(a, b), t.rows(a, b)
         one(oo, dd, pp, tt)
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
+        e0, e1 = hip.Event(), hip.Event()
         reps = max(3, args.steps // 2)
         e0.record(stream)
         for _ in range(reps):
             one(oo, dd, pp, tt)
         e1.record(stream)
-        torch.cuda.synchronize()
-        ms_s = e0.elapsed_time(e1) / reps
-        nn = sl.stop - sl.start
-        cs = c[sl]
-        ci = c[n + sl.start:n + sl.stop]
+        ms_s = e0.elapsed_ms(e1) / reps
+        nn = b - a
         split[name] = dict(mrays_per_s=round(nn / (ms_s * 1e-3) / 1e6, 1), ms=round(ms_s, 3),
-                           nodes_per_ray=round(float(cs.mean()), 2),
-                           tris_per_ray=round(float(ci.mean()), 2))
+                           nodes_per_ray=round(float(c[a:b].mean()), 2),
+                           tris_per_ray=round(float(c[n + a:n + b].mean()), 2))
     log(f"[bench] split: {split}")
 
     achieved = alg_bytes / (ms_avg * 1e-3) / 1e9
@@ -144,11 +152,10 @@ def main():
         r = osc.closest_hit(so, sd, threads=threads)
         secs = r["seconds"]
         # bit-exact spot check of the same rays on the GPU
-        gp = torch.empty(len(so), dtype=torch.int32, device=dev)
-        scene.closest_hit(torch.from_numpy(so).to(dev), torch.from_numpy(sd).to(dev), gp,
+        gp = hip.DeviceArray(len(so), np.int32)
+        scene.closest_hit(hip.DeviceArray.from_numpy(so), hip.DeviceArray.from_numpy(sd), gp,
                           stream=stream)
-        torch.cuda.synchronize()
-        same = bool(np.array_equal(gp.cpu().numpy(), r["prim"]))
+        same = bool(np.array_equal(gp.numpy(), r["prim"]))
         cpu = dict(value=round(len(so) / secs / 1e6, 3), unit="Mrays/s", cores=threads,
                    kind="port",
                    sample=f"every {stride}th ray of both batches ({len(so)} rays) on the full "

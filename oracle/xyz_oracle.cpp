@@ -1045,8 +1045,9 @@ double orc_render(void* s, const float* cam, uint32_t W, uint32_t H, uint32_t sp
   return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
-// Reference gradient of the image loss sum_p w_p . I_p, where I_p is the mean of
-// `spp` samples of pixel p and w = d_film (W*H*3), with respect to every texel
+// Reference gradient of the image loss sum_p w_p . S_p, where S_p is the SUM of
+// the `spp` sample colours of pixel p (the film) and w = d_film (W*H*3) =
+// dLoss/dS, with respect to every texel
 // of every texture (d_tex, concatenated like the scene's texels) and to each
 // BSDF's emission L_e (d_emit, 3 per BSDF; non-zero for lights only).
 // Each sample's colour is a sum of terms; a term added after Lambert vertex i
@@ -1071,7 +1072,6 @@ double orc_render_backward(void* s, const float* cam, uint32_t W, uint32_t H, ui
   auto t0 = std::chrono::steady_clock::now();
   std::vector<std::vector<double>> gt(size_t(nt), std::vector<double>(tex_off.back(), 0.0));
   std::vector<std::vector<double>> ge(size_t(nt), std::vector<double>(sc.bsdfs.size() * 3, 0.0));
-  double const inv_spp = 1.0 / double(spp);
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
   for (int64_t y = 0; y < int64_t(H); ++y) {
 #ifdef _OPENMP
@@ -1083,7 +1083,7 @@ double orc_render_backward(void* s, const float* cam, uint32_t W, uint32_t H, ui
     std::vector<V3> lc(sc.bsdfs.size());
     for (uint32_t x = 0; x < W; ++x) {
       size_t const p = size_t(y) * W + x;
-      double const w[3] = {d_film[3 * p] * inv_spp, d_film[3 * p + 1] * inv_spp, d_film[3 * p + 2] * inv_spp};
+      double const w[3] = {d_film[3 * p], d_film[3 * p + 1], d_film[3 * p + 2]};
       for (uint32_t smp = sample0; smp < sample0 + spp; ++smp) {
         rec.clear();
         for (V3& v : lc) v = V3(0, 0, 0);

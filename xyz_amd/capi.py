@@ -6,8 +6,8 @@ This is the host-side mirror of the reference's interfaces for the hot path:
 materials as `Bsdf` records (xyz/bsdf/bsdf.hpp:21-28) and the progressive
 `PathTracer` loop (xyz/integrator/integrator.cpp:88-162).
 
-Device buffers are torch tensors on a HIP device (torch is only plumbing for
-device memory and streams here); every computation runs in libxyz_hip.so.
+Device buffers are xyz_amd.hip.DeviceArray objects (plain HIP allocations);
+every computation runs in libxyz_hip.so.
 The library must be built (xyz_amd.build.build()); nothing falls back to a
 CPU implementation.
 """
@@ -144,23 +144,19 @@ def _ptr(a, t):
 
 
 def _dptr(t):
-    """Device pointer of a torch tensor (or None)."""
+    """Device pointer of a DeviceArray (xyz_amd.hip) or None."""
     if t is None:
         return None
-    if not t.is_cuda:
-        raise ValueError("device buffer expected (torch tensor on a HIP device)")
-    if not t.is_contiguous():
-        raise ValueError("contiguous device buffer expected")
     return ctypes.c_void_p(t.data_ptr())
 
 
 def _stream(stream):
+    """hipStream_t as void*: None selects the null stream."""
     if stream is None:
-        import torch
-        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        return ctypes.c_void_p(None)
     if isinstance(stream, int):
         return ctypes.c_void_p(stream)
-    return ctypes.c_void_p(stream.cuda_stream)
+    return ctypes.c_void_p(stream.value if isinstance(stream, ctypes.c_void_p) else stream)
 
 
 class KDTree:
@@ -273,8 +269,9 @@ class Scene:
 
     def closest_hit(self, org, dir, prim, t=None, f=None, g=None, back=None, counters=None,
                     stream=None):
-        """FindIntersection for a batch (device tensors: org/dir float32 (n,3),
-        prim int32 (n), optional t/f/g float32, back uint8, counters uint32 (2n))."""
+        """FindIntersection for a batch (device arrays: org/dir float32 (n,3),
+        prim int32 (n), optional t/f/g float32, back uint8, counters uint32 (2n):
+        nodes visited in [0, n), triangles tested in [n, 2n))."""
         n = org.shape[0]
         check(lib().xyz_closest_hit(self.h, n, _dptr(org), _dptr(dir), _dptr(prim), _dptr(t),
                                     _dptr(f), _dptr(g), _dptr(back), _dptr(counters),
@@ -282,7 +279,8 @@ class Scene:
 
     def render(self, camera, film, spp, max_depth, rr_start=0, rr_threshold=0.2, seed=0,
                sample_offset=0, stream=None):
-        """Adds the per-pixel SUM of `spp` samples to `film` (device float32 (H, W, 3))."""
+        """Adds the per-pixel SUM of `spp` samples to `film` (device float32 (H, W, 3),
+        row 0 = bottom)."""
         cam = make_camera(camera, film.shape[1], film.shape[0])
         p = make_params(spp, max_depth, rr_start, rr_threshold, seed, sample_offset)
         st = RenderStats()

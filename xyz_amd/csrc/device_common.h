@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/xyz_hip.h"
+
 namespace xyzhip {
 
 // Per-material record on the device.
@@ -74,14 +76,34 @@ __device__ __forceinline__ float sel3(float3 v, uint32_t a) {
   return a == 0 ? v.x : (a == 1 ? v.y : v.z);
 }
 
+// Recomputes what the traversal decided at interior node `w`: the distance e to
+// its split plane and its far child (xyz/geom/kdtree.cpp:110-122).
+__device__ __forceinline__ float split_distance(uint2 w, float3 o, float3 d, uint32_t node,
+                                                uint32_t* farc, uint32_t* nearc) {
+  uint32_t const axis = w.x & 3u;
+  float const split = __uint_as_float(w.y);
+  float const oa = sel3(o, axis);
+  float const da = sel3(d, axis);
+  bool const i = oa < split;
+  uint32_t const left = node + 1u, right = w.x >> 2;
+  *nearc = i ? left : right;
+  *farc = i ? right : left;
+  return (split - oa) / da;
+}
+
 // Closest-hit query, the reference's recursive traversal (xyz/geom/kdtree.cpp:82-150,
-// 366-372) run iteratively.  The stack holds (far node, e) pairs; t_max after a
-// pop is the e of the new top (or +inf) because only pushes change t_max before
-// a hit ends the query.  `stk` points at this lane's first stack slot; slots are
-// `stride` uint2 apart (LDS, interleaved by lane).
+// 366-372) run iteratively.
+//
+// Stack: one 4-byte entry per pending far child, holding the interior node whose
+// far child is pending.  Only pushes change t_max before a hit ends the query,
+// so t_max always equals the split distance e of the top entry (+inf when the
+// stack is empty).  A pop therefore sets t_min = t_max (the popped entry's e)
+// and recomputes t_max from the new top; both recomputations repeat the same
+// IEEE division and give the same bits.  `stk` points at this lane's first
+// stack slot; slots are `stride` words apart (LDS, interleaved by lane).
 template <bool kCount>
 __device__ __forceinline__ HitRec trace_closest(const DevScene& s, float3 o, float3 d,
-                                                uint2* stk, uint32_t stride,
+                                                uint32_t* stk, uint32_t stride,
                                                 uint32_t* cnt_nodes, uint32_t* cnt_tris) {
   const float kInf = __builtin_inff();
   float t_min = 1e-5f;  // Intersection::Init, xyz/geom/intersection.hpp:11-14
@@ -98,23 +120,16 @@ __device__ __forceinline__ HitRec trace_closest(const DevScene& s, float3 o, flo
   for (;;) {
     uint2 w = s.nodes[node];
     if (kCount) nn++;
-    // descend to a leaf: xyz/geom/kdtree.cpp:82-122
+    // descend to a leaf: xyz/geom/kdtree.cpp:82-108
     while ((w.x & 3u) != 3u) {
-      uint32_t const axis = w.x & 3u;
-      float const split = __uint_as_float(w.y);
-      float const oa = sel3(o, axis);
-      float const da = sel3(d, axis);
-      float const e = (split - oa) / da;
-      bool const i = oa < split;
-      uint32_t const left = node + 1u, right = w.x >> 2;
-      uint32_t const nearc = i ? left : right;
-      uint32_t const farc = i ? right : left;
+      uint32_t farc, nearc;
+      float const e = split_distance(w, o, d, node, &farc, &nearc);
       if ((e > t_max) || (e < 0.0f)) {
         node = nearc;
       } else if (e < t_min) {
         node = farc;
       } else {
-        stk[sp * stride] = make_uint2(farc, __float_as_uint(e));
+        stk[sp * stride] = node;
         sp++;
         t_max = e;
         node = nearc;
@@ -122,35 +137,45 @@ __device__ __forceinline__ HitRec trace_closest(const DevScene& s, float3 o, flo
       w = s.nodes[node];
       if (kCount) nn++;
     }
-    // leaf: every triangle in stored order, last accept wins (kdtree.cpp:140-150)
+    // leaf: every triangle in stored order, last accept wins (kdtree.cpp:140-150).
+    // The next record's loads are issued before the current one is tested.
     uint32_t const cnt = w.x >> 2;
     uint32_t const off = w.y;
     int32_t hit = -1;
-    for (uint32_t j = 0; j < cnt; ++j) {
-      if (kCount) nt++;
-      float4 const A = s.triA[off + j];
-      // xyz/geom/triangle.cpp:127-165 (FindIntersectionDF)
-      float4 const B = s.triB[off + j];
-      float4 const C = s.triC[off + j];
-      float const nx = A.w, ny = B.w, nz = C.w;
-      float const det = d.x * nx + d.y * ny + d.z * nz;
-      if (fabsf(det) < 1e-6f) continue;
-      float const inv = 1.0f / det;
-      float const mx = A.x - o.x, my = A.y - o.y, mz = A.z - o.z;
-      float const t = inv * (mx * nx + my * ny + mz * nz);
-      if ((t < t_min) || (t_max < t)) continue;
-      float const qx = d.y * mz - d.z * my;
-      float const qy = d.z * mx - d.x * mz;
-      float const qz = d.x * my - d.y * mx;
-      float const f = inv * (C.x * qx + C.y * qy + C.z * qz);
-      if ((f < 0.0f) || (f > 1.0f)) continue;
-      float const g = inv * -(B.x * qx + B.y * qy + B.z * qz);
-      if ((g < 0.0f) || ((f + g) > 1.0f)) continue;
-      t_max = t;
-      h.f = f;
-      h.g = g;
-      h.back = det > 0.0f;
-      hit = int32_t(off + j);
+    if (cnt > 0) {
+      float4 A = s.triA[off], B = s.triB[off], C = s.triC[off];
+      for (uint32_t j = 0; j < cnt; ++j) {
+        uint32_t const jn = (j + 1 < cnt) ? j + 1 : j;
+        float4 const nA = s.triA[off + jn], nB = s.triB[off + jn], nC = s.triC[off + jn];
+        if (kCount) nt++;
+        // xyz/geom/triangle.cpp:127-165 (FindIntersectionDF)
+        float const nx = A.w, ny = B.w, nz = C.w;
+        float const det = d.x * nx + d.y * ny + d.z * nz;
+        if (!(fabsf(det) < 1e-6f)) {
+          float const inv = 1.0f / det;
+          float const mx = A.x - o.x, my = A.y - o.y, mz = A.z - o.z;
+          float const t = inv * (mx * nx + my * ny + mz * nz);
+          if (!((t < t_min) || (t_max < t))) {
+            float const qx = d.y * mz - d.z * my;
+            float const qy = d.z * mx - d.x * mz;
+            float const qz = d.x * my - d.y * mx;
+            float const f = inv * (C.x * qx + C.y * qy + C.z * qz);
+            if (!((f < 0.0f) || (f > 1.0f))) {
+              float const g = inv * -(B.x * qx + B.y * qy + B.z * qz);
+              if (!((g < 0.0f) || ((f + g) > 1.0f))) {
+                t_max = t;
+                h.f = f;
+                h.g = g;
+                h.back = det > 0.0f;
+                hit = int32_t(off + j);
+              }
+            }
+          }
+        }
+        A = nA;
+        B = nB;
+        C = nC;
+      }
     }
     if (hit >= 0) {
       h.prim = s.ref_prim[hit];
@@ -158,11 +183,20 @@ __device__ __forceinline__ HitRec trace_closest(const DevScene& s, float3 o, flo
       break;
     }
     if (sp == 0) break;
+    // pop: continue at the far child of the top entry with [e, e_below]
     sp--;
-    uint2 const top = stk[sp * stride];
-    node = top.x;
-    t_min = __uint_as_float(top.y);
-    t_max = (sp > 0) ? __uint_as_float(stk[(sp - 1) * stride].y) : kInf;
+    uint32_t const p = stk[sp * stride];
+    uint32_t farc, nearc;
+    float const e = split_distance(s.nodes[p], o, d, p, &farc, &nearc);
+    node = farc;
+    t_min = e;
+    if (sp > 0) {
+      uint32_t const pb = stk[(sp - 1) * stride];
+      uint32_t f2, n2;
+      t_max = split_distance(s.nodes[pb], o, d, pb, &f2, &n2);
+    } else {
+      t_max = kInf;
+    }
   }
   if (kCount) {
     *cnt_nodes = nn;

@@ -1,0 +1,469 @@
+// Device functions of the unidirectional path tracer, shared by the wavefront
+// renderer, the path-replay adjoint and the persistent batched kernel.
+//
+// Every function restates one piece of the reference's integrator in the same
+// binary32 operation order (double where the reference promotes via M_1_PI or
+// 0.5 literals), so that a path traced here is bit-identical to the same path
+// traced on the CPU in that order.
+#ifndef XYZ_PT_DEVICE_H_
+#define XYZ_PT_DEVICE_H_
+
+#include "device_common.h"
+
+namespace xyzhip {
+
+constexpr double kInvPi = 0.31830988618379067154;  // M_1_PI
+constexpr double kPi = 3.14159265358979323846;
+
+// Camera basis and lens-film distance, computed on the host
+// (xyz/core/thinlenscamera.cpp:71-90 with an explicit vertical FOV).
+struct DevCamera {
+  float eye[3], right[3], up[3], back[3];
+  float film, aspect, v;
+  uint32_t width, height;
+};
+
+struct PTConst {
+  uint32_t kmax;        // XYZ_CONFIG_kMaxRandomWalkDepth = max_depth + 1
+  uint32_t rr_start;    // Russian roulette at vertices j > rr_start
+  float rr_threshold;   // CONFIG_TERMINATE_THRESHOLD
+  uint32_t seed_lo, seed_hi;
+};
+
+// ---------------------------------------------------------------- vectors
+__device__ __forceinline__ float3 f3(float x, float y, float z) { return make_float3(x, y, z); }
+__device__ __forceinline__ float3 add3(float3 a, float3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ float3 sub3(float3 a, float3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ float3 mul3(float3 a, float3 b) { return f3(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ float3 scl3(float3 a, float s) { return f3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ float3 scl3l(float s, float3 a) { return f3(s * a.x, s * a.y, s * a.z); }
+__device__ __forceinline__ float3 neg3(float3 a) { return f3(-a.x, -a.y, -a.z); }
+// libhi/hi/math/vector3.hpp:119-131
+__device__ __forceinline__ float dot3(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float3 cross3(float3 a, float3 b) {
+  return f3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+// libhi/hi/math/math.hpp:146-157,254-257: normalize(a) = a * (1 / sqrt(dot(a, a)))
+__device__ __forceinline__ float rcpf(float x) { return 1.0f / x; }
+__device__ __forceinline__ float3 normalize3(float3 a) { return scl3(a, 1.0f / sqrtf(dot3(a, a))); }
+__device__ __forceinline__ float3 ld3(const float4& v) { return f3(v.x, v.y, v.z); }
+
+// ------------------------------------------------------- deterministic cos/sin
+// cos/sin of a float angle in double precision: two-part pi/2 reduction and
+// Taylor series (degree 21/22).  The same operations run on the CPU oracle.
+__device__ __forceinline__ double taylor_sin(double r) {
+  double const r2 = r * r;
+  double term = r, sum = r;
+#pragma unroll
+  for (int k = 1; k <= 10; ++k) {
+    term = term * (-r2) / double((2 * k) * (2 * k + 1));
+    sum = sum + term;
+  }
+  return sum;
+}
+__device__ __forceinline__ double taylor_cos(double r) {
+  double const r2 = r * r;
+  double term = 1.0, sum = 1.0;
+#pragma unroll
+  for (int k = 1; k <= 11; ++k) {
+    term = term * (-r2) / double((2 * k - 1) * (2 * k));
+    sum = sum + term;
+  }
+  return sum;
+}
+__device__ __forceinline__ void cos_sin(float a, float* c, float* s) {
+  double const x = double(a);
+  double const k = floor(x * 0.63661977236758134308 + 0.5);
+  double const P1 = 1.5707963267341256141662597656;
+  double const P1t = 6.07710050650619224932e-11;
+  double const r = (x - k * P1) - k * P1t;
+  long long const ki = (long long)k;
+  int const q = int(((ki % 4) + 4) % 4);
+  double const ts = taylor_sin(r), tc = taylor_cos(r);
+  double cv, sv;
+  switch (q) {
+    case 0: cv = tc; sv = ts; break;
+    case 1: cv = -ts; sv = tc; break;
+    case 2: cv = -tc; sv = -ts; break;
+    default: cv = ts; sv = -tc; break;
+  }
+  *c = float(cv);
+  *s = float(sv);
+}
+
+// ----------------------------------------------------------- Philox4x32-10
+// Salmon et al., SC'11.  key = (pixel, seed_lo), counter = (sample, bounce, seed_hi, 0).
+__device__ __forceinline__ uint4 philox(uint4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    uint32_t const hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
+    uint32_t const hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+    c = make_uint4(hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0);
+  }
+  return c;
+}
+__device__ __forceinline__ float u01(uint32_t x) { return float(x >> 8) * (1.0f / 16777216.0f); }
+__device__ __forceinline__ float4 draw4(const PTConst& c, uint32_t pixel, uint32_t sample,
+                                        uint32_t bounce) {
+  uint4 const r = philox(make_uint4(sample, bounce, c.seed_hi, 0u), pixel, c.seed_lo);
+  return make_float4(u01(r.x), u01(r.y), u01(r.z), u01(r.w));
+}
+
+// ----------------------------------------------------------------- camera
+// xyz/core/thinlenscamera.cpp:111-124 (pinhole) and integrator.cpp:124-125
+__device__ __forceinline__ float3 primary_dir(const DevCamera& cam, uint32_t px, uint32_t py,
+                                              float jx, float jy) {
+  float const s = (float(int(px)) + jx) / float(int(cam.width));
+  float const t = (float(int(py)) + jy) / float(int(cam.height));
+  float const lx = float((double(s) - 0.5) * double(cam.film) * double(cam.aspect));
+  float const ly = float((double(t) - 0.5) * double(cam.film));
+  float const lz = -cam.v;
+  float3 const r = f3(cam.right[0], cam.right[1], cam.right[2]);
+  float3 const u = f3(cam.up[0], cam.up[1], cam.up[2]);
+  float3 const b = f3(cam.back[0], cam.back[1], cam.back[2]);
+  return normalize3(add3(add3(scl3l(lx, r), scl3l(ly, u)), scl3l(lz, b)));
+}
+
+// ------------------------------------------------------------ light sample
+struct LightVertex {
+  int32_t prim;  // -1: no light in the scene
+  float3 pos, ns, ng;
+  float3 power;  // Phi of the sampled light
+  uint32_t bsdf;
+};
+
+// xyz/core/scene.cpp:42-78 and xyz/geom/triangle.cpp:232-263
+__device__ __forceinline__ LightVertex sample_light(const DevScene& s, float u0, float u1) {
+  LightVertex lv;
+  if (s.n_lights == 0) {
+    lv.prim = -1;
+    lv.pos = lv.ns = lv.ng = lv.power = f3(0.f, 0.f, 0.f);
+    lv.bsdf = 0;
+    return lv;
+  }
+  float const total = s.light_cdf[s.n_lights];
+  float const u = u0 * total;
+  // std::upper_bound over cdf[0..n]: first index with cdf[i] > u
+  uint32_t lo = 0, hi = s.n_lights + 1;
+  while (lo < hi) {
+    uint32_t const mid = (lo + hi) >> 1;
+    if (u < s.light_cdf[mid]) hi = mid; else lo = mid + 1;
+  }
+  uint32_t n = lo - 1;
+  if (n >= s.n_lights) n = s.n_lights - 1;
+  float const v = (u - s.light_cdf[n]) / (s.light_cdf[n + 1] - s.light_cdf[n]);
+  uint32_t const p = s.light_prim[n];
+  float3 const v0 = ld3(s.prim_v0[p]), e1 = ld3(s.prim_e1[p]), e2 = ld3(s.prim_e2[p]);
+  float const ss = sqrtf(v);
+  float const tt = u1;
+  float const uu = ss * (1.0f - tt);
+  float const vv = ss * tt;
+  lv.prim = int32_t(p);
+  lv.pos = add3(add3(v0, scl3l(uu, e1)), scl3l(vv, e2));
+  float4 const ngb = s.prim_ng[p];
+  lv.ng = ld3(ngb);
+  if (s.prim_n0) {
+    float3 const n0 = ld3(s.prim_n0[p]), s1 = ld3(s.prim_s1[p]), s2 = ld3(s.prim_s2[p]);
+    lv.ns = normalize3(add3(add3(n0, scl3l(uu, s1)), scl3l(vv, s2)));
+  } else {
+    float3 const z = f3(0.f, 0.f, 0.f);
+    lv.ns = normalize3(add3(add3(lv.ng, scl3l(uu, z)), scl3l(vv, z)));
+  }
+  lv.bsdf = __float_as_uint(ngb.w);
+  DevBsdf const& b = s.bsdfs[lv.bsdf];
+  lv.power = f3(b.phi[0], b.phi[1], b.phi[2]);
+  return lv;
+}
+
+// ------------------------------------------------------------ shading basis
+struct Basis {
+  float3 ns, ng, t, b;
+};
+
+// xyz/geom/triangle.cpp:194-219
+__device__ __forceinline__ Basis shading_basis(const DevScene& s, int32_t prim, float f, float g,
+                                               bool back, uint32_t* bsdf) {
+  float4 const ngb = s.prim_ng[prim];
+  *bsdf = __float_as_uint(ngb.w);
+  float3 n0, s1, s2;
+  if (s.prim_n0) {
+    n0 = ld3(s.prim_n0[prim]);
+    s1 = ld3(s.prim_s1[prim]);
+    s2 = ld3(s.prim_s2[prim]);
+  } else {
+    n0 = ld3(ngb);
+    s1 = s2 = f3(0.f, 0.f, 0.f);
+  }
+  Basis B;
+  B.ns = normalize3(add3(add3(n0, scl3l(f, s1)), scl3l(g, s2)));
+  B.ng = ld3(ngb);
+  if (back) {
+    B.ng = neg3(B.ng);
+    B.ns = neg3(B.ns);
+  }
+  float3 t = f3(-B.ns.z, 0.0f, B.ns.x);
+  float const l2 = dot3(t, t);
+  if (l2 > 0.0f) {
+    t = scl3(t, 1.0f / sqrtf(l2));
+    B.b = cross3(t, B.ns);
+  } else {
+    B.b = f3(0.0f, 0.0f, (B.ns.y > 0.0f) ? 1.0f : -1.0f);
+    t = cross3(B.ns, B.b);
+  }
+  B.t = t;
+  return B;
+}
+
+// Lambert albedo, optionally from a texture at the hit's uv (nearest texel).
+__device__ __forceinline__ float3 albedo_at(const DevScene& s, const DevBsdf& b, int32_t prim,
+                                            float f, float g, int32_t* texel) {
+  *texel = -1;
+  if (b.texture < 0) return f3(b.rgb[0], b.rgb[1], b.rgb[2]);
+  DevTex const tx = s.tex[b.texture];
+  float2 const a = s.prim_uv[3 * prim], c1 = s.prim_uv[3 * prim + 1], c2 = s.prim_uv[3 * prim + 2];
+  float const u = (a.x + f * (c1.x - a.x)) + g * (c2.x - a.x);
+  float const v = (a.y + f * (c1.y - a.y)) + g * (c2.y - a.y);
+  int ix = int(u * float(tx.w));
+  int iy = int(v * float(tx.h));
+  ix = ix < 0 ? 0 : (ix >= int(tx.w) ? int(tx.w) - 1 : ix);
+  iy = iy < 0 ? 0 : (iy >= int(tx.h) ? int(tx.h) - 1 : iy);
+  int const idx = iy * int(tx.w) + ix;
+  *texel = idx;
+  const float* p = s.texels + tx.offset + 3 * size_t(idx);
+  return f3(p[0], p[1], p[2]);
+}
+
+// xyz/stdafx.hpp:44-47, libhi/hi/math/math.hpp:41-44
+__device__ __forceinline__ float fake_fresnel(float hk) {
+  float const fresnel0 = 0.9f;
+  float const x = 1.0f - hk;
+  float const x2 = x * x;
+  return fresnel0 + (1.0f - fresnel0) * (x * (x2 * x2));
+}
+
+// --------------------------------------------------------------- path state
+struct PathState {
+  float3 pos, dir, power, color;
+  float f_next;
+  uint32_t k;     // the reference's vertex counter (starts at 1)
+  bool specular;
+};
+
+// An NEE visibility query and what is needed to finish its contribution.
+struct ShadowReq {
+  float3 o, w;       // from the light sample toward the shading point
+  int32_t expected;  // the shading point's triangle
+  uint32_t type;     // LAMBERT or METAL
+  uint32_t light_bsdf;
+  float nk2, nk3;
+  float3 pe;         // light power * path power
+  float fsn, fres, mx;  // METAL only
+};
+
+// Contribution of a visible NEE sample (xyz/bsdf/lambert.cpp:157-172,
+// xyz/bsdf/metal.cpp:100-116), given the shadow ray's hit distance t.
+__device__ __forceinline__ float3 nee_contribution(const ShadowReq& r, float t, float A) {
+  if (r.type == XYZ_LAMBERT) {
+    float const G = r.nk2 * r.nk3 / (t * t);
+    float const p_imp = float(kInvPi * double(G));
+    float const p_exp = rcpf(A);
+    float const dens = p_imp / p_exp;
+    double const wgt = (double(G) * (kInvPi * kInvPi)) * double(rcpf(1.0f + dens * dens));
+    return f3(float(double(r.pe.x) * wgt), float(double(r.pe.y) * wgt),
+              float(double(r.pe.z) * wgt));
+  }
+  float const wg = r.nk3 / (t * t);
+  float const G = r.nk2 * wg;
+  float const dens = r.fsn * wg * A;
+  double const s1 = double(G) * kInvPi;
+  double const mx = double(r.mx);
+  double const rw = double(rcpf(1.0f + dens * dens));
+  double const fr = double(r.fres);
+  double const fs = double(r.fsn);
+  return f3(float(((((double(r.pe.x) * s1) * fs) / mx) * fr) * rw),
+            float(((((double(r.pe.y) * s1) * fs) / mx) * fr) * rw),
+            float(((((double(r.pe.z) * s1) * fs) / mx) * fr) * rw));
+}
+
+// Gradient hooks of the path-replay adjoint; the primal uses NoGrad.
+struct NoGrad {
+  __device__ __forceinline__ void albedo(int32_t, int32_t, float3, float3) {}
+  __device__ __forceinline__ void emission(uint32_t, float3) {}
+};
+
+// One vertex of Scene::EvalutePathTracing (xyz/core/scene_pathtracing.cpp:52-142)
+// after the extension ray found `h`.  Updates the path; emits at most one
+// shadow request.  Returns false when the path ends.
+template <class Grad>
+__device__ __forceinline__ bool shade_vertex(const DevScene& s, const PTConst& c, PathState& ps,
+                                             const HitRec& h, uint32_t pixel, uint32_t sample,
+                                             const LightVertex& lv, ShadowReq* sr, bool* has_sr,
+                                             Grad& grad) {
+  *has_sr = false;
+  if (h.prim < 0) return false;  // through the air
+  uint32_t bsdf_id;
+  Basis const B = shading_basis(s, h.prim, h.f, h.g, h.back, &bsdf_id);
+  if (-dot3(ps.dir, B.ns) <= 0.0f) return false;
+  ps.pos = add3(ps.pos, scl3(ps.dir, h.t));
+  DevBsdf const& b = s.bsdfs[bsdf_id];
+  float const A = s.light_area;
+  if (b.type == XYZ_LIGHT && !h.back) {
+    float3 const pe = mul3(f3(b.phi[0], b.phi[1], b.phi[2]), ps.power);
+    float3 add;
+    if (ps.specular) {
+      double const w = kInvPi / double(A);
+      add = f3(float(double(pe.x) * w), float(double(pe.y) * w), float(double(pe.z) * w));
+    } else {
+      float const cos_t = fabsf(dot3(ps.dir, B.ns));
+      float const dist2 = h.t * h.t;
+      float const p_imp = ps.f_next * cos_t / dist2;
+      float const p_exp = rcpf(A);
+      float const dens = p_exp / p_imp;
+      double const w1 = kInvPi / double(A);
+      double const w2 = double(rcpf(1.0f + dens * dens));
+      add = f3(float(double(pe.x) * w1 * w2), float(double(pe.y) * w1 * w2),
+               float(double(pe.z) * w1 * w2));
+    }
+    grad.emission(bsdf_id, add);
+    ps.color = add3(ps.color, add);
+    return false;
+  }
+  if (++ps.k >= c.kmax) return false;
+  uint32_t const j = ps.k - 1;
+  float4 const u = draw4(c, pixel, sample, j);
+
+  // NEE: Bsdf::CalculateWeightedDirectLighting
+  if (b.type == XYZ_LAMBERT || b.type == XYZ_METAL) {
+    int32_t texel;
+    float3 const alb = albedo_at(s, b, h.prim, h.f, h.g, &texel);
+    if (b.type == XYZ_LAMBERT && texel >= 0) grad.albedo(b.texture, texel, alb, ps.color);
+    ps.power = mul3(ps.power, alb);
+    if (lv.prim >= 0) {
+      float3 const w = normalize3(sub3(ps.pos, lv.pos));
+      float const nk3 = dot3(w, lv.ng);
+      bool ok = !((nk3 <= 0.0f) || (dot3(w, lv.ns) <= 0.0f));
+      float nk2 = 0.0f;
+      if (ok) {
+        nk2 = -dot3(w, B.ns);
+        ok = !((nk2 <= 0.0f) || (-dot3(w, B.ng) <= 0.0f));
+      }
+      if (ok) {
+        sr->o = lv.pos;
+        sr->w = w;
+        sr->expected = h.prim;
+        sr->type = b.type;
+        sr->light_bsdf = lv.bsdf;
+        sr->nk2 = nk2;
+        sr->nk3 = nk3;
+        sr->pe = mul3(lv.power, ps.power);
+        sr->fsn = sr->fres = sr->mx = 0.0f;
+        if (b.type == XYZ_METAL) {
+          float3 const hv = neg3(normalize3(add3(w, ps.dir)));
+          float const hk = -dot3(hv, w);
+          float const nh = dot3(B.ns, hv);
+          float const nk1 = -dot3(B.ns, ps.dir);
+          sr->fsn = float(double(b.shininess + 1.0f) * (kInvPi / 8) *
+                          double(powf(nh, b.shininess)) / double(hk));
+          sr->fres = fake_fresnel(hk);
+          sr->mx = (nk1 < nk2) ? nk2 : nk1;
+        }
+        *has_sr = true;
+      }
+    }
+  }
+
+  // Russian roulette: xyz/core/scene_pathtracing.cpp:128-137
+  if (j > c.rr_start) {
+    float const t = ps.power.y;
+    if (t < c.rr_threshold) {
+      float const q = t / c.rr_threshold;
+      if (u.x >= q) return false;
+      float const inv = rcpf(q);
+      ps.power = scl3(ps.power, inv);
+    }
+  }
+
+  // Bsdf::NextScatteringDirection
+  if (b.type == XYZ_LAMBERT) {
+    // xyz/bsdf/lambert.cpp:176-187, xyz/stdafx.hpp:50-63
+    float const phi = 6.28318530717958647692f * u.z;
+    float const cos_t = sqrtf(u.y);
+    float const sin_t = sqrtf(1.0f - u.y);
+    float cp, sp;
+    cos_sin(phi, &cp, &sp);
+    ps.dir = add3(add3(scl3(B.t, sin_t * cp), scl3(B.b, sin_t * sp)), scl3(B.ns, cos_t));
+    ps.specular = false;
+    ps.f_next = float(kInvPi * double(cos_t));
+    return dot3(ps.dir, B.ng) > 0.0f;
+  }
+  if (b.type == XYZ_MIRROR) {
+    // xyz/bsdf/mirror.cpp:56-105
+    float const cth = -dot3(ps.dir, B.ns);
+    ps.dir = add3(ps.dir, scl3l(2.0f * cth, B.ns));
+    if (dot3(ps.dir, B.ng) <= 0.0f) return false;
+    ps.specular = true;
+    ps.f_next = 1.0f;
+    return true;
+  }
+  if (b.type == XYZ_GLASS) {
+    // xyz/bsdf/glass.cpp:104-165
+    float const r = h.back ? rcpf(b.ior) : b.ior;
+    float const ior_sq = r * r;
+    float const nk = -dot3(ps.dir, B.ns);
+    float const bb = ior_sq + nk * nk - 1.0f;
+    if (bb > 0.0f) {
+      float const g = sqrtf(bb);
+      float const gpc = g + nk;
+      float const gmc = g - nk;
+      float const alpha = gmc / gpc;
+      float const beta = (nk * gpc - 1.0f) / (nk * gmc + 1.0f);
+      float const fr = 0.5f * alpha * alpha * (beta * beta + 1.0f);
+      if (u.y < fr) {
+        ps.dir = add3(ps.dir, scl3l(2.0f * nk, B.ns));
+        if (dot3(ps.dir, B.ng) <= 0.0f) return false;
+      } else {
+        float const sq = sqrtf(ior_sq);
+        float3 const v = sub3(ps.dir, scl3l(gmc, B.ns));
+        ps.dir = f3(v.x / sq, v.y / sq, v.z / sq);
+        if (dot3(ps.dir, B.ng) >= 0.0f) return false;
+      }
+    } else {
+      ps.dir = add3(ps.dir, scl3l(2.0f * nk, B.ns));
+      if (dot3(ps.dir, B.ng) <= 0.0f) return false;
+    }
+    ps.specular = true;
+    ps.f_next = 1.0f;
+    return true;
+  }
+  if (b.type == XYZ_METAL) {
+    // xyz/bsdf/metal.cpp:122-153, xyz/stdafx.hpp:154-168
+    float const phi = 6.28318530717958647692f * u.z;
+    float const cos_h = powf(u.y, rcpf(b.shininess));
+    float const sin_h = sqrtf(1.0f - cos_h * cos_h);
+    float cp, sp;
+    cos_sin(phi, &cp, &sp);
+    float3 const hv = add3(add3(scl3(B.t, sin_h * cp), scl3(B.b, sin_h * sp)), scl3(B.ns, cos_h));
+    float const nk1 = -dot3(B.ns, ps.dir);
+    float const hk = -dot3(hv, ps.dir);
+    ps.dir = add3(ps.dir, scl3l(2.0f * hk, hv));
+    float const nk2 = dot3(B.ns, ps.dir);
+    if ((nk2 <= 0.0f) || (dot3(B.ng, ps.dir) <= 0.0f)) return false;
+    float const fres = fake_fresnel(hk);
+    float const mx = (nk1 < nk2) ? nk2 : nk1;
+    float const sc = fres / mx * nk2;
+    ps.power = scl3(ps.power, sc);
+    ps.specular = false;
+    ps.f_next = float(double(b.shininess + 1.0f) * (kInvPi / 8) * double(powf(cos_h, b.shininess)) /
+                      double(hk));
+    return true;
+  }
+  return false;  // LIGHT seen from behind: xyz/bsdf/light.cpp:27-31
+}
+
+}  // namespace xyzhip
+
+#endif  // XYZ_PT_DEVICE_H_

@@ -28,20 +28,24 @@ int hip_fail(hipError_t e, const char* what) {
 
 // ------------------------------------------------------------ closest hit
 constexpr uint32_t kHitBlock = 256;
+constexpr uint32_t kXcdChunk = 8;
 
-// One ray per lane; blocks are remapped so that each XCD (blocks are dealt to
-// the 8 XCDs round robin) walks one contiguous range of rays, keeping the
-// kd nodes and triangles that neighbouring (coherent) rays share in one L2.
+// One ray per lane.  Workgroups are dealt to the 8 XCDs round robin; the
+// remap below gives each XCD runs of kXcdChunk consecutive blocks, so rays
+// that are neighbours (coherent rays share kd nodes and leaf triangles) meet
+// in one XCD's L2, while every XCD still gets an even share of every part of
+// the batch (a mixed coherent/incoherent batch stays balanced).
 template <bool kCount>
 __global__ void __launch_bounds__(kHitBlock)
 closest_hit_kernel(DevScene s, uint32_t n, const float* __restrict__ org,
                    const float* __restrict__ dir, int32_t* __restrict__ prim,
                    float* __restrict__ tout, float* __restrict__ fout, float* __restrict__ gout,
                    uint8_t* __restrict__ bout, uint32_t* __restrict__ counters,
-                   uint32_t blocks_per_xcd) {
-  extern __shared__ uint2 lds_stack[];
+                   uint32_t) {
+  extern __shared__ uint32_t lds_stack[];
   uint32_t const b = blockIdx.x;
-  uint32_t const logical = (b & 7u) * blocks_per_xcd + (b >> 3);
+  uint32_t const k = b >> 3;
+  uint32_t const logical = (k / kXcdChunk) * (8 * kXcdChunk) + (b & 7u) * kXcdChunk + (k % kXcdChunk);
   uint32_t const i = logical * kHitBlock + threadIdx.x;
   if (i >= n) return;
   float3 const o = make_float3(org[3 * size_t(i)], org[3 * size_t(i) + 1], org[3 * size_t(i) + 2]);
@@ -64,9 +68,10 @@ int launch_closest_hit(const DevScene& s, uint32_t n, const float* org, const fl
                        uint32_t* counters, hipStream_t stream) {
   if (n == 0) return XYZ_OK;
   uint32_t const blocks = (n + kHitBlock - 1) / kHitBlock;
-  uint32_t const per_xcd = (blocks + 7) / 8;
-  uint32_t const grid = per_xcd * 8;
-  size_t const lds = size_t(s.stack_cap) * kHitBlock * sizeof(uint2);
+  uint32_t const group = 8 * kXcdChunk;
+  uint32_t const grid = (blocks + group - 1) / group * group;
+  uint32_t const per_xcd = grid / 8;
+  size_t const lds = size_t(s.stack_cap) * kHitBlock * sizeof(uint32_t);
   if (counters) {
     hipLaunchKernelGGL(closest_hit_kernel<true>, dim3(grid), dim3(kHitBlock), lds, stream, s, n,
                        org, dir, prim, t, f, g, back, counters, per_xcd);
