@@ -1,0 +1,174 @@
+"""Path-tracing parity on the GPU: the wavefront renderer (and the batched
+megakernel) against the oracle integrator on the same seeds.
+
+Both sides evaluate the same binary32 operations in the same order with the
+same counter-based random numbers, so films are compared bit for bit; METAL's
+lobe uses powf, whose last bit may differ between the host and device libm,
+so METAL scenes are compared statistically."""
+import math
+
+import numpy as np
+import pytest
+
+from xyz_amd import scenes as S
+
+pytestmark = pytest.mark.gpu
+F = np.float32
+LAMBERT, MIRROR, GLASS, METAL, LIGHT = 0, 1, 2, 3, 4
+
+
+def gpu_render(capi_mod, scene, cam, W, H, spp, D, rr_start=0, seed=0, sample_offset=0):
+    from xyz_amd.hip import DeviceArray
+    film = DeviceArray.zeros((H, W, 3), np.float32)
+    st = scene.render(cam, film, spp, D, rr_start=rr_start, seed=seed, sample_offset=sample_offset)
+    return film.numpy(), st
+
+
+def both(oracle_mod, capi_mod, sc, **kw):
+    o = oracle_mod.OracleScene(sc["v0"], sc["v1"], sc["v2"], tri_bsdf=sc.get("tri_bsdf"),
+                               bsdfs=sc.get("bsdfs"), normals=sc.get("normals"), uv=sc.get("uv"),
+                               textures=sc.get("textures"), **kw)
+    g = capi_mod.Scene.from_dict(sc, **kw)
+    return o, g
+
+
+def _assert_same(gpu, orc):
+    diff = gpu.view(np.uint32) != orc.view(np.uint32)
+    assert not diff.any(), (f"{diff.any(-1).mean():.4%} pixels differ; max abs "
+                            f"{np.abs(gpu - orc).max()}")
+
+
+def test_cornell_box_bit_exact(oracle_mod, capi_mod, device):
+    sc = S.cornell_box()
+    o, g = both(oracle_mod, capi_mod, sc)
+    cam = sc["camera"]
+    ref = o.render(cam, 64, 64, 16, max_depth=8, rr_start=3, seed=7)
+    got, st = gpu_render(capi_mod, g, cam, 64, 64, 16, 8, rr_start=3, seed=7)
+    _assert_same(got, ref)
+    assert st["paths"] == 64 * 64 * 16
+
+
+def test_cornell_box_reference_depth_and_rr(oracle_mod, capi_mod, device):
+    # the reference's own settings: depth cap 16 (D = 15), RR at every vertex
+    sc = S.cornell_box()
+    o, g = both(oracle_mod, capi_mod, sc)
+    ref = o.render(sc["camera"], 48, 40, 8, max_depth=15, rr_start=0, seed=99)
+    got, _ = gpu_render(capi_mod, g, sc["camera"], 48, 40, 8, 15, rr_start=0, seed=99)
+    _assert_same(got, ref)
+
+
+def test_progressive_passes_equal_one_pass(capi_mod, device):
+    sc = S.cornell_box()
+    g = capi_mod.Scene.from_dict(sc)
+    from xyz_amd.hip import DeviceArray
+    a = DeviceArray.zeros((32, 32, 3), np.float32)
+    g.render(sc["camera"], a, 12, 8, rr_start=3, seed=1)
+    b = DeviceArray.zeros((32, 32, 3), np.float32)
+    g.render(sc["camera"], b, 5, 8, rr_start=3, seed=1)
+    g.render(sc["camera"], b, 7, 8, rr_start=3, seed=1, sample_offset=5)
+    assert np.array_equal(a.numpy().view(np.uint32), b.numpy().view(np.uint32))
+
+
+def test_textured_cornell_box_bit_exact(oracle_mod, capi_mod, device):
+    sc = S.cornell_box(textured=True, tex_res=16)
+    o, g = both(oracle_mod, capi_mod, sc)
+    ref = o.render(sc["camera"], 40, 40, 8, max_depth=6, rr_start=3, seed=4)
+    got, _ = gpu_render(capi_mod, g, sc["camera"], 40, 40, 8, 6, rr_start=3, seed=4)
+    _assert_same(got, ref)
+
+
+def _specular_scene(smooth=False):
+    """Glass sphere and mirror box on a Lambert floor under two lights of
+    different emission."""
+    v, f = S.icosphere(2)
+    sph = S.displaced_sphere(v, f, 0.0, 0, center=(0, 0.6, 0), radius=0.5)
+    parts = [S.quad((-3, 0, -3), (6, 0, 0), (0, 0, 6), (0, 1, 0)), sph,
+             S.box((1.0, 0.4, 0.3), (0.5, 0.8, 0.5), S.rot_y(25)),
+             S.quad((-1.0, 2.5, -0.5), (0.6, 0, 0), (0, 0, 0.6), (0, -1, 0)),
+             S.quad((0.8, 2.2, 0.5), (0.4, 0, 0), (0, 0, 0.4), (0, -1, 0))]
+    v0 = np.concatenate([p[0] for p in parts]).astype(F)
+    v1 = np.concatenate([p[1] for p in parts]).astype(F)
+    v2 = np.concatenate([p[2] for p in parts]).astype(F)
+    n = [len(p[0]) for p in parts]
+    bs = np.concatenate([np.full(n[0], 0), np.full(n[1], 1), np.full(n[2], 2),
+                         np.full(n[3], 3), np.full(n[4], 4)]).astype(np.uint32)
+    bsdfs = [dict(type=LAMBERT, rgb=(0.6, 0.55, 0.5)), dict(type=GLASS, ior=1.5),
+             dict(type=MIRROR), dict(type=LIGHT, rgb=(12, 10, 8)), dict(type=LIGHT, rgb=(3, 6, 9))]
+    sc = dict(v0=v0, v1=v1, v2=v2, tri_bsdf=bs, bsdfs=bsdfs,
+              camera=np.array([0.3, 1.2, -3.0, 0.2, 0.5, 0, 0, 1, 0, 45.0], F))
+    if smooth:
+        # per-vertex normals: the radial direction for the sphere, flat elsewhere
+        ng = np.cross(v2 - v0, v1 - v0)
+        ng /= np.linalg.norm(ng, axis=1, keepdims=True)
+        n0, n1, n2 = ng.copy(), ng.copy(), ng.copy()
+        a, b = n[0], n[0] + n[1]
+        for vv, nn in ((v0, n0), (v1, n1), (v2, n2)):
+            r = vv[a:b] - np.array([0, 0.6, 0], F)
+            nn[a:b] = r / np.linalg.norm(r, axis=1, keepdims=True)
+        sc["normals"] = (n0.astype(F), n1.astype(F), n2.astype(F))
+    return sc
+
+
+@pytest.mark.parametrize("smooth", [False, True])
+def test_glass_mirror_two_lights_bit_exact(oracle_mod, capi_mod, device, smooth):
+    sc = _specular_scene(smooth)
+    o, g = both(oracle_mod, capi_mod, sc)
+    ref = o.render(sc["camera"], 48, 32, 8, max_depth=12, rr_start=4, seed=2)
+    got, _ = gpu_render(capi_mod, g, sc["camera"], 48, 32, 8, 12, rr_start=4, seed=2)
+    _assert_same(got, ref)
+
+
+def test_metal_statistical(oracle_mod, capi_mod, device):
+    sc = _specular_scene()
+    sc["bsdfs"][2] = dict(type=METAL, rgb=(0.9, 0.8, 0.6), shininess=40.0)
+    o, g = both(oracle_mod, capi_mod, sc)
+    W, H, spp = 32, 24, 64
+    ref = o.render(sc["camera"], W, H, spp, max_depth=8, rr_start=3, seed=3) / spp
+    got, _ = gpu_render(capi_mod, g, sc["camera"], W, H, spp, 8, rr_start=3, seed=3)
+    got = got / spp
+    # identical except where powf's last bit differs; the image means agree
+    assert abs(got.mean() - ref.mean()) < 0.01 * ref.mean()
+    assert (got == ref).mean() > 0.5
+
+
+def test_batch_equals_sequential(oracle_mod, capi_mod, device):
+    from xyz_amd.hip import DeviceArray
+    jobs = [(S.cornell_box(), 32, 32, 4), (_specular_scene(), 24, 16, 6),
+            (S.cornell_box(textured=True, tex_res=8), 16, 24, 3)]
+    scenes = [capi_mod.Scene.from_dict(j[0]) for j in jobs]
+    cams = [j[0]["camera"] for j in jobs]
+    seq = []
+    for sc, cam, (_, W, H, spp) in zip(scenes, cams, jobs):
+        f = DeviceArray.zeros((H, W, 3), np.float32)
+        sc.render(cam, f, spp, 8, rr_start=3, seed=11)
+        seq.append(f.numpy())
+    for use_graph in (0, 1):
+        films = [DeviceArray.zeros((j[2], j[1], 3), np.float32) for j in jobs]
+        t = capi_mod.BatchTuning(64, 2, 128, use_graph)
+        capi_mod.render_batch(scenes, cams, films, [j[3] for j in jobs], 8, rr_start=3,
+                              seeds=[11, 11, 11], tuning=t)
+        for f, s in zip(films, seq):
+            assert np.array_equal(f.numpy().view(np.uint32), s.view(np.uint32))
+    # graph replays: each replay adds the same samples again (film += sum)
+    films = [DeviceArray.zeros((j[2], j[1], 3), np.float32) for j in jobs]
+    t = capi_mod.BatchTuning(64, 2, 128, 1)
+    for rep in range(3):
+        capi_mod.render_batch(scenes, cams, films, [j[3] for j in jobs], 8, rr_start=3,
+                              seeds=[11, 11, 11], tuning=t)
+        for f, s in zip(films, seq):
+            assert np.allclose(f.numpy(), s * (rep + 1), rtol=1e-5, atol=1e-6)
+
+
+def test_render_errors(capi_mod, device):
+    from xyz_amd.hip import DeviceArray
+    sc = S.cornell_box()
+    g = capi_mod.Scene.from_dict(sc)
+    film = DeviceArray.zeros((8, 8, 3), np.float32)
+    with pytest.raises(capi_mod.XyzError):
+        g.render(sc["camera"], film, 1, 0)  # max_depth 0
+    bad = sc["camera"].copy()
+    bad[9] = 0.0
+    with pytest.raises(capi_mod.XyzError):
+        g.render(bad, film, 1, 4)  # fov 0
+    g.render(sc["camera"], film, 0, 4)  # zero samples: nothing to do
+    assert not film.numpy().any()
