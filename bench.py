@@ -30,6 +30,13 @@ def parse():
     ap.add_argument("--cpu-sample-stride", type=int, default=16,
                     help="the CPU baseline traces every n-th ray of each batch")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--skip-pt", action="store_true", help="skip the 1080p path-tracing leg")
+    ap.add_argument("--skip-adjoint", action="store_true", help="skip the adjoint/primal leg")
+    ap.add_argument("--skip-batch", action="store_true", help="skip the batched/sequential leg")
+    ap.add_argument("--pt-spp", type=int, default=16,
+                    help="samples per pixel timed in the 1080p leg (the workload renders 256)")
+    ap.add_argument("--pt-cpu-rows", type=int, default=64,
+                    help="image rows the CPU baseline renders at 1 spp in the 1080p leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 256 MiB device-to-device copy (the reference dispatch "
                          "that scales rocprof FETCH_SIZE / WRITE_SIZE readings)")
@@ -163,6 +170,10 @@ def main():
                           f"GPU ids identical on the sample: {same}")
         log(f"[bench] cpu baseline: {cpu}")
 
+    pt = None if args.skip_pt else bench_pt(args, hip, capi, scenes)
+    adj = None if args.skip_adjoint else bench_adjoint(args, hip, capi, scenes)
+    bat = None if args.skip_batch else bench_batch(args, hip, capi, scenes)
+
     out = {
         "metric": "Mrays/s closest-hit",
         "value": round(mrays, 1),
@@ -187,8 +198,138 @@ def main():
         "split": split,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "pt_1080p": pt,
+        "adjoint": adj,
+        "batched": bat,
     }
     print(json.dumps(out), flush=True)
+
+
+def bench_pt(args, hip, capi, scenes):
+    """Path tracing at 1920x1080 on the ~2M-triangle production scene: max depth
+    16, Russian roulette after depth 4.  Msamples/s over `--pt-spp` samples per
+    pixel of the 256-spp workload; the CPU oracle renders a band of rows at 1 spp."""
+    W, H, D, R = 1920, 1080, 16, 4
+    sc = scenes.production_scene()
+    g = capi.Scene.from_dict(sc)
+    info = g.info()
+    film = hip.DeviceArray.zeros((H, W, 3), np.float32)
+    g.render(sc["camera"], film, 1, D, rr_start=R, seed=1, sample_offset=10**6)  # warm-up
+    film.zero()
+    e0, e1 = hip.Event(), hip.Event()
+    e0.record(None)
+    st = g.render(sc["camera"], film, args.pt_spp, D, rr_start=R, seed=1)
+    e1.record(None)
+    ms = e0.elapsed_ms(e1)
+    samples = W * H * args.pt_spp
+    rate = samples / (ms * 1e-3) / 1e6
+    res = dict(msamples_per_s=round(rate, 2), ms=round(ms, 1), spp=args.pt_spp,
+               frame_256spp_s=round(W * H * 256 / (rate * 1e6), 2),
+               queries_per_sample=round(st["queries"] / samples, 3),
+               mrays_per_s=round(st["queries"] / (ms * 1e-3) / 1e6, 1),
+               n_tris=int(info["n_tris"]), config="1920x1080, depth 16, RR after 4")
+    log(f"[bench] pt 1080p: {res}")
+    if not args.no_cpu_baseline:
+        from oracle import oracle
+        tb = time.time()
+        o = oracle.OracleScene(sc["v0"], sc["v1"], sc["v2"], tri_bsdf=sc["tri_bsdf"],
+                               bsdfs=sc["bsdfs"])
+        build_s = time.time() - tb
+        y0 = H // 2
+        rows = (y0, y0 + args.pt_cpu_rows)
+        threads = args.cpu_threads or oracle.max_threads()
+        ofilm, ost = o.render(sc["camera"], W, H, 1, D, rr_start=R, seed=1, rows=rows,
+                              threads=threads, stats=True)
+        gfilm = hip.DeviceArray.zeros((H, W, 3), np.float32)
+        g.render(sc["camera"], gfilm, 1, D, rr_start=R, seed=1)
+        same = bool(np.array_equal(gfilm.numpy()[rows[0]:rows[1]].view(np.uint32),
+                                   ofilm[rows[0]:rows[1]].view(np.uint32)))
+        n = W * args.pt_cpu_rows
+        res["cpu_baseline"] = dict(
+            value=round(n / ost["seconds"] / 1e6, 4), unit="Msamples/s", cores=threads,
+            kind="port", sample=f"rows {rows[0]}-{rows[1] - 1} of the same frame at 1 spp "
+                                f"({n} samples); oracle kd build {build_s:.1f}s excluded; "
+                                f"GPU film bit-identical on those rows: {same}")
+        res["vs_cpu"] = round(rate / res["cpu_baseline"]["value"], 1)
+        log(f"[bench] pt cpu baseline: {res['cpu_baseline']}")
+    return res
+
+
+def bench_adjoint(args, hip, capi, scenes):
+    """Cornell box with 64x64 RGB albedo textures on the walls, 512x512, 64 spp,
+    depth 6: time of the path-replay backward pass over the primal pass."""
+    W = H = 512
+    spp, D, R = 64, 6, 3
+    sc = scenes.cornell_box(textured=True, tex_res=64)
+    g = capi.Scene.from_dict(sc)
+    film = hip.DeviceArray.zeros((H, W, 3), np.float32)
+    w = hip.DeviceArray.from_numpy(np.random.default_rng(0).uniform(-1, 1, (H, W, 3)).astype(np.float32))
+    dt = hip.DeviceArray.zeros(g.n_texels, np.float32)
+    de = hip.DeviceArray.zeros(g.n_bsdfs * 3, np.float32)
+    g.render(sc["camera"], film, 4, D, rr_start=R, seed=2)  # warm-up
+    g.render_backward(sc["camera"], w, dt, de, 4, D, rr_start=R, seed=2)
+    e0, e1, e2 = hip.Event(), hip.Event(), hip.Event()
+    e0.record(None)
+    g.render(sc["camera"], film, spp, D, rr_start=R, seed=3)
+    e1.record(None)
+    g.render_backward(sc["camera"], w, dt, de, spp, D, rr_start=R, seed=3)
+    e2.record(None)
+    fwd = e0.elapsed_ms(e1)
+    bwd = e1.elapsed_ms(e2)
+    res = dict(primal_ms=round(fwd, 2), backward_ms=round(bwd, 2),
+               adjoint_over_primal=round(bwd / fwd, 3),
+               config="textured Cornell box (61,440 texels), 512x512, 64 spp, depth 6")
+    log(f"[bench] adjoint: {res}")
+    return res
+
+
+def bench_batch(args, hip, capi, scenes):
+    """64 generated scenes (about 1e3..5e5 triangles, 128^2..1024^2, 16..256 spp,
+    depth 8): one persistent graph-captured launch (autotuned) against the same
+    renders one after another through the wavefront entry point."""
+    D = 8
+    jobs = [scenes.batch_scene(s) for s in range(64)]
+    gs = [capi.Scene.from_dict(j) for j in jobs]
+    films = [hip.DeviceArray.zeros((j["height"], j["width"], 3), np.float32) for j in jobs]
+    cams = [j["camera"] for j in jobs]
+    spps = [j["spp"] for j in jobs]
+    seeds = list(range(64))
+    samples = sum(j["width"] * j["height"] * j["spp"] for j in jobs)
+    tb = time.time()
+    best = capi.batch_autotune(gs, cams, films, spps, D, rr_start=3, seeds=seeds)
+    tune_s = time.time() - tb
+    # sequential plain renders
+    for f in films:
+        f.zero()
+    e0, e1 = hip.Event(), hip.Event()
+    e0.record(None)
+    for s, c, f, sp, sd in zip(gs, cams, films, spps, seeds):
+        s.render(c, f, sp, D, rr_start=3, seed=sd)
+    e1.record(None)
+    seq_ms = e0.elapsed_ms(e1)
+    seq_films = [f.numpy() for f in films]
+    # batched: one graph launch (first call captures; the timed call replays)
+    for f in films:
+        f.zero()
+    capi.render_batch(gs, cams, films, spps, D, rr_start=3, seeds=seeds, tuning=best)
+    same = all(np.array_equal(f.numpy().view(np.uint32), s.view(np.uint32))
+               for f, s in zip(films, seq_films))
+    for f in films:
+        f.zero()
+    e0.record(None)
+    capi.render_batch(gs, cams, films, spps, D, rr_start=3, seeds=seeds, tuning=best)
+    e1.record(None)
+    bat_ms = e0.elapsed_ms(e1)
+    res = dict(batched_ms=round(bat_ms, 1), sequential_ms=round(seq_ms, 1),
+               batched_over_sequential=round(seq_ms / bat_ms, 3),
+               batched_msamples_per_s=round(samples / (bat_ms * 1e-3) / 1e6, 1),
+               sequential_msamples_per_s=round(samples / (seq_ms * 1e-3) / 1e6, 1),
+               samples=int(samples), films_bit_identical=bool(same),
+               tuning=dict(tile_pixels=best.tile_pixels, blocks_per_cu=best.blocks_per_cu,
+                           block_threads=best.block_threads, use_graph=best.use_graph),
+               autotune_s=round(tune_s, 1))
+    log(f"[bench] batched: {res}")
+    return res
 
 
 if __name__ == "__main__":

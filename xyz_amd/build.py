@@ -28,14 +28,16 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose=False):
+def build(verbose=False, out=None, defines=(), tmp_dir=None):
+    """Compiles SOURCES into `out` (default xyz_amd/libxyz_hip.so)."""
     objs = []
-    tmp_dir = os.path.join(HERE, "_build")
+    tmp_dir = tmp_dir or os.path.join(HERE, "_build")
     os.makedirs(tmp_dir, exist_ok=True)
     for src in SOURCES:
         path = os.path.join(CSRC, src)
         obj = os.path.join(tmp_dir, src + ".o")
-        cmd = [hipcc()] + [f for f in HIPCC_FLAGS if f != "-shared"] + ["-c", path, "-o", obj]
+        cmd = ([hipcc()] + [f for f in HIPCC_FLAGS if f != "-shared"] + list(defines)
+               + ["-c", path, "-o", obj])
         if src.endswith(".cpp"):
             cmd = [hipcc(), "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
                    "-Wall", "-x", "c++", "-c", path, "-o", obj]
@@ -43,13 +45,14 @@ def build(verbose=False):
             print(" ".join(cmd), flush=True)
         subprocess.check_call(cmd)
         objs.append(obj)
-    tmp = LIB + ".tmp"
+    target = out or LIB
+    tmp = target + ".tmp"
     cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, target)
+    return target
 
 
 if __name__ == "__main__":

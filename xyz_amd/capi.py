@@ -97,9 +97,10 @@ def lib():
     """Load libxyz_hip.so (raises if it has not been built)."""
     global _lib
     if _lib is None:
-        if not os.path.exists(_build.LIB):
-            raise RuntimeError(f"{_build.LIB} is missing: run xyz_amd.build.build() first")
-        L = ctypes.CDLL(_build.LIB)
+        path = os.environ.get("XYZ_HIP_LIB", _build.LIB)  # developer override of the build
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing: run xyz_amd.build.build() first")
+        L = ctypes.CDLL(path)
         L.xyz_kdtree_build.argtypes = [ctypes.c_uint32, _f32p, _f32p, _f32p, ctypes.c_uint32,
                                        ctypes.c_uint32, ctypes.POINTER(_vp)]
         L.xyz_kdtree_info.argtypes = [_vp, ctypes.POINTER(SceneInfo)]

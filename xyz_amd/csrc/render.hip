@@ -37,7 +37,9 @@ struct WaveCounters {
   uint32_t cur[8];
   uint32_t next[8];
   uint32_t shadow;
-  uint32_t pad[5];
+  uint32_t work_extend;  // persistent extend kernel's work counter
+  uint32_t work_shadow;  // persistent shadow kernel's work counter
+  uint32_t pad[3];
   unsigned long long queries;
 };
 
@@ -65,6 +67,49 @@ struct GradArgs {
   float* d_emit;
 };
 
+// Per-workgroup aggregation of the adjoint's gradient deposits in LDS: the
+// emission gradient is a dense array (a handful of lights, hit by every path),
+// texel gradients go to a small open-addressing table; one global atomic per
+// touched entry is issued when the workgroup ends.  Entries that do not fit go
+// straight to global memory.
+constexpr uint32_t kTexSlots = 1024;
+constexpr uint32_t kTexProbe = 8;
+constexpr uint32_t kEmitLds = 3 * 256;
+constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;
+
+struct GradLds {
+  uint32_t keys[kTexSlots];
+  float vals[kTexSlots * 3];
+  float emit[kEmitLds];
+};
+
+__device__ __forceinline__ void grad_lds_init(GradLds& L) {
+  for (uint32_t i = threadIdx.x; i < kTexSlots; i += blockDim.x) {
+    L.keys[i] = kEmptyKey;
+    L.vals[3 * i] = L.vals[3 * i + 1] = L.vals[3 * i + 2] = 0.0f;
+  }
+  for (uint32_t i = threadIdx.x; i < kEmitLds; i += blockDim.x) L.emit[i] = 0.0f;
+  __syncthreads();
+}
+
+__device__ __forceinline__ void grad_lds_flush(GradLds& L, float* d_tex, float* d_emit,
+                                               uint32_t n_emit) {
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < kTexSlots; i += blockDim.x) {
+    uint32_t const k = L.keys[i];
+    if (k == kEmptyKey) continue;
+    for (int c = 0; c < 3; ++c) {
+      float const v = L.vals[3 * i + c];
+      if (v != 0.0f) unsafeAtomicAdd(d_tex + 3 * size_t(k) + c, v);
+    }
+  }
+  uint32_t const m = n_emit < kEmitLds ? n_emit : kEmitLds;
+  for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
+    float const v = L.emit[i];
+    if (v != 0.0f) unsafeAtomicAdd(d_emit + i, v);
+  }
+}
+
 struct GradReplay {
   const DevTex* tex;
   const DevBsdf* bsdfs;
@@ -72,20 +117,44 @@ struct GradReplay {
   float3 ctot;  // the path's total colour from the primal pass
   float* d_tex;
   float* d_emit;
+  GradLds* lds;
   __device__ __forceinline__ void albedo(int32_t tex, int32_t texel, float3 alb, float3 prefix) {
     // every later term carries this albedo as a factor: d/dalb = suffix / alb
-    size_t const base = size_t(this->tex[tex].offset) + 3 * size_t(texel);
+    uint32_t const key = (tex_offset(tex) / 3) + uint32_t(texel);
     float const sx = ctot.x - prefix.x, sy = ctot.y - prefix.y, sz = ctot.z - prefix.z;
-    if (sx != 0.0f) unsafeAtomicAdd(d_tex + base, w.x * sx / alb.x);
-    if (sy != 0.0f) unsafeAtomicAdd(d_tex + base + 1, w.y * sy / alb.y);
-    if (sz != 0.0f) unsafeAtomicAdd(d_tex + base + 2, w.z * sz / alb.z);
+    float const gx = (sx != 0.0f) ? w.x * sx / alb.x : 0.0f;
+    float const gy = (sy != 0.0f) ? w.y * sy / alb.y : 0.0f;
+    float const gz = (sz != 0.0f) ? w.z * sz / alb.z : 0.0f;
+    if (gx == 0.0f && gy == 0.0f && gz == 0.0f) return;
+    uint32_t h = (key * 2654435761u) >> 22;  // 10 bits
+    for (uint32_t p = 0; p < kTexProbe; ++p) {
+      uint32_t const slot = (h + p) & (kTexSlots - 1);
+      uint32_t const prev = atomicCAS(&lds->keys[slot], kEmptyKey, key);
+      if (prev == kEmptyKey || prev == key) {
+        atomicAdd(&lds->vals[3 * slot], gx);
+        atomicAdd(&lds->vals[3 * slot + 1], gy);
+        atomicAdd(&lds->vals[3 * slot + 2], gz);
+        return;
+      }
+    }
+    size_t const base = 3 * size_t(key);
+    unsafeAtomicAdd(d_tex + base, gx);
+    unsafeAtomicAdd(d_tex + base + 1, gy);
+    unsafeAtomicAdd(d_tex + base + 2, gz);
   }
+  __device__ __forceinline__ uint32_t tex_offset(int32_t t) const { return tex[t].offset; }
   __device__ __forceinline__ void emission(uint32_t bsdf, float3 add) {
     // Phi = L_e * pi * A, every emission term is linear in its light's L_e
     DevBsdf const& b = bsdfs[bsdf];
-    if (b.rgb[0] != 0.0f && add.x != 0.0f) unsafeAtomicAdd(d_emit + 3 * bsdf, w.x * add.x / b.rgb[0]);
-    if (b.rgb[1] != 0.0f && add.y != 0.0f) unsafeAtomicAdd(d_emit + 3 * bsdf + 1, w.y * add.y / b.rgb[1]);
-    if (b.rgb[2] != 0.0f && add.z != 0.0f) unsafeAtomicAdd(d_emit + 3 * bsdf + 2, w.z * add.z / b.rgb[2]);
+    float const g[3] = {(b.rgb[0] != 0.0f && add.x != 0.0f) ? w.x * add.x / b.rgb[0] : 0.0f,
+                        (b.rgb[1] != 0.0f && add.y != 0.0f) ? w.y * add.y / b.rgb[1] : 0.0f,
+                        (b.rgb[2] != 0.0f && add.z != 0.0f) ? w.z * add.z / b.rgb[2] : 0.0f};
+    for (int c = 0; c < 3; ++c) {
+      if (g[c] == 0.0f) continue;
+      uint32_t const i = 3 * bsdf + uint32_t(c);
+      if (i < kEmitLds) atomicAdd(&lds->emit[i], g[c]);
+      else unsafeAtomicAdd(d_emit + i, g[c]);
+    }
   }
 };
 
@@ -138,6 +207,8 @@ k_camera(DevScene s, DevCamera cam, PTConst c, WaveBufs wb, uint64_t q0, uint32_
     for (int o = 1; o < 8; ++o) wb.ctr->cur[o] = 0;
     for (int o = 0; o < 8; ++o) wb.ctr->next[o] = 0;
     wb.ctr->shadow = 0;
+    wb.ctr->work_extend = 0;
+    wb.ctr->work_shadow = 0;
   }
   if (i >= n) return;
   uint64_t const q = q0 + i;
@@ -152,20 +223,42 @@ k_camera(DevScene s, DevCamera cam, PTConst c, WaveBufs wb, uint64_t q0, uint32_
   wb.qa[i] = i;
 }
 
+// Persistent: lanes take active paths from the queues in batches (WaveFeeder)
+// and trace them one step at a time, so a wave never waits for its slowest ray.
 __global__ void __launch_bounds__(kBlock)
 k_extend(DevScene s, WaveBufs wb, const uint32_t* __restrict__ qcur) {
   extern __shared__ uint32_t lds_stack[];
   __shared__ uint32_t cnt[8];
   if (threadIdx.x < 8) cnt[threadIdx.x] = wb.ctr->cur[threadIdx.x];
   __syncthreads();
+  __shared__ unsigned long long keys[kBlock];
   uint32_t const total = queue_total(cnt);
-  for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < total; j += gridDim.x * kBlock) {
-    uint32_t const slot = queue_slot(cnt, qcur, wb.cap, j);
-    float4 const p = wb.pos[slot], d = wb.dir[slot];
-    HitRec const h = trace_closest<false>(s, ld3(p), ld3(d), lds_stack + threadIdx.x, kBlock,
-                                          nullptr, nullptr);
-    wb.hit[slot] = make_float4(__int_as_float(h.prim < 0 ? -1 : (h.prim | (h.back ? 0x80000000 : 0))),
-                               h.t, h.f, h.g);
+  uint32_t* const stk = lds_stack + threadIdx.x;
+  WaveFeeder feed;
+  feed.init();
+  TraceState st;
+  uint32_t slot = 0xFFFFFFFFu;
+  bool exhausted = false;
+  for (;;) {
+    bool const need = (slot == 0xFFFFFFFFu) && !exhausted;
+    uint32_t const j = feed.refill(need, &wb.ctr->work_extend, total, &exhausted);
+    if (j != 0xFFFFFFFFu) {
+      slot = queue_slot(cnt, qcur, wb.cap, j);
+      trace_init(st, ld3(wb.pos[slot]), ld3(wb.dir[slot]));
+    }
+    if (__ballot(slot != 0xFFFFFFFFu) == 0) break;
+    bool const act = slot != 0xFFFFFFFFu;
+#if XYZ_COOP
+    bool const fin = coop_step<false>(s, st, act, stk, kBlock, keys + (threadIdx.x & ~63u));
+#else
+    bool const fin = act && trace_step<false>(s, st, stk, kBlock);
+#endif
+    if (act && fin) {
+      HitRec const h = trace_result(s, st);
+      wb.hit[slot] = make_float4(__int_as_float(h.prim < 0 ? -1 : (h.prim | (h.back ? 0x80000000 : 0))),
+                                 h.t, h.f, h.g);
+      slot = 0xFFFFFFFFu;
+    }
   }
 }
 
@@ -219,6 +312,12 @@ __global__ void __launch_bounds__(kBlock)
 k_shade(DevScene s, PTConst c, WaveBufs wb, const uint32_t* __restrict__ qcur, uint32_t* qnext,
         uint64_t q0, uint32_t wh, uint32_t sample0, GradArgs ga) {
   __shared__ uint32_t cnt[8];
+  GradLds* glds = nullptr;
+  if constexpr (kReplay) {
+    __shared__ GradLds glds_mem;  // only the replay instantiation reserves it
+    glds = &glds_mem;
+    grad_lds_init(glds_mem);
+  }
   if (threadIdx.x < 8) cnt[threadIdx.x] = wb.ctr->cur[threadIdx.x];
   __syncthreads();
   uint32_t const total = queue_total(cnt);
@@ -249,6 +348,7 @@ k_shade(DevScene s, PTConst c, WaveBufs wb, const uint32_t* __restrict__ qcur, u
         g.ctot = ld3(wb.ctot[slot]);
         g.d_tex = ga.d_tex;
         g.d_emit = ga.d_emit;
+        g.lds = glds;
         cont = shade_vertex(s, c, ps, h, pixel, sample, lv, &sr, &has_sr, g);
       } else {
         NoGrad g;
@@ -267,20 +367,49 @@ k_shade(DevScene s, PTConst c, WaveBufs wb, const uint32_t* __restrict__ qcur, u
       wb.smet[r] = make_float4(sr.fsn, sr.fres, sr.mx, 0.0f);
     }
   }
+  if constexpr (kReplay) grad_lds_flush(*glds, ga.d_tex, ga.d_emit, 3 * s.n_bsdfs);
 }
 
 template <bool kReplay>
 __global__ void __launch_bounds__(kBlock)
 k_shadow(DevScene s, WaveBufs wb, uint64_t q0, uint32_t wh, GradArgs ga) {
   extern __shared__ uint32_t lds_stack[];
+  GradLds* glds = nullptr;
+  if constexpr (kReplay) {
+    __shared__ GradLds glds_mem;
+    glds = &glds_mem;
+    grad_lds_init(glds_mem);
+  }
+  __shared__ unsigned long long keys[kBlock];
   uint32_t const total = wb.ctr->shadow;
-  for (uint32_t r = blockIdx.x * kBlock + threadIdx.x; r < total; r += gridDim.x * kBlock) {
-    float4 const o = wb.so[r], w = wb.sw[r];
-    HitRec const h = trace_closest<false>(s, ld3(o), ld3(w), lds_stack + threadIdx.x, kBlock,
-                                          nullptr, nullptr);
-    uint4 const meta = wb.smeta[r];
+  uint32_t* const stk = lds_stack + threadIdx.x;
+  WaveFeeder feed;
+  feed.init();
+  TraceState st;
+  uint32_t r = 0xFFFFFFFFu;
+  bool exhausted = false;
+  for (;;) {
+    bool const need = (r == 0xFFFFFFFFu) && !exhausted;
+    uint32_t const got = feed.refill(need, &wb.ctr->work_shadow, total, &exhausted);
+    if (got != 0xFFFFFFFFu) {
+      r = got;
+      trace_init(st, ld3(wb.so[r]), ld3(wb.sw[r]));
+    }
+    if (__ballot(r != 0xFFFFFFFFu) == 0) break;
+    bool const act = r != 0xFFFFFFFFu;
+#if XYZ_COOP
+    bool const fin = coop_step<false>(s, st, act, stk, kBlock, keys + (threadIdx.x & ~63u));
+#else
+    bool const fin = act && trace_step<false>(s, st, stk, kBlock);
+#endif
+    if (!act || !fin) continue;
+    HitRec const h = trace_result(s, st);
+    uint32_t const rr = r;
+    r = 0xFFFFFFFFu;
+    uint4 const meta = wb.smeta[rr];
     if (h.prim != int32_t(meta.x)) continue;
-    float4 const pe = wb.spe[r], met = wb.smet[r];
+    float4 const o = wb.so[rr], w = wb.sw[rr];
+    float4 const pe = wb.spe[rr], met = wb.smet[rr];
     ShadowReq sr;
     sr.type = meta.y;
     sr.nk2 = o.w;
@@ -291,7 +420,7 @@ k_shadow(DevScene s, WaveBufs wb, uint64_t q0, uint32_t wh, GradArgs ga) {
     sr.mx = met.z;
     float3 const add = nee_contribution(sr, h.t, s.light_area);
     uint32_t const slot = __float_as_uint(pe.w);
-    if (kReplay) {
+    if constexpr (kReplay) {
       uint64_t const q = q0 + slot;
       uint32_t const pixel = uint32_t(q % wh);
       GradReplay g;
@@ -301,11 +430,13 @@ k_shadow(DevScene s, WaveBufs wb, uint64_t q0, uint32_t wh, GradArgs ga) {
                ga.d_film[3 * size_t(pixel) + 2]);
       g.d_tex = ga.d_tex;
       g.d_emit = ga.d_emit;
+      g.lds = glds;
       g.emission(meta.z, add);
     }
     float4 const cc = wb.col[slot];
     wb.col[slot] = make_float4(cc.x + add.x, cc.y + add.y, cc.z + add.z, 0.0f);
   }
+  if constexpr (kReplay) grad_lds_flush(*glds, ga.d_tex, ga.d_emit, 3 * s.n_bsdfs);
 }
 
 __global__ void k_prep(WaveCounters* ctr) {
@@ -317,6 +448,8 @@ __global__ void k_prep(WaveCounters* ctr) {
     ctr->next[o] = 0;
   }
   ctr->shadow = 0;
+  ctr->work_extend = 0;
+  ctr->work_shadow = 0;
   ctr->queries += t;
 }
 
@@ -476,7 +609,7 @@ int run_bounces(const DevScene& s, const PTConst& c, WaveBufs wb, uint64_t q0, u
                 uint32_t wh, uint32_t sample0, uint32_t max_depth, GradArgs ga, hipStream_t st) {
   int const ncu = num_cu();
   size_t const lds = size_t(s.stack_cap) * kBlock * sizeof(uint32_t);
-  uint32_t const trace_grid = std::max(1u, std::min((n + kBlock - 1) / kBlock, uint32_t(ncu) * 4));
+  uint32_t const trace_grid = std::max(1u, std::min((n + kBlock - 1) / kBlock, uint32_t(ncu) * 8));
   uint32_t const shade_grid = std::max(1u, std::min((n + kBlock - 1) / kBlock, uint32_t(ncu) * 8));
   uint32_t* qc = wb.qa;
   uint32_t* qn = wb.qb;

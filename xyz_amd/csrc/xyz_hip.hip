@@ -2,6 +2,7 @@
 // The path-tracing entry points live in render.hip.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -28,58 +29,95 @@ int hip_fail(hipError_t e, const char* what) {
 
 // ------------------------------------------------------------ closest hit
 constexpr uint32_t kHitBlock = 256;
-constexpr uint32_t kXcdChunk = 8;
+#ifndef XYZ_HIT_BLOCKS_PER_CU
+#define XYZ_HIT_BLOCKS_PER_CU 8
+#endif
+constexpr uint32_t kHitBlocksPerCu = XYZ_HIT_BLOCKS_PER_CU;
 
-// One ray per lane.  Workgroups are dealt to the 8 XCDs round robin; the
-// remap below gives each XCD runs of kXcdChunk consecutive blocks, so rays
-// that are neighbours (coherent rays share kd nodes and leaf triangles) meet
-// in one XCD's L2, while every XCD still gets an even share of every part of
-// the batch (a mixed coherent/incoherent batch stays balanced).
+// Persistent closest-hit kernel: every lane keeps a ray in flight; when its
+// query finishes the lane takes the next ray of its wave's batch (batches of 64
+// consecutive rays from a global counter), so a wave's SIMD lanes stay busy
+// whatever the spread of per-ray cost.
 template <bool kCount>
 __global__ void __launch_bounds__(kHitBlock)
 closest_hit_kernel(DevScene s, uint32_t n, const float* __restrict__ org,
                    const float* __restrict__ dir, int32_t* __restrict__ prim,
                    float* __restrict__ tout, float* __restrict__ fout, float* __restrict__ gout,
                    uint8_t* __restrict__ bout, uint32_t* __restrict__ counters,
-                   uint32_t) {
+                   uint32_t* __restrict__ work) {
   extern __shared__ uint32_t lds_stack[];
-  uint32_t const b = blockIdx.x;
-  uint32_t const k = b >> 3;
-  uint32_t const logical = (k / kXcdChunk) * (8 * kXcdChunk) + (b & 7u) * kXcdChunk + (k % kXcdChunk);
-  uint32_t const i = logical * kHitBlock + threadIdx.x;
-  if (i >= n) return;
-  float3 const o = make_float3(org[3 * size_t(i)], org[3 * size_t(i) + 1], org[3 * size_t(i) + 2]);
-  float3 const d = make_float3(dir[3 * size_t(i)], dir[3 * size_t(i) + 1], dir[3 * size_t(i) + 2]);
-  uint32_t cn = 0, ct = 0;
-  HitRec const h = trace_closest<kCount>(s, o, d, lds_stack + threadIdx.x, kHitBlock, &cn, &ct);
-  prim[i] = h.prim;
-  if (tout) tout[i] = h.t;
-  if (fout) fout[i] = h.f;
-  if (gout) gout[i] = h.g;
-  if (bout) bout[i] = h.back ? 1 : 0;
-  if (kCount) {
-    counters[i] = cn;
-    counters[size_t(n) + i] = ct;
+  __shared__ unsigned long long keys[kHitBlock];
+  uint32_t* const stk = lds_stack + threadIdx.x;
+  WaveFeeder feed;
+  feed.init();
+  TraceState st;
+  uint32_t r = 0xFFFFFFFFu;
+  bool exhausted = false;
+  for (;;) {
+    bool const need = (r == 0xFFFFFFFFu) && !exhausted;
+    uint32_t const got = feed.refill(need, work, n, &exhausted);
+    if (got != 0xFFFFFFFFu) {
+      r = got;
+      trace_init(st, make_float3(org[3 * size_t(r)], org[3 * size_t(r) + 1], org[3 * size_t(r) + 2]),
+                 make_float3(dir[3 * size_t(r)], dir[3 * size_t(r) + 1], dir[3 * size_t(r) + 2]));
+    }
+    if (__ballot(r != 0xFFFFFFFFu) == 0) break;
+    bool const act = r != 0xFFFFFFFFu;
+#if XYZ_COOP
+    bool const fin = coop_step<kCount>(s, st, act, stk, kHitBlock, keys + (threadIdx.x & ~63u));
+#else
+    bool const fin = act && trace_step<kCount>(s, st, stk, kHitBlock);
+#endif
+    if (act && fin) {
+      HitRec const h = trace_result(s, st);
+      prim[r] = h.prim;
+      if (tout) tout[r] = h.t;
+      if (fout) fout[r] = h.f;
+      if (gout) gout[r] = h.g;
+      if (bout) bout[r] = h.back ? 1 : 0;
+      if (kCount) {
+        counters[r] = st.nn;
+        counters[size_t(n) + r] = st.nt;
+      }
+      r = 0xFFFFFFFFu;
+    }
   }
 }
+
+__global__ void k_zero_u32(uint32_t* p) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
+}
+
+int g_hit_cu = 0;
 
 int launch_closest_hit(const DevScene& s, uint32_t n, const float* org, const float* dir,
                        int32_t* prim, float* t, float* f, float* g, uint8_t* back,
                        uint32_t* counters, hipStream_t stream) {
   if (n == 0) return XYZ_OK;
-  uint32_t const blocks = (n + kHitBlock - 1) / kHitBlock;
-  uint32_t const group = 8 * kXcdChunk;
-  uint32_t const grid = (blocks + group - 1) / group * group;
-  uint32_t const per_xcd = grid / 8;
+  if (g_hit_cu == 0) {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
+      v = 256;
+    g_hit_cu = v;
+  }
   size_t const lds = size_t(s.stack_cap) * kHitBlock * sizeof(uint32_t);
+  uint32_t const need_blocks = (n + kHitBlock - 1) / kHitBlock;
+  uint32_t const grid = std::min<uint32_t>(need_blocks, uint32_t(g_hit_cu) * kHitBlocksPerCu);
+  void* work = nullptr;
+  hipError_t e = hipMallocAsync(&work, sizeof(uint32_t), stream);
+  if (e != hipSuccess) return hip_fail(e, "hipMallocAsync(work counter)");
+  uint32_t* w = static_cast<uint32_t*>(work);
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, w);
   if (counters) {
     hipLaunchKernelGGL(closest_hit_kernel<true>, dim3(grid), dim3(kHitBlock), lds, stream, s, n,
-                       org, dir, prim, t, f, g, back, counters, per_xcd);
+                       org, dir, prim, t, f, g, back, counters, w);
   } else {
     hipLaunchKernelGGL(closest_hit_kernel<false>, dim3(grid), dim3(kHitBlock), lds, stream, s, n,
-                       org, dir, prim, t, f, g, back, counters, per_xcd);
+                       org, dir, prim, t, f, g, back, counters, w);
   }
-  hipError_t const e = hipGetLastError();
+  e = hipGetLastError();
+  (void)hipFreeAsync(work, stream);
   if (e != hipSuccess) return hip_fail(e, "closest_hit launch");
   return XYZ_OK;
 }

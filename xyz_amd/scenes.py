@@ -40,6 +40,8 @@ def box(center, size, rot=None, tess=1):
     center = np.asarray(center, np.float64)
     h = np.asarray(size, np.float64) / 2
     rot = np.eye(3) if rot is None else np.asarray(rot, np.float64)
+    ii, jj = np.meshgrid(np.arange(tess), np.arange(tess), indexing="ij")
+    ii, jj = ii.reshape(-1, 1), jj.reshape(-1, 1)
     V0, V1, V2 = [], [], []
     for ax in range(3):
         for sgn in (-1.0, 1.0):
@@ -54,16 +56,36 @@ def box(center, size, rot=None, tess=1):
             base[ax] = sgn * h[ax]
             base[a1] = -h[a1]
             base[a2] = -h[a2]
-            for i in range(tess):
-                for j in range(tess):
-                    p = base + i * e_u + j * e_v
-                    v0, v1, v2 = quad(p, e_u, e_v, n)
-                    V0.append(v0)
-                    V1.append(v1)
-                    V2.append(v2)
+            p = base + ii * e_u + jj * e_v
+            a, b, c, d = p, p + e_u, p + e_u + e_v, p + e_v
+            v0 = np.concatenate([a, a])
+            v1 = np.concatenate([b, c])
+            v2 = np.concatenate([c, d])
+            v0, v1, v2 = _orient(v0, v1, v2, np.tile(n, (len(v0), 1)))
+            V0.append(v0)
+            V1.append(v1)
+            V2.append(v2)
     V0, V1, V2 = (np.concatenate(x) for x in (V0, V1, V2))
     V0, V1, V2 = (x @ rot.T + center for x in (V0, V1, V2))
     return V0, V1, V2
+
+
+def boxes_y(centers, sizes, yaw_deg):
+    """Many boxes at once: centres (N,3), sizes (N,3), rotation about y (N,) in
+    degrees; 12 triangles per box in box order, outward normals."""
+    unit = box((0, 0, 0), (1, 1, 1))  # 12 triangles of the unit cube
+    centers = np.asarray(centers, np.float64)
+    sizes = np.asarray(sizes, np.float64)
+    a = np.radians(np.asarray(yaw_deg, np.float64))
+    c, s = np.cos(a), np.sin(a)
+    out = []
+    for v in unit:
+        p = v[None, :, :] * sizes[:, None, :]  # (N, 12, 3)
+        x = c[:, None] * p[..., 0] + s[:, None] * p[..., 2]
+        z = -s[:, None] * p[..., 0] + c[:, None] * p[..., 2]
+        q = np.stack([x, p[..., 1], z], -1) + centers[:, None, :]
+        out.append(q.reshape(-1, 3))
+    return out[0], out[1], out[2]
 
 
 def rot_y(deg):
@@ -125,31 +147,29 @@ def icosphere(level):
 
 
 def geodesic_sphere(freq):
-    """Unit class-I geodesic sphere: each icosahedron face split into freq**2 triangles."""
+    """Unit class-I geodesic sphere: each icosahedron face split into freq**2
+    triangles (20 * freq**2 faces); vertices on shared edges are computed from
+    the same corner pair in the same order, so neighbouring faces meet exactly."""
     v, f = _icosahedron()
+    n = int(freq)
+    ii, jj = np.meshgrid(np.arange(n + 1), np.arange(n + 1), indexing="ij")
+    mask = (ii + jj) <= n
+    I, J = ii[mask], jj[mask]
+    idx = -np.ones((n + 1, n + 1), np.int64)
+    idx[I, J] = np.arange(len(I))
+    up_i, up_j = np.nonzero((ii + jj) <= n - 1)
+    dn_i, dn_j = np.nonzero((ii + jj) <= n - 2)
+    tri = np.concatenate([
+        np.stack([idx[up_i, up_j], idx[up_i + 1, up_j], idx[up_i, up_j + 1]], 1),
+        np.stack([idx[dn_i + 1, dn_j], idx[dn_i + 1, dn_j + 1], idx[dn_i, dn_j + 1]], 1)])
+    npts = len(I)
     V, F = [], []
-    off = 0
-    for a, b, c in f:
+    for fi, (a, b, c) in enumerate(f):
         A, B, C = v[a], v[b], v[c]
-        idx = {}
-        pts = []
-        for i in range(freq + 1):
-            for j in range(freq + 1 - i):
-                k = freq - i - j
-                p = (A * k + B * i + C * j) / freq
-                idx[(i, j)] = len(pts)
-                pts.append(p)
-        tris = []
-        for i in range(freq):
-            for j in range(freq - i):
-                tris.append((idx[(i, j)], idx[(i + 1, j)], idx[(i, j + 1)]))
-                if j < freq - i - 1:
-                    tris.append((idx[(i + 1, j)], idx[(i + 1, j + 1)], idx[(i, j + 1)]))
-        pts = np.array(pts)
-        pts /= np.linalg.norm(pts, axis=1, keepdims=True)
-        V.append(pts)
-        F.append(np.array(tris, np.int64) + off)
-        off += len(pts)
+        P = (np.outer(n - I - J, A) + np.outer(I, B) + np.outer(J, C)) / n
+        P /= np.linalg.norm(P, axis=1, keepdims=True)
+        V.append(P)
+        F.append(tri + fi * npts)
     return np.concatenate(V), np.concatenate(F)
 
 
@@ -327,3 +347,105 @@ def closest_hit_rays(mesh, W=4096, H=2048, n_incoherent=None, seed=2):
     n_inc = W * H if n_incoherent is None else n_incoherent
     oi, di = incoherent_rays(n_inc, lo, hi, seed)
     return (oc, dc), (oi, di)
+
+
+def _material(rng):
+    """70 % Lambert (albedo uniform in [0.1, 0.9]^3), 20 % mirror, 10 % glass."""
+    u = rng.random()
+    if u < 0.7:
+        return dict(type=LAMBERT, rgb=tuple(float(x) for x in rng.uniform(0.1, 0.9, 3)))
+    if u < 0.9:
+        return dict(type=MIRROR)
+    return dict(type=GLASS, ior=1.5)
+
+
+def _assemble(objects, bsdfs):
+    """objects: list of ((v0, v1, v2), bsdf index)."""
+    v0 = np.concatenate([o[0][0] for o in objects]).astype(np.float32)
+    v1 = np.concatenate([o[0][1] for o in objects]).astype(np.float32)
+    v2 = np.concatenate([o[0][2] for o in objects]).astype(np.float32)
+    tb = np.concatenate([np.full(len(o[0][0]), o[1], np.uint32) for o in objects])
+    return dict(v0=v0, v1=v1, v2=v2, tri_bsdf=tb, bsdfs=bsdfs)
+
+
+def production_scene(sphere_freq=274, n_boxes=512, box_tess=9, seed=3):
+    """Displaced geodesic sphere (20*freq^2 tris; 1,501,520 at freq 274) of radius
+    1.5 resting on a 2-triangle ground, 512 boxes with faces split into
+    box_tess^2 x 2 triangles (972 each), 4 quad lights with emission uniform in
+    [5, 20].  Each object draws its material: 70 % Lambert, 20 % mirror, 10 % glass.
+    Camera: pinhole, 40 deg vertical field of view."""
+    rng = np.random.default_rng(seed)
+    bsdfs, objects = [], []
+
+    def add_mat(m):
+        bsdfs.append(m)
+        return len(bsdfs) - 1
+
+    v, f = geodesic_sphere(sphere_freq)
+    objects.append((displaced_sphere(v, f, 0.05, seed, center=(0, 1.5, 0), radius=1.5),
+                    add_mat(_material(rng))))
+    objects.append((quad((-12, 0, -12), (24, 0, 0), (0, 0, 24), (0, 1, 0)),
+                    add_mat(dict(type=LAMBERT, rgb=(0.5, 0.5, 0.5)))))
+    placed = 0
+    while placed < n_boxes:
+        c = rng.uniform(-10, 10, 2)
+        if np.hypot(c[0], c[1]) < 2.2:
+            continue
+        s = rng.uniform(0.2, 0.7, 3)
+        objects.append((box((c[0], s[1] / 2, c[1]), s, rot_y(rng.uniform(0, 90)), tess=box_tess),
+                        add_mat(_material(rng))))
+        placed += 1
+    for i in range(4):
+        ang = i * np.pi / 2 + 0.4
+        p = np.array([5 * np.cos(ang), rng.uniform(5, 7), 5 * np.sin(ang)])
+        sz = rng.uniform(1.0, 2.0)
+        le = float(rng.uniform(5, 20))
+        objects.append((quad(p - np.array([sz / 2, 0, sz / 2]), (sz, 0, 0), (0, 0, sz), (0, -1, 0)),
+                        add_mat(dict(type=LIGHT, rgb=(le, le, le)))))
+    sc = _assemble(objects, bsdfs)
+    sc["camera"] = np.array([0.0, 3.0, -7.5, 0.0, 1.2, 0.0, 0, 1, 0, 40.0], np.float32)
+    return sc
+
+
+def batch_scene(seed):
+    """One job of the batched workload: a displaced sphere plus boxes with about
+    exp(U[ln 1e3, ln 5e5]) triangles, materials as production_scene, two lights,
+    resolution from {128^2, 256^2, 512^2, 1024^2}, spp from {16, 64, 256}."""
+    rng = np.random.default_rng(1000 + seed)
+    target = float(np.exp(rng.uniform(np.log(1e3), np.log(5e5))))
+    res = int(rng.choice([128, 256, 512, 1024]))
+    spp = int(rng.choice([16, 64, 256]))
+    bsdfs, objects = [], []
+
+    def add_mat(m):
+        bsdfs.append(m)
+        return len(bsdfs) - 1
+
+    freq = max(2, int(round(np.sqrt(0.8 * target / 20))))
+    v, f = geodesic_sphere(freq)
+    objects.append((displaced_sphere(v, f, 0.05, seed, center=(0, 1, 0), radius=1.0),
+                    add_mat(_material(rng))))
+    objects.append((quad((-6, 0, -6), (12, 0, 0), (0, 0, 12), (0, 1, 0)),
+                    add_mat(dict(type=LAMBERT, rgb=(0.5, 0.5, 0.5)))))
+    n_boxes = max(1, int(0.2 * target / 12))
+    c = rng.uniform(-5, 5, (n_boxes, 2))
+    r = np.hypot(c[:, 0], c[:, 1])
+    near = r < 1.3
+    c[near] = c[near] / np.maximum(r[near], 1e-3)[:, None] * 1.5
+    sz = rng.uniform(0.1, 0.5, (n_boxes, 3))
+    yaw = rng.uniform(0, 90, n_boxes)
+    mats = [add_mat(_material(rng)) for _ in range(n_boxes)]
+    bv = boxes_y(np.stack([c[:, 0], sz[:, 1] / 2, c[:, 1]], 1), sz, yaw)
+    for k in range(n_boxes):
+        objects.append(((bv[0][12 * k:12 * k + 12], bv[1][12 * k:12 * k + 12],
+                         bv[2][12 * k:12 * k + 12]), mats[k]))
+    for i in range(2):
+        p = np.array([(-2.5 if i == 0 else 2.0), 4.0, (-1.0 if i == 0 else 1.5)])
+        le = float(rng.uniform(5, 20))
+        objects.append((quad(p - 0.5, (1.0, 0, 0), (0, 0, 1.0), (0, -1, 0)),
+                        add_mat(dict(type=LIGHT, rgb=(le, le, le)))))
+    sc = _assemble(objects, bsdfs)
+    sc["camera"] = np.array([0.0, 2.0, -5.0, 0.0, 0.8, 0.0, 0, 1, 0, 40.0], np.float32)
+    sc["width"] = sc["height"] = res
+    sc["spp"] = spp
+    return sc
