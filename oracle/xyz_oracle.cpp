@@ -96,23 +96,34 @@ inline real hmax(real x, real y) { return (x < y) ? y : x; }
 // --------------------------------------------------------- deterministic cos/sin
 // Double-precision evaluation of cos/sin of a float angle: quadrant reduction by
 // a two-part pi/2 (P1 has 33 significant bits, so k*P1 is exact for the |k| <= 8
-// that float angles in [0, 4pi) give), then Taylor series to degree 21/22 on
-// |r| <= pi/4.
+// that float angles in [0, 4pi) give), then Horner evaluation of the Taylor
+// polynomials (sin through r^17, cos through r^18; truncation < 1e-19 on
+// |r| <= pi/4) with the coefficients (-1)^k/(2k+1)! and (-1)^k/(2k)! rounded
+// to double.
 double taylor_sin(double r) {
-  double r2 = r * r, term = r, sum = r;
-  for (int k = 1; k <= 10; ++k) {
-    term = term * (-r2) / double((2 * k) * (2 * k + 1));
-    sum = sum + term;
-  }
-  return sum;
+  double const z = r * r;
+  double p = 2.8114572543455206e-15;
+  p = -7.647163731819816e-13 + z * p;
+  p = 1.6059043836821613e-10 + z * p;
+  p = -2.505210838544172e-08 + z * p;
+  p = 2.7557319223985893e-06 + z * p;
+  p = -0.0001984126984126984 + z * p;
+  p = 0.008333333333333333 + z * p;
+  p = -0.16666666666666666 + z * p;
+  return r + (r * z) * p;
 }
 double taylor_cos(double r) {
-  double r2 = r * r, term = 1.0, sum = 1.0;
-  for (int k = 1; k <= 11; ++k) {
-    term = term * (-r2) / double((2 * k - 1) * (2 * k));
-    sum = sum + term;
-  }
-  return sum;
+  double const z = r * r;
+  double q = -1.5619206968586225e-16;
+  q = 4.779477332387385e-14 + z * q;
+  q = -1.1470745597729725e-11 + z * q;
+  q = 2.08767569878681e-09 + z * q;
+  q = -2.755731922398589e-07 + z * q;
+  q = 2.48015873015873e-05 + z * q;
+  q = -0.001388888888888889 + z * q;
+  q = 0.041666666666666664 + z * q;
+  q = -0.5 + z * q;
+  return 1.0 + z * q;
 }
 void reduce_quadrant(float xf, double* r, int* q) {
   double x = double(xf);

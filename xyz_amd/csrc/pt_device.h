@@ -50,26 +50,33 @@ __device__ __forceinline__ float3 ld3(const float4& v) { return f3(v.x, v.y, v.z
 
 // ------------------------------------------------------- deterministic cos/sin
 // cos/sin of a float angle in double precision: two-part pi/2 reduction and
-// Taylor series (degree 21/22).  The same operations run on the CPU oracle.
+// Horner evaluation of the Taylor polynomials (sin through r^17, cos through
+// r^18, coefficients rounded to double).  The same operations run on the CPU
+// oracle.
 __device__ __forceinline__ double taylor_sin(double r) {
-  double const r2 = r * r;
-  double term = r, sum = r;
-#pragma unroll
-  for (int k = 1; k <= 10; ++k) {
-    term = term * (-r2) / double((2 * k) * (2 * k + 1));
-    sum = sum + term;
-  }
-  return sum;
+  double const z = r * r;
+  double p = 2.8114572543455206e-15;
+  p = -7.647163731819816e-13 + z * p;
+  p = 1.6059043836821613e-10 + z * p;
+  p = -2.505210838544172e-08 + z * p;
+  p = 2.7557319223985893e-06 + z * p;
+  p = -0.0001984126984126984 + z * p;
+  p = 0.008333333333333333 + z * p;
+  p = -0.16666666666666666 + z * p;
+  return r + (r * z) * p;
 }
 __device__ __forceinline__ double taylor_cos(double r) {
-  double const r2 = r * r;
-  double term = 1.0, sum = 1.0;
-#pragma unroll
-  for (int k = 1; k <= 11; ++k) {
-    term = term * (-r2) / double((2 * k - 1) * (2 * k));
-    sum = sum + term;
-  }
-  return sum;
+  double const z = r * r;
+  double q = -1.5619206968586225e-16;
+  q = 4.779477332387385e-14 + z * q;
+  q = -1.1470745597729725e-11 + z * q;
+  q = 2.08767569878681e-09 + z * q;
+  q = -2.755731922398589e-07 + z * q;
+  q = 2.48015873015873e-05 + z * q;
+  q = -0.001388888888888889 + z * q;
+  q = 0.041666666666666664 + z * q;
+  q = -0.5 + z * q;
+  return 1.0 + z * q;
 }
 __device__ __forceinline__ void cos_sin(float a, float* c, float* s) {
   double const x = double(a);

@@ -719,10 +719,13 @@ int render_impl(xyz_scene* sc, const xyz_camera* cam, const xyz_render_params* p
 
 // ================================================================ batched mode
 // One persistent launch renders many jobs.  Work items are (job, tile of
-// `tile` pixels); a workgroup takes the next item from a global counter, each
-// lane takes one pixel of the tile and traces all of the job's samples for it
-// in sample order (megakernel: extension, shading and the NEE shadow ray of a
-// vertex run back to back in registers), adding them to the film in order.
+// `tile` pixels) and are taken by whole waves, so the lanes of a wave always
+// share one scene (the wave-cooperative traversal tests other lanes' rays).
+// Each lane owns one pixel of the tile and traces that pixel's samples in
+// sample order, adding them to the film in that order (integrator.cpp:119-128),
+// so films are bit-identical to the wavefront renderer's.  A lane is a small
+// state machine -- extension trace, shading, NEE shadow trace -- so that every
+// lane reaches the cooperative traversal step together.
 struct BatchJob {
   DevScene s;
   DevCamera cam;
@@ -731,65 +734,130 @@ struct BatchJob {
   uint32_t spp, sample0, n_tiles, tile_base;
 };
 
-__device__ __forceinline__ float3 trace_path(const DevScene& s, const DevCamera& cam,
-                                             const PTConst& c, uint32_t pixel, uint32_t sample,
-                                             uint32_t* stk, uint32_t stride) {
-  float4 const u0 = draw4(c, pixel, sample, 0);
-  LightVertex const lv = sample_light(s, u0.z, u0.w);
-  PathState ps;
-  ps.pos = f3(cam.eye[0], cam.eye[1], cam.eye[2]);
-  ps.dir = primary_dir(cam, pixel % cam.width, pixel / cam.width, u0.x, u0.y);
-  ps.power = f3(1.0f, 1.0f, 1.0f);
-  ps.color = f3(0.0f, 0.0f, 0.0f);
-  ps.f_next = 0.0f;
-  ps.k = 1;
-  ps.specular = true;
-  NoGrad g;
-  for (;;) {
-    HitRec const h = trace_closest<false>(s, ps.pos, ps.dir, stk, stride, nullptr, nullptr);
-    ShadowReq sr;
-    bool has_sr = false;
-    bool const cont = shade_vertex(s, c, ps, h, pixel, sample, lv, &sr, &has_sr, g);
-    if (has_sr) {
-      HitRec const hs = trace_closest<false>(s, sr.o, sr.w, stk, stride, nullptr, nullptr);
-      if (hs.prim == sr.expected) ps.color = add3(ps.color, nee_contribution(sr, hs.t, s.light_area));
-    }
-    if (!cont) break;
-  }
-  return ps.color;
-}
+enum : uint32_t { kIdle = 0, kExtend = 1, kShadow = 2 };
 
 __global__ void k_batch(const BatchJob* __restrict__ jobs, uint32_t n_jobs, uint32_t total_tiles,
                         uint32_t tile, uint32_t* work) {
   extern __shared__ uint32_t lds_stack[];
-  __shared__ uint32_t item;
+  __shared__ unsigned long long keys_all[1024];
+  unsigned long long* const keys = keys_all + (threadIdx.x & ~63u);
+  uint32_t* const stk = lds_stack + threadIdx.x;
+  uint32_t const lane = __lane_id();
+  unsigned long long const below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+
+  // wave-uniform tile state
+  uint32_t job = 0, tile_next = 0, tile_end = 0;
+  bool have_tile = false;
+  // lane state
+  uint32_t phase = kIdle, pixel = 0, smp = 0, smp_end = 0;
+  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+  PathState ps;
+  LightVertex lv;
+  ShadowReq sr;
+  bool pending_cont = false;
+  TraceState st;
+
   for (;;) {
-    if (threadIdx.x == 0) item = atomicAdd(work, 1u);
-    __syncthreads();
-    uint32_t const it = item;
-    __syncthreads();
-    if (it >= total_tiles) break;
-    // job of this tile: last job with tile_base <= it
-    uint32_t lo = 0, hi = n_jobs;
-    while (hi - lo > 1) {
-      uint32_t const mid = (lo + hi) >> 1;
-      if (jobs[mid].tile_base <= it) lo = mid; else hi = mid;
-    }
-    BatchJob const& jb = jobs[lo];
-    uint32_t const wh = jb.cam.width * jb.cam.height;
-    uint32_t const p0 = (it - jb.tile_base) * tile;
-    for (uint32_t p = p0 + threadIdx.x; p < min(p0 + tile, wh); p += blockDim.x) {
-      float* px = jb.film + 3 * size_t(p);
-      float fx = px[0], fy = px[1], fz = px[2];
-      for (uint32_t smp = jb.sample0; smp < jb.sample0 + jb.spp; ++smp) {
-        float3 const col = trace_path(jb.s, jb.cam, jb.c, p, smp, lds_stack + threadIdx.x, blockDim.x);
-        fx = fx + col.x;
-        fy = fy + col.y;
-        fz = fz + col.z;
+    // next tile once every lane is idle and the current tile is handed out
+    if (__ballot(phase != kIdle) == 0 && tile_next >= tile_end) {
+      uint32_t it = 0;
+      if (lane == 0) it = atomicAdd(work, 1u);
+      it = __shfl(it, 0);
+      if (it >= total_tiles) break;
+      uint32_t lo = 0, hi = n_jobs;
+      while (hi - lo > 1) {
+        uint32_t const mid = (lo + hi) >> 1;
+        if (jobs[mid].tile_base <= it) lo = mid; else hi = mid;
       }
-      px[0] = fx;
-      px[1] = fy;
-      px[2] = fz;
+      job = lo;
+      uint32_t const wh = jobs[job].cam.width * jobs[job].cam.height;
+      tile_next = (it - jobs[job].tile_base) * tile;
+      tile_end = min(tile_next + tile, wh);
+      have_tile = true;
+    }
+    BatchJob const& jb = jobs[job];
+    // idle lanes take the tile's next pixels
+    bool const need = have_tile && phase == kIdle;
+    unsigned long long const m = __ballot(need && tile_next < tile_end);
+    if (m != 0) {
+      uint32_t const rank = uint32_t(__popcll(m & below));
+      uint32_t const avail = tile_end - tile_next;
+      if (need && rank < avail) {
+        pixel = tile_next + rank;
+        float* px = jb.film + 3 * size_t(pixel);
+        fx = px[0];
+        fy = px[1];
+        fz = px[2];
+        smp = jb.sample0;
+        smp_end = jb.sample0 + jb.spp;
+        phase = (smp < smp_end) ? kExtend : kIdle;
+        if (phase == kExtend) {
+          float4 const u0 = draw4(jb.c, pixel, smp, 0);
+          lv = sample_light(jb.s, u0.z, u0.w);
+          ps.pos = f3(jb.cam.eye[0], jb.cam.eye[1], jb.cam.eye[2]);
+          ps.dir = primary_dir(jb.cam, pixel % jb.cam.width, pixel / jb.cam.width, u0.x, u0.y);
+          ps.power = f3(1.0f, 1.0f, 1.0f);
+          ps.color = f3(0.0f, 0.0f, 0.0f);
+          ps.f_next = 0.0f;
+          ps.k = 1;
+          ps.specular = true;
+          trace_init(st, ps.pos, ps.dir);
+        }
+      }
+      uint32_t const want = uint32_t(__popcll(m));
+      tile_next += want < avail ? want : avail;
+    }
+    bool const act = phase != kIdle;
+    bool const fin = coop_step<false>(jb.s, st, act, stk, blockDim.x, keys);
+    if (!(act && fin)) continue;
+    HitRec const h = trace_result(jb.s, st);
+    bool path_done = false;
+    if (phase == kExtend) {
+      NoGrad g;
+      bool has_sr = false;
+      bool const cont = shade_vertex(jb.s, jb.c, ps, h, pixel, smp, lv, &sr, &has_sr, g);
+      if (has_sr) {
+        phase = kShadow;
+        pending_cont = cont;
+        trace_init(st, sr.o, sr.w);
+      } else if (cont) {
+        trace_init(st, ps.pos, ps.dir);
+      } else {
+        path_done = true;
+      }
+    } else {
+      if (h.prim == sr.expected) ps.color = add3(ps.color, nee_contribution(sr, h.t, jb.s.light_area));
+      if (pending_cont) {
+        phase = kExtend;
+        trace_init(st, ps.pos, ps.dir);
+      } else {
+        path_done = true;
+      }
+    }
+    if (path_done) {
+      fx = fx + ps.color.x;
+      fy = fy + ps.color.y;
+      fz = fz + ps.color.z;
+      ++smp;
+      if (smp < smp_end) {
+        float4 const u0 = draw4(jb.c, pixel, smp, 0);
+        lv = sample_light(jb.s, u0.z, u0.w);
+        ps.pos = f3(jb.cam.eye[0], jb.cam.eye[1], jb.cam.eye[2]);
+        ps.dir = primary_dir(jb.cam, pixel % jb.cam.width, pixel / jb.cam.width, u0.x, u0.y);
+        ps.power = f3(1.0f, 1.0f, 1.0f);
+        ps.color = f3(0.0f, 0.0f, 0.0f);
+        ps.f_next = 0.0f;
+        ps.k = 1;
+        ps.specular = true;
+        phase = kExtend;
+        trace_init(st, ps.pos, ps.dir);
+      } else {
+        float* px = jb.film + 3 * size_t(pixel);
+        px[0] = fx;
+        px[1] = fy;
+        px[2] = fz;
+        phase = kIdle;
+      }
     }
   }
 }
