@@ -441,21 +441,23 @@ struct WaveFeeder {
   uint32_t base, count, next;
   __device__ __forceinline__ void init() { base = count = next = 0; }
   // Lanes with `need` receive an index (returned) or learn that the work is
-  // exhausted (returns UINT32_MAX and sets *exhausted).
+  // exhausted (returns UINT32_MAX and sets *exhausted).  At most `room` lanes
+  // of the wave receive one; indices are claimed `batch` (1..64) at a time.
   __device__ __forceinline__ uint32_t refill(bool need, uint32_t* counter, uint32_t total,
-                                            bool* exhausted) {
+                                            bool* exhausted, uint32_t room = 64u,
+                                            uint32_t batch = 64u) {
     uint32_t const lane = __lane_id();
     unsigned long long const below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     uint32_t got = 0xFFFFFFFFu;
     for (;;) {
       unsigned long long const m = __ballot(need);
-      if (m == 0) break;
+      if (m == 0 || room == 0) break;
       if (next == count) {
         uint32_t b = 0;
-        if (lane == 0) b = atomicAdd(counter, 64u);
+        if (lane == 0) b = atomicAdd(counter, batch);
         b = __shfl(b, 0);
         base = b;
-        count = (b >= total) ? 0u : ((total - b < 64u) ? total - b : 64u);
+        count = (b >= total) ? 0u : ((total - b < batch) ? total - b : batch);
         next = 0;
         if (count == 0) {
           if (need) *exhausted = true;
@@ -465,12 +467,14 @@ struct WaveFeeder {
       uint32_t const avail = count - next;
       uint32_t const rank = uint32_t(__popcll(m & below));
       uint32_t const want = uint32_t(__popcll(m));
-      uint32_t const take = want < avail ? want : avail;
+      uint32_t take = want < avail ? want : avail;
+      take = take < room ? take : room;
       if (need && rank < take) {
         got = base + next + rank;
         need = false;
       }
       next += take;
+      room -= take;
     }
     return got;
   }

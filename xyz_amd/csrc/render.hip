@@ -33,13 +33,20 @@ namespace {
 constexpr uint32_t kBlock = 256;
 constexpr uint64_t kWaveCap = 1ull << 22;
 
+// A counter alone in its 256-byte line: device-scope atomics on one line are
+// serialised, so counters that many workgroups hit at once do not share one.
+struct alignas(256) LineU32 {
+  uint32_t v;
+  uint32_t pad[63];
+};
+
+constexpr uint32_t kShadowAppend = 8;  // append[0..7]: octant queues, append[8]: shadow requests
+
 struct WaveCounters {
+  LineU32 append[9];
+  LineU32 work_extend;  // persistent extend kernel's work counter
+  LineU32 work_shadow;  // persistent shadow kernel's work counter
   uint32_t cur[8];
-  uint32_t next[8];
-  uint32_t shadow;
-  uint32_t work_extend;  // persistent extend kernel's work counter
-  uint32_t work_shadow;  // persistent shadow kernel's work counter
-  uint32_t pad[3];
   unsigned long long queries;
 };
 
@@ -198,6 +205,15 @@ __device__ __forceinline__ uint32_t queue_slot(const uint32_t* cnt, const uint32
   return 0;
 }
 
+// Rays a wave holds at once: 64 while there is work for every wave of the
+// grid; when a late bounce leaves few (and expensive) rays, they are spread
+// over all waves instead of being packed 64 to a wave.
+__device__ __forceinline__ uint32_t wave_ray_limit(uint32_t total) {
+  uint32_t const waves = gridDim.x * (blockDim.x / 64u);
+  uint32_t const per = (total + waves - 1) / waves;
+  return per < 1u ? 1u : (per > 64u ? 64u : per);
+}
+
 __global__ void __launch_bounds__(kBlock)
 k_camera(DevScene s, DevCamera cam, PTConst c, WaveBufs wb, uint64_t q0, uint32_t n, uint32_t wh,
          uint32_t sample0) {
@@ -205,10 +221,9 @@ k_camera(DevScene s, DevCamera cam, PTConst c, WaveBufs wb, uint64_t q0, uint32_
   if (i == 0) {
     wb.ctr->cur[0] = n;
     for (int o = 1; o < 8; ++o) wb.ctr->cur[o] = 0;
-    for (int o = 0; o < 8; ++o) wb.ctr->next[o] = 0;
-    wb.ctr->shadow = 0;
-    wb.ctr->work_extend = 0;
-    wb.ctr->work_shadow = 0;
+    for (int o = 0; o < 9; ++o) wb.ctr->append[o].v = 0;
+    wb.ctr->work_extend.v = 0;
+    wb.ctr->work_shadow.v = 0;
   }
   if (i >= n) return;
   uint64_t const q = q0 + i;
@@ -233,6 +248,7 @@ k_extend(DevScene s, WaveBufs wb, const uint32_t* __restrict__ qcur) {
   __syncthreads();
   __shared__ unsigned long long keys[kBlock];
   uint32_t const total = queue_total(cnt);
+  uint32_t const lim = wave_ray_limit(total);
   uint32_t* const stk = lds_stack + threadIdx.x;
   WaveFeeder feed;
   feed.init();
@@ -241,7 +257,8 @@ k_extend(DevScene s, WaveBufs wb, const uint32_t* __restrict__ qcur) {
   bool exhausted = false;
   for (;;) {
     bool const need = (slot == 0xFFFFFFFFu) && !exhausted;
-    uint32_t const j = feed.refill(need, &wb.ctr->work_extend, total, &exhausted);
+    uint32_t const busy = uint32_t(__popcll(__ballot(slot != 0xFFFFFFFFu)));
+    uint32_t const j = feed.refill(need, &wb.ctr->work_extend.v, total, &exhausted, lim - min(lim, busy), lim);
     if (j != 0xFFFFFFFFu) {
       slot = queue_slot(cnt, qcur, wb.cap, j);
       trace_init(st, ld3(wb.pos[slot]), ld3(wb.dir[slot]));
@@ -278,33 +295,24 @@ __device__ __forceinline__ HitRec unpack_hit(float4 v) {
   return h;
 }
 
-// Appends the lanes with `pred` to 8 octant queues; one atomic per wave and octant.
-__device__ __forceinline__ void wave_append_octant(bool pred, uint32_t oct, uint32_t value,
-                                                   uint32_t* counters, uint32_t* q, uint32_t cap) {
+// Workgroup-level append: lanes with `pred` reserve a place in list `key` of
+// the workgroup's LDS counters `cnt`; returns the lane's offset in that list.
+// One LDS atomic per wave and list.
+__device__ __forceinline__ uint32_t lds_wave_append(bool pred, uint32_t key, uint32_t nkeys,
+                                                    uint32_t first, uint32_t* cnt) {
   uint32_t const lane = __lane_id();
   unsigned long long const below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-#pragma unroll
-  for (uint32_t o = 0; o < 8; ++o) {
-    unsigned long long const m = __ballot(pred && oct == o);
+  uint32_t off = 0;
+  for (uint32_t o = 0; o < nkeys; ++o) {
+    unsigned long long const m = __ballot(pred && key == o);
     if (m == 0) continue;
     uint32_t const leader = __ffsll((long long)m) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counters + o, uint32_t(__popcll(m)));
-    base = __shfl(base, leader);
-    if (pred && oct == o) q[size_t(o) * cap + base + __popcll(m & below)] = value;
+    uint32_t b = 0;
+    if (lane == leader) b = atomicAdd(&cnt[first + o], uint32_t(__popcll(m)));
+    b = __shfl(b, leader);
+    if (pred && key == o) off = b + uint32_t(__popcll(m & below));
   }
-}
-
-__device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* counter) {
-  uint32_t const lane = __lane_id();
-  unsigned long long const below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  unsigned long long const m = __ballot(pred);
-  if (m == 0) return 0;
-  uint32_t const leader = __ffsll((long long)m) - 1;
-  uint32_t base = 0;
-  if (lane == leader) base = atomicAdd(counter, uint32_t(__popcll(m)));
-  base = __shfl(base, leader);
-  return base + uint32_t(__popcll(m & below));
+  return off;
 }
 
 template <bool kReplay>
@@ -312,6 +320,7 @@ __global__ void __launch_bounds__(kBlock)
 k_shade(DevScene s, PTConst c, WaveBufs wb, const uint32_t* __restrict__ qcur, uint32_t* qnext,
         uint64_t q0, uint32_t wh, uint32_t sample0, GradArgs ga) {
   __shared__ uint32_t cnt[8];
+  __shared__ uint32_t bcnt[9], bbase[9];  // this workgroup's appends: 8 octant queues + shadow
   GradLds* glds = nullptr;
   if constexpr (kReplay) {
     __shared__ GradLds glds_mem;  // only the replay instantiation reserves it
@@ -357,8 +366,19 @@ k_shade(DevScene s, PTConst c, WaveBufs wb, const uint32_t* __restrict__ qcur, u
       store_state(wb, slot, ps);
       oct = (ps.dir.x < 0.0f ? 1u : 0u) | (ps.dir.y < 0.0f ? 2u : 0u) | (ps.dir.z < 0.0f ? 4u : 0u);
     }
-    wave_append_octant(active && cont, oct, slot, wb.ctr->next, qnext, wb.cap);
-    uint32_t const r = wave_append(active && has_sr, &wb.ctr->shadow);
+    // appends: LDS counts per workgroup, then one global atomic per list
+    if (threadIdx.x < 9) bcnt[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t const qoff = lds_wave_append(active && cont, oct, 8, 0, bcnt);
+    uint32_t const soff = lds_wave_append(active && has_sr, 0, 1, kShadowAppend, bcnt);
+    __syncthreads();
+    if (threadIdx.x < 9) {
+      uint32_t const c = bcnt[threadIdx.x];
+      bbase[threadIdx.x] = c ? atomicAdd(&wb.ctr->append[threadIdx.x].v, c) : 0u;
+    }
+    __syncthreads();
+    if (active && cont) qnext[size_t(oct) * wb.cap + bbase[oct] + qoff] = slot;
+    uint32_t const r = bbase[kShadowAppend] + soff;
     if (active && has_sr) {
       wb.so[r] = make_float4(sr.o.x, sr.o.y, sr.o.z, sr.nk2);
       wb.sw[r] = make_float4(sr.w.x, sr.w.y, sr.w.z, sr.nk3);
@@ -381,7 +401,8 @@ k_shadow(DevScene s, WaveBufs wb, uint64_t q0, uint32_t wh, GradArgs ga) {
     grad_lds_init(glds_mem);
   }
   __shared__ unsigned long long keys[kBlock];
-  uint32_t const total = wb.ctr->shadow;
+  uint32_t const total = wb.ctr->append[kShadowAppend].v;
+  uint32_t const lim = wave_ray_limit(total);
   uint32_t* const stk = lds_stack + threadIdx.x;
   WaveFeeder feed;
   feed.init();
@@ -390,7 +411,8 @@ k_shadow(DevScene s, WaveBufs wb, uint64_t q0, uint32_t wh, GradArgs ga) {
   bool exhausted = false;
   for (;;) {
     bool const need = (r == 0xFFFFFFFFu) && !exhausted;
-    uint32_t const got = feed.refill(need, &wb.ctr->work_shadow, total, &exhausted);
+    uint32_t const busy = uint32_t(__popcll(__ballot(r != 0xFFFFFFFFu)));
+    uint32_t const got = feed.refill(need, &wb.ctr->work_shadow.v, total, &exhausted, lim - min(lim, busy), lim);
     if (got != 0xFFFFFFFFu) {
       r = got;
       trace_init(st, ld3(wb.so[r]), ld3(wb.sw[r]));
@@ -441,15 +463,15 @@ k_shadow(DevScene s, WaveBufs wb, uint64_t q0, uint32_t wh, GradArgs ga) {
 
 __global__ void k_prep(WaveCounters* ctr) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  uint32_t t = ctr->shadow;
+  uint32_t t = ctr->append[kShadowAppend].v;
   for (int o = 0; o < 8; ++o) {
     t += ctr->cur[o];
-    ctr->cur[o] = ctr->next[o];
-    ctr->next[o] = 0;
+    ctr->cur[o] = ctr->append[o].v;
+    ctr->append[o].v = 0;
   }
-  ctr->shadow = 0;
-  ctr->work_extend = 0;
-  ctr->work_shadow = 0;
+  ctr->append[kShadowAppend].v = 0;
+  ctr->work_extend.v = 0;
+  ctr->work_shadow.v = 0;
   ctr->queries += t;
 }
 
