@@ -27,16 +27,20 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-stride", type=int, default=16,
+    ap.add_argument("--cpu-sample-stride", type=int, default=4,
                     help="the CPU baseline traces every n-th ray of each batch")
+    ap.add_argument("--cpu-single-stride", type=int, default=64,
+                    help="the single-thread CPU figure traces every n-th ray of each batch")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--skip-pt", action="store_true", help="skip the 1080p path-tracing leg")
     ap.add_argument("--skip-adjoint", action="store_true", help="skip the adjoint/primal leg")
     ap.add_argument("--skip-batch", action="store_true", help="skip the batched/sequential leg")
     ap.add_argument("--pt-spp", type=int, default=16,
                     help="samples per pixel timed in the 1080p leg (the workload renders 256)")
-    ap.add_argument("--pt-cpu-rows", type=int, default=64,
+    ap.add_argument("--pt-cpu-rows", type=int, default=128,
                     help="image rows the CPU baseline renders at 1 spp in the 1080p leg")
+    ap.add_argument("--pt-cpu-single-rows", type=int, default=8,
+                    help="image rows of the single-thread CPU figure in the 1080p leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 256 MiB device-to-device copy (the reference dispatch "
                          "that scales rocprof FETCH_SIZE / WRITE_SIZE readings)")
@@ -163,11 +167,17 @@ def main():
         scene.closest_hit(hip.DeviceArray.from_numpy(so), hip.DeviceArray.from_numpy(sd), gp,
                           stream=stream)
         same = bool(np.array_equal(gp.numpy(), r["prim"]))
+        s1 = args.cpu_single_stride
+        r1 = osc.closest_hit(np.concatenate([oc[::s1], oi[::s1]]),
+                             np.concatenate([dc[::s1], di[::s1]]), threads=1)
+        n1 = len(r1["prim"])
         cpu = dict(value=round(len(so) / secs / 1e6, 3), unit="Mrays/s", cores=threads,
                    kind="port",
                    sample=f"every {stride}th ray of both batches ({len(so)} rays) on the full "
                           f"mesh; oracle kd build {build_s:.1f}s excluded; "
-                          f"GPU ids identical on the sample: {same}")
+                          f"GPU ids identical on the sample: {same}",
+                   single_thread=dict(value=round(n1 / r1["seconds"] / 1e6, 3), cores=1,
+                                      sample=f"every {s1}th ray of both batches ({n1} rays)"))
         log(f"[bench] cpu baseline: {cpu}")
 
     pt = None if args.skip_pt else bench_pt(args, hip, capi, scenes)
@@ -245,11 +255,17 @@ def bench_pt(args, hip, capi, scenes):
         same = bool(np.array_equal(gfilm.numpy()[rows[0]:rows[1]].view(np.uint32),
                                    ofilm[rows[0]:rows[1]].view(np.uint32)))
         n = W * args.pt_cpu_rows
+        rows1 = (y0, y0 + args.pt_cpu_single_rows)
+        _, ost1 = o.render(sc["camera"], W, H, 1, D, rr_start=R, seed=1, rows=rows1,
+                           threads=1, stats=True)
+        n1 = W * args.pt_cpu_single_rows
         res["cpu_baseline"] = dict(
             value=round(n / ost["seconds"] / 1e6, 4), unit="Msamples/s", cores=threads,
             kind="port", sample=f"rows {rows[0]}-{rows[1] - 1} of the same frame at 1 spp "
                                 f"({n} samples); oracle kd build {build_s:.1f}s excluded; "
-                                f"GPU film bit-identical on those rows: {same}")
+                                f"GPU film bit-identical on those rows: {same}",
+            single_thread=dict(value=round(n1 / ost1["seconds"] / 1e6, 4), cores=1,
+                               sample=f"rows {rows1[0]}-{rows1[1] - 1} at 1 spp ({n1} samples)"))
         res["vs_cpu"] = round(rate / res["cpu_baseline"]["value"], 1)
         log(f"[bench] pt cpu baseline: {res['cpu_baseline']}")
     return res

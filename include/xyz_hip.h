@@ -129,13 +129,18 @@ int xyz_closest_hit(xyz_scene *scene, uint32_t n, const float *org, const float 
                     uint32_t *counters, void *stream);
 
 /* ---- path tracing -------------------------------------------------------------
- * Pinhole camera (reference ThinLensCamera pinhole mode,
- * xyz/core/thinlenscamera.cpp:71-124) with an explicit vertical field of view:
- * the lens-film distance is V = 0.0125 / tan(fov_y / 2). */
+ * Camera (reference ThinLensCamera, xyz/core/thinlenscamera.cpp:71-205):
+ *   focal_length == 0: pinhole with an explicit vertical field of view, the
+ *     lens-film distance is V = 0.0125 / tan(fov_y / 2);
+ *   focal_length > 0: the reference's default thin lens (its defaults are
+ *     focal_length 0.035, f_number 1.4): focus distance L = |eye - at|,
+ *     V = L f / (L - f), lens radius f / (2 f_number); fov_y_deg is unused.
+ *     The lens position takes two numbers of a separate random stream. */
 typedef struct {
   float eye[3], at[3], up[3];
   float fov_y_deg;
   uint32_t width, height;
+  float focal_length, f_number;
 } xyz_camera;
 
 /* Integrator settings (reference constants xyz/core/config.hpp:30,60-61 made

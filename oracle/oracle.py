@@ -56,6 +56,9 @@ def lib():
                                         ctypes.c_int, _f32p]
         L.orc_sample_light.restype = ctypes.c_int
         L.orc_sample_light.argtypes = [_vp, ctypes.c_float, ctypes.c_float, _f32p]
+        L.orc_primary_ray.argtypes = [_f32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float,
+                                      ctypes.c_float, ctypes.c_float, ctypes.c_float, _f32p, _f32p]
+        L.orc_primary_ray.restype = None
         L.orc_primary_dir.argtypes = [_f32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float,
                                       ctypes.c_float, _f32p]
         L.orc_render.restype = ctypes.c_double
@@ -203,7 +206,7 @@ class OracleScene:
     def render(self, camera, W, H, spp, max_depth, rr_start=0, rr_threshold=0.2, seed=0,
                sample0=0, rows=None, threads=0, film=None, stats=False):
         """Sum of `spp` sample colours per pixel, (H, W, 3) float32, row 0 = bottom."""
-        cam = np.asarray(camera, np.float32).reshape(10)
+        cam = camera12(camera)
         if film is None:
             film = np.zeros((H, W, 3), np.float32)
         y0, y1 = (0, H) if rows is None else rows
@@ -217,7 +220,7 @@ class OracleScene:
 
     def render_backward(self, camera, W, H, spp, max_depth, d_film, rr_start=0,
                         rr_threshold=0.2, seed=0, sample0=0, threads=0):
-        cam = np.asarray(camera, np.float32).reshape(10)
+        cam = camera12(camera)
         d_film = _c(d_film, np.float32).reshape(H, W, 3)
         d_tex = np.zeros(max(self.n_texels, 1), np.float64)
         d_emit = np.zeros(self.n_bsdfs * 3, np.float64)
@@ -251,8 +254,30 @@ def cos_sin(x):
     return c, s
 
 
+def camera12(camera):
+    """eye[3], at[3], up[3], fov_y_deg[, focal_length, f_number] -> 12 float32
+    (focal_length 0 = pinhole)."""
+    c = np.zeros(12, np.float32)
+    a = np.asarray(camera, np.float32).reshape(-1)
+    if a.size not in (10, 12):
+        raise ValueError("camera: 10 or 12 floats")
+    c[:a.size] = a
+    return c
+
+
+def primary_ray(camera, W, H, s, t, ls=0.0, lt=0.0):
+    """(origin, direction) of the primary ray through film point (s, t) and,
+    for a thin lens, lens numbers (ls, lt)."""
+    cam = camera12(camera)
+    o = np.zeros(3, np.float32)
+    d = np.zeros(3, np.float32)
+    lib().orc_primary_ray(_p(cam, _f32p), W, H, ctypes.c_float(s), ctypes.c_float(t),
+                          ctypes.c_float(ls), ctypes.c_float(lt), _p(o, _f32p), _p(d, _f32p))
+    return o, d
+
+
 def primary_dir(camera, W, H, s, t):
-    cam = np.asarray(camera, np.float32).reshape(10)
+    cam = camera12(camera)
     out = np.zeros(3, np.float32)
     lib().orc_primary_dir(_p(cam, _f32p), W, H, s, t, _p(out, _f32p))
     return out

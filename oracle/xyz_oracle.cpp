@@ -136,6 +136,24 @@ void reduce_quadrant(float xf, double* r, int* q) {
   long long ki = (long long)k;
   *q = int(((ki % 4) + 4) % 4);
 }
+// cos and sin of a double angle in [0, 2pi] (the thin lens's disk angle,
+// xyz/core/thinlenscamera.cpp:4-14, evaluates std::cos/std::sin in double):
+// the same reduction and polynomials, kept in double.
+void cos_sin_d(double x, double* c, double* s) {
+  double const k = std::floor(x * 0.63661977236758134308 + 0.5);
+  const double P1 = 1.5707963267341256141662597656;
+  const double P1t = 6.07710050650619224932e-11;
+  double const r = (x - k * P1) - k * P1t;
+  long long const ki = (long long)k;
+  int const q = int(((ki % 4) + 4) % 4);
+  double const ts = taylor_sin(r), tc = taylor_cos(r);
+  switch (q) {
+    case 0: *c = tc; *s = ts; break;
+    case 1: *c = -ts; *s = tc; break;
+    case 2: *c = -tc; *s = -ts; break;
+    default: *c = ts; *s = -tc; break;
+  }
+}
 float xyz_cos(float a) {
   double r; int q;
   reduce_quadrant(a, &r, &q);
@@ -184,8 +202,9 @@ U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
 }
 inline real u01(uint32_t x) { return real(x >> 8) * real(1.0f / 16777216.0f); }
 struct Draw { real u[4]; };
-Draw draw(uint32_t pixel, uint64_t seed, uint32_t sample, uint32_t bounce) {
-  U4 c; c.v[0] = sample; c.v[1] = bounce; c.v[2] = uint32_t(seed >> 32); c.v[3] = 0;
+// stream 0: pixel jitter, light and per-bounce numbers; stream 1: lens position
+Draw draw(uint32_t pixel, uint64_t seed, uint32_t sample, uint32_t bounce, uint32_t stream = 0) {
+  U4 c; c.v[0] = sample; c.v[1] = bounce; c.v[2] = uint32_t(seed >> 32); c.v[3] = stream;
   U4 r = philox4x32_10(c, pixel, uint32_t(seed));
   Draw d;
   for (int i = 0; i < 4; ++i) d.u[i] = u01(r.v[i]);
@@ -527,9 +546,14 @@ void sample_light(Scene const& sc, real s, real t, Vertex* lv, int* light_index)
 struct Camera {
   V3 eye, back, right, up;
   real film, aspect, v;  // film size F, aspect A, lens-film distance V
+  bool thin = false;     // thin-lens model (the reference's default camera)
+  real focus, lens_r, focal_plane;  // L, R = f/(2 F#), F L / V
 };
 
-// xyz/core/thinlenscamera.cpp:71-90 with an explicit vertical field of view
+// xyz/core/thinlenscamera.cpp:71-90.  cam = eye[3], at[3], up[3], fov_y_deg,
+// focal_length, f_number.  focal_length 0: pinhole with an explicit vertical
+// field of view (V = 0.0125 / tan(fov/2)); focal_length > 0: the reference's
+// thin lens, V = L f / (L - f), focus distance L = |eye - at|, fov unused.
 Camera make_camera(float const* cam, uint32_t W, uint32_t H) {
   Camera c;
   V3 eye(cam[0], cam[1], cam[2]), at(cam[3], cam[4], cam[5]), up(cam[6], cam[7], cam[8]);
@@ -537,8 +561,18 @@ Camera make_camera(float const* cam, uint32_t W, uint32_t H) {
   c.eye = eye;
   c.film = real(0.025);
   c.aspect = static_cast<real>(int(W)) / int(H);
-  c.v = real(double(c.film) * 0.5 / std::tan(double(fov) * 3.14159265358979323846 / 360.0));
   V3 const view = eye - at;
+  if (cam[10] > 0.0f) {
+    real const f = cam[10], fnum = cam[11];
+    c.thin = true;
+    c.focus = std::sqrt(dot(view, view));     // hi::length
+    c.v = c.focus * f / (c.focus - f);         // thinlenscamera.cpp:74-75
+    c.lens_r = f / (2 * fnum);                 // :76
+    c.focal_plane = c.film * c.focus / c.v;    // :192-193
+  } else {
+    c.v = real(double(c.film) * 0.5 / std::tan(double(fov) * 3.14159265358979323846 / 360.0));
+    c.focus = c.lens_r = c.focal_plane = 0;
+  }
   c.back = normalize(view);
   c.right = normalize(cross(up, c.back));
   c.up = cross(c.back, c.right);
@@ -551,6 +585,27 @@ V3 primary_dir(Camera const& c, real s, real t) {
   real const ly = real((double(t) - 0.5) * double(c.film));
   real const lz = -c.v;
   return normalize((lx * c.right + ly * c.up) + lz * c.back);
+}
+
+// Primary ray of either model.  Thin lens: SampleDisk (thinlenscamera.cpp:4-14)
+// with (ls, lt), origin on the lens (:159-181), direction through the focus
+// plane (:186-205).
+void primary_ray(Camera const& c, real s, real t, real ls, real lt, V3* org, V3* dir) {
+  if (!c.thin) {
+    *org = c.eye;
+    *dir = primary_dir(c, s, t);
+    return;
+  }
+  real const radius = c.lens_r * std::sqrt(ls);
+  double cs, sn;
+  cos_sin_d((2 * 3.14159265358979323846) * double(lt), &cs, &sn);
+  real const lx = real(double(radius) * cs);
+  real const ly = real(double(radius) * sn);
+  *org = (c.eye + lx * c.right) + ly * c.up;
+  real const dx = real((double(s) - 0.5) * double(c.focal_plane) * double(c.aspect) - double(lx));
+  real const dy = real((double(t) - 0.5) * double(c.focal_plane) - double(ly));
+  real const dz = -c.focus;
+  *dir = normalize((dx * c.right + dy * c.up) + dz * c.back);
 }
 
 V3 albedo_at(Scene const& sc, Bsdf const& b, Tri const& tr, Hit const& h, int* texel) {
@@ -603,10 +658,15 @@ V3 pt_path(Scene const& sc, Camera const& cam, PTParams const& pp, uint32_t W, u
   if (light_n >= 0) lv.power = sc.bsdfs[sc.tris[size_t(lv.tri)].bsdf].phi;
 
   Vertex ev;
-  ev.pos = cam.eye;
   ev.power = V3(1, 1, 1);
   ev.f_next = 0;
-  ev.dir = primary_dir(cam, sx, sy);
+  if (cam.thin) {
+    // two more numbers after the light's (scene_pathtracing.cpp:24-26)
+    Draw const dl = draw(pixel, pp.seed, sample, 0, 1);
+    primary_ray(cam, sx, sy, dl.u[0], dl.u[1], &ev.pos, &ev.dir);
+  } else {
+    primary_ray(cam, sx, sy, 0, 0, &ev.pos, &ev.dir);
+  }
   ev.specular = true;
 
   V3 color(0, 0, 0);
@@ -1013,6 +1073,15 @@ int orc_sample_light(void* s, float u, float v, float* out9) {
   if (lv.tri < 0) return -1;
   for (int k = 0; k < 3; ++k) { out9[k] = lv.pos[k]; out9[3 + k] = lv.ns[k]; out9[6 + k] = lv.ng[k]; }
   return lv.tri;
+}
+
+// cam: 12 floats (see make_camera); (ls, lt) = lens numbers, unused by the pinhole.
+void orc_primary_ray(const float* cam, uint32_t W, uint32_t H, float s, float t, float ls,
+                     float lt, float* org3, float* dir3) {
+  Camera const c = make_camera(cam, W, H);
+  V3 o, d;
+  primary_ray(c, s, t, ls, lt, &o, &d);
+  for (int k = 0; k < 3; ++k) { org3[k] = o[k]; dir3[k] = d[k]; }
 }
 
 void orc_primary_dir(const float* cam, uint32_t W, uint32_t H, float s, float t, float* out3) {

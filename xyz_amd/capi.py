@@ -58,7 +58,8 @@ class SceneInfo(ctypes.Structure):
 class Camera(ctypes.Structure):
     _fields_ = [("eye", ctypes.c_float * 3), ("at", ctypes.c_float * 3),
                 ("up", ctypes.c_float * 3), ("fov_y_deg", ctypes.c_float),
-                ("width", ctypes.c_uint32), ("height", ctypes.c_uint32)]
+                ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+                ("focal_length", ctypes.c_float), ("f_number", ctypes.c_float)]
 
 
 class RenderParams(ctypes.Structure):
@@ -299,8 +300,12 @@ class Scene:
 
 
 def make_camera(camera, W, H):
+    """camera: eye[3], at[3], up[3], fov_y_deg[, focal_length, f_number]
+    (focal_length 0 or absent: pinhole; > 0: thin lens, fov unused)."""
     c = Camera()
-    cam = np.asarray(camera, np.float32).reshape(10)
+    cam = np.asarray(camera, np.float32).reshape(-1)
+    if cam.size not in (10, 12):
+        raise ValueError("camera: 10 or 12 floats")
     for k in range(3):
         c.eye[k] = cam[k]
         c.at[k] = cam[3 + k]
@@ -308,6 +313,9 @@ def make_camera(camera, W, H):
     c.fov_y_deg = cam[9]
     c.width = W
     c.height = H
+    if cam.size == 12:
+        c.focal_length = cam[10]
+        c.f_number = cam[11]
     return c
 
 

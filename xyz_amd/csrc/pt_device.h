@@ -21,6 +21,8 @@ struct DevCamera {
   float eye[3], right[3], up[3], back[3];
   float film, aspect, v;
   uint32_t width, height;
+  uint32_t thin;                     // thin-lens model
+  float focus, lens_r, focal_plane;  // L, f / (2 F#), F L / V
 };
 
 struct PTConst {
@@ -78,6 +80,23 @@ __device__ __forceinline__ double taylor_cos(double r) {
   q = -0.5 + z * q;
   return 1.0 + z * q;
 }
+// cos and sin of a double angle (the thin lens's disk angle, evaluated in double
+// by xyz/core/thinlenscamera.cpp:4-14), results kept in double.
+__device__ __forceinline__ void cos_sin_d(double x, double* c, double* s) {
+  double const k = floor(x * 0.63661977236758134308 + 0.5);
+  double const P1 = 1.5707963267341256141662597656;
+  double const P1t = 6.07710050650619224932e-11;
+  double const r = (x - k * P1) - k * P1t;
+  long long const ki = (long long)k;
+  int const q = int(((ki % 4) + 4) % 4);
+  double const ts = taylor_sin(r), tc = taylor_cos(r);
+  switch (q) {
+    case 0: *c = tc; *s = ts; break;
+    case 1: *c = -ts; *s = tc; break;
+    case 2: *c = -tc; *s = -ts; break;
+    default: *c = ts; *s = -tc; break;
+  }
+}
 __device__ __forceinline__ void cos_sin(float a, float* c, float* s) {
   double const x = double(a);
   double const k = floor(x * 0.63661977236758134308 + 0.5);
@@ -114,9 +133,10 @@ __device__ __forceinline__ uint4 philox(uint4 c, uint32_t k0, uint32_t k1) {
   return c;
 }
 __device__ __forceinline__ float u01(uint32_t x) { return float(x >> 8) * (1.0f / 16777216.0f); }
+// stream 0: pixel jitter, light and per-bounce numbers; stream 1: lens position
 __device__ __forceinline__ float4 draw4(const PTConst& c, uint32_t pixel, uint32_t sample,
-                                        uint32_t bounce) {
-  uint4 const r = philox(make_uint4(sample, bounce, c.seed_hi, 0u), pixel, c.seed_lo);
+                                        uint32_t bounce, uint32_t stream = 0u) {
+  uint4 const r = philox(make_uint4(sample, bounce, c.seed_hi, stream), pixel, c.seed_lo);
   return make_float4(u01(r.x), u01(r.y), u01(r.z), u01(r.w));
 }
 
@@ -133,6 +153,38 @@ __device__ __forceinline__ float3 primary_dir(const DevCamera& cam, uint32_t px,
   float3 const u = f3(cam.up[0], cam.up[1], cam.up[2]);
   float3 const b = f3(cam.back[0], cam.back[1], cam.back[2]);
   return normalize3(add3(add3(scl3l(lx, r), scl3l(ly, u)), scl3l(lz, b)));
+}
+
+// Primary ray of either camera model for pixel (px, py) of sample `sample`:
+// jitter from draw4(.., 0).xy; the thin lens samples the disk
+// (thinlenscamera.cpp:4-14) with stream 1, starts on the lens (:159-181) and
+// aims through the focus plane (:186-205).
+__device__ __forceinline__ void primary_ray(const DevCamera& cam, const PTConst& c, uint32_t pixel,
+                                            uint32_t sample, float jx, float jy, float3* org,
+                                            float3* dir) {
+  uint32_t const px = pixel % cam.width, py = pixel / cam.width;
+  float3 const eye = f3(cam.eye[0], cam.eye[1], cam.eye[2]);
+  if (!cam.thin) {
+    *org = eye;
+    *dir = primary_dir(cam, px, py, jx, jy);
+    return;
+  }
+  float4 const ul = draw4(c, pixel, sample, 0u, 1u);
+  float const s = (float(int(px)) + jx) / float(int(cam.width));
+  float const t = (float(int(py)) + jy) / float(int(cam.height));
+  float const radius = cam.lens_r * sqrtf(ul.x);
+  double cs, sn;
+  cos_sin_d((2 * 3.14159265358979323846) * double(ul.y), &cs, &sn);
+  float const lx = float(double(radius) * cs);
+  float const ly = float(double(radius) * sn);
+  float3 const r = f3(cam.right[0], cam.right[1], cam.right[2]);
+  float3 const u = f3(cam.up[0], cam.up[1], cam.up[2]);
+  float3 const b = f3(cam.back[0], cam.back[1], cam.back[2]);
+  *org = add3(add3(eye, scl3l(lx, r)), scl3l(ly, u));
+  float const dx = float((double(s) - 0.5) * double(cam.focal_plane) * double(cam.aspect) - double(lx));
+  float const dy = float((double(t) - 0.5) * double(cam.focal_plane) - double(ly));
+  float const dz = -cam.focus;
+  *dir = normalize3(add3(add3(scl3l(dx, r), scl3l(dy, u)), scl3l(dz, b)));
 }
 
 // ------------------------------------------------------------ light sample

@@ -230,8 +230,9 @@ k_camera(DevScene s, DevCamera cam, PTConst c, WaveBufs wb, uint64_t q0, uint32_
   uint32_t const pixel = uint32_t(q % wh);
   uint32_t const sample = sample0 + uint32_t(q / wh);
   float4 const u = draw4(c, pixel, sample, 0);
-  float3 const d = primary_dir(cam, pixel % cam.width, pixel / cam.width, u.x, u.y);
-  wb.pos[i] = make_float4(cam.eye[0], cam.eye[1], cam.eye[2], 0.0f);
+  float3 o, d;
+  primary_ray(cam, c, pixel, sample, u.x, u.y, &o, &d);
+  wb.pos[i] = make_float4(o.x, o.y, o.z, 0.0f);
   wb.dir[i] = make_float4(d.x, d.y, d.z, __uint_as_float(pack_k(1u, true)));
   wb.pw[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
   wb.col[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -527,7 +528,7 @@ int make_dev_camera(const xyz_camera* cam, DevCamera* dc) {
     set_error("camera: zero image size");
     return XYZ_E_INVALID;
   }
-  if (!(cam->fov_y_deg > 0.0f && cam->fov_y_deg < 180.0f)) {
+  if (!(cam->focal_length > 0.0f) && !(cam->fov_y_deg > 0.0f && cam->fov_y_deg < 180.0f)) {
     set_error("camera: fov_y_deg must be in (0, 180)");
     return XYZ_E_INVALID;
   }
@@ -560,7 +561,22 @@ int make_dev_camera(const xyz_camera* cam, DevCamera* dc) {
   }
   dc->film = 0.025f;
   dc->aspect = static_cast<float>(int(cam->width)) / int(cam->height);
-  dc->v = float(double(dc->film) * 0.5 / std::tan(double(cam->fov_y_deg) * 3.14159265358979323846 / 360.0));
+  if (cam->focal_length > 0.0f) {
+    // thin lens: xyz/core/thinlenscamera.cpp:71-77, 192-193
+    float const f = cam->focal_length, fnum = cam->f_number;
+    float const L = std::sqrt(view[0] * view[0] + view[1] * view[1] + view[2] * view[2]);
+    if (!(fnum > 0.0f) || !(L > f) || !std::isfinite(L)) {
+      set_error("camera: thin lens needs f_number > 0 and |eye - at| > focal_length");
+      return XYZ_E_INVALID;
+    }
+    dc->thin = 1;
+    dc->focus = L;
+    dc->v = L * f / (L - f);
+    dc->lens_r = f / (2 * fnum);
+    dc->focal_plane = dc->film * L / dc->v;
+  } else {
+    dc->v = float(double(dc->film) * 0.5 / std::tan(double(cam->fov_y_deg) * 3.14159265358979323846 / 360.0));
+  }
   dc->width = cam->width;
   dc->height = cam->height;
   return XYZ_OK;
@@ -816,8 +832,7 @@ __global__ void k_batch(const BatchJob* __restrict__ jobs, uint32_t n_jobs, uint
         if (phase == kExtend) {
           float4 const u0 = draw4(jb.c, pixel, smp, 0);
           lv = sample_light(jb.s, u0.z, u0.w);
-          ps.pos = f3(jb.cam.eye[0], jb.cam.eye[1], jb.cam.eye[2]);
-          ps.dir = primary_dir(jb.cam, pixel % jb.cam.width, pixel / jb.cam.width, u0.x, u0.y);
+          primary_ray(jb.cam, jb.c, pixel, smp, u0.x, u0.y, &ps.pos, &ps.dir);
           ps.power = f3(1.0f, 1.0f, 1.0f);
           ps.color = f3(0.0f, 0.0f, 0.0f);
           ps.f_next = 0.0f;
@@ -864,8 +879,7 @@ __global__ void k_batch(const BatchJob* __restrict__ jobs, uint32_t n_jobs, uint
       if (smp < smp_end) {
         float4 const u0 = draw4(jb.c, pixel, smp, 0);
         lv = sample_light(jb.s, u0.z, u0.w);
-        ps.pos = f3(jb.cam.eye[0], jb.cam.eye[1], jb.cam.eye[2]);
-        ps.dir = primary_dir(jb.cam, pixel % jb.cam.width, pixel / jb.cam.width, u0.x, u0.y);
+        primary_ray(jb.cam, jb.c, pixel, smp, u0.x, u0.y, &ps.pos, &ps.dir);
         ps.power = f3(1.0f, 1.0f, 1.0f);
         ps.color = f3(0.0f, 0.0f, 0.0f);
         ps.f_next = 0.0f;
