@@ -28,21 +28,52 @@ struct DevTex {
   uint32_t w, h, offset, pad;
 };
 
+// Triangles of a leaf are stored in clusters of kClusterSize records; a leaf's
+// records occupy whole clusters (the last one may be partly used).
+#ifndef XYZ_CLUSTER_SHIFT
+#define XYZ_CLUSTER_SHIFT 4
+#endif
+#ifndef XYZ_CLUSTERS_PER_STEP
+#define XYZ_CLUSTERS_PER_STEP 16
+#endif
+constexpr uint32_t kClusterShift = XYZ_CLUSTER_SHIFT;
+constexpr uint32_t kClusterSize = 1u << kClusterShift;
+// Clusters a lane examines per cooperative step (<= 16: survivor indices are
+// packed 4 bits each in one 64-bit word).
+constexpr uint32_t kClustersPerStep = XYZ_CLUSTERS_PER_STEP;
+static_assert(kClustersPerStep >= 1 && kClustersPerStep <= 16, "survivor list holds 16 entries");
+// Interior nodes a lane may visit per cooperative step (0: no limit); a lane
+// that has not reached a leaf continues in the next step.
+#ifndef XYZ_DESCENT_BUDGET
+#define XYZ_DESCENT_BUDGET 0
+#endif
+
 // Everything a kernel needs to know about one scene; passed by value.
 //
 // HBM layout (all arrays 16-byte aligned):
-//   nodes     uint2[n_nodes]   pre-order kd nodes, 8 bytes each
-//   triA/B/C  float4[n_refs]   leaf-ordered triangle records:
+//   nodes     uint2[n_nodes]   pre-order kd nodes, 8 bytes each; a leaf is
+//                              (count << 2 | 3, first cluster)
+//   clo/chi   float4[n_cl]     cluster boxes: (lo.xyz, N), (hi.xyz, E) with
+//                              N >= max |e1||e2| and E >= max(|e1|, |e2|)
+//                              over the cluster's triangles
+//   triA/B/C  float4[n_cl*16]  triangle records in cluster order:
 //             A = (v0.xyz, n.x), B = (e1.xyz, n.y), C = (e2.xyz, n.z),
 //             n = cross(e2, e1) exactly as the reference recomputes it per test
-//   ref_prim  int32[n_refs]    primitive id of each record
+//   ref_prim  int32[n_cl*16]   primitive id of each record
+//   rec_rank  uint32[n_cl*16]  the record's position in the reference's leaf
+//                              order (the tie-break of equal distances)
+//   rank_pos  uint32[n_cl*16]  per leaf, record position of each rank
 //   prim_*    per primitive    shading data, read once per hit
 struct DevScene {
   const uint2* nodes;
+  const float4* clo;
+  const float4* chi;
   const float4* triA;
   const float4* triB;
   const float4* triC;
   const int32_t* ref_prim;
+  const uint32_t* rec_rank;
+  const uint32_t* rank_pos;
   uint32_t stack_cap;  // kd_max_depth + 1 entries of 8 bytes per lane
   // shading
   const float4* prim_ng;    // (ng.xyz, bsdf index as bits)
@@ -115,8 +146,11 @@ constexpr uint32_t kLeafChunk = XYZ_LEAF_CHUNK;
 struct TraceState {
   float3 o, d;
   float t_min, t_max;
-  uint32_t sp, node, lj, lend;
-  int32_t hit;
+  uint32_t sp, node;
+  uint32_t lj, lend;        // cooperative: next / end cluster; sequential: next rank / count
+  uint32_t lc0, ln;         // current leaf: first cluster, triangle count
+  unsigned long long key;   // cooperative: best (bits(t) << 32 | ~rank) in the leaf
+  int32_t hit;              // record position of the hit
   float f, g;
   bool back, in_leaf;
   uint32_t nn, nt;
@@ -130,6 +164,8 @@ __device__ __forceinline__ void trace_init(TraceState& st, float3 o, float3 d) {
   st.sp = 0;
   st.node = 0;
   st.lj = st.lend = 0;
+  st.lc0 = st.ln = 0;
+  st.key = ~0ull;
   st.hit = -1;
   st.f = st.g = 0.0f;
   st.back = false;
@@ -140,7 +176,7 @@ __device__ __forceinline__ void trace_init(TraceState& st, float3 o, float3 d) {
 // One double-faced ray/triangle test, xyz/geom/triangle.cpp:127-165, written
 // without branches: every quantity is computed in the reference's order and
 // the accept flag combines the reference's reject tests.
-__device__ __forceinline__ bool tri_test(float3 o, float3 d, float4 A, float4 B, float4 C,
+__host__ __device__ __forceinline__ bool tri_test(float3 o, float3 d, float4 A, float4 B, float4 C,
                                          float t_min, float t_max, float* t_out, float* f_out,
                                          float* g_out, float* det_out) {
   float const nx = A.w, ny = B.w, nz = C.w;
@@ -186,32 +222,29 @@ __device__ __forceinline__ bool trace_step(const DevScene& s, TraceState& st, ui
       w = s.nodes[st.node];
       if (kCount) st.nn++;
     }
-    st.lj = w.y;
-    st.lend = w.y + (w.x >> 2);
+    st.lc0 = w.y;
+    st.ln = w.x >> 2;
+    st.lj = 0;
+    st.lend = st.ln;
     st.in_leaf = true;
   }
-  // up to kLeafChunk triangles of the leaf, in stored order (kdtree.cpp:140-150)
+  // up to kLeafChunk triangles of the leaf, in the reference's order (kdtree.cpp:140-150)
   uint32_t const stop = (st.lend - st.lj > kLeafChunk) ? st.lj + kLeafChunk : st.lend;
-  if (st.lj < stop) {
-    float4 A = s.triA[st.lj], B = s.triB[st.lj], C = s.triC[st.lj];
-    for (uint32_t j = st.lj; j < stop; ++j) {
-      uint32_t const jn = (j + 1 < stop) ? j + 1 : j;
-      float4 const nA = s.triA[jn], nB = s.triB[jn], nC = s.triC[jn];
-      if (kCount) st.nt++;
-      float t, f, g, det;
-      if (tri_test(st.o, st.d, A, B, C, st.t_min, st.t_max, &t, &f, &g, &det)) {
-        st.t_max = t;
-        st.f = f;
-        st.g = g;
-        st.back = det > 0.0f;
-        st.hit = int32_t(j);
-      }
-      A = nA;
-      B = nB;
-      C = nC;
+  uint32_t const* const pos = s.rank_pos + (size_t(st.lc0) << kClusterShift);
+  for (uint32_t r = st.lj; r < stop; ++r) {
+    uint32_t const j = pos[r];
+    if (kCount) st.nt++;
+    float t, f, g, det;
+    if (tri_test(st.o, st.d, s.triA[j], s.triB[j], s.triC[j], st.t_min, st.t_max, &t, &f, &g,
+                 &det)) {
+      st.t_max = t;
+      st.f = f;
+      st.g = g;
+      st.back = det > 0.0f;
+      st.hit = int32_t(j);
     }
-    st.lj = stop;
   }
+  st.lj = stop;
   if (st.lj < st.lend) return false;  // more of this leaf next step
   st.in_leaf = false;
   if (st.hit >= 0 || st.sp == 0) return true;
@@ -233,30 +266,144 @@ __device__ __forceinline__ bool trace_step(const DevScene& s, TraceState& st, ui
 }
 
 // ---------------------------------------------------------------------------
+// Cluster culling.  cluster_may_hit() returns false only when no triangle of
+// cluster `ci` can be accepted by tri_test with t in [t_min, t_max].
+//
+// Error bound (first order in u = 2^-24, every operation of tri_test rounded
+// once): for an accepted triangle the computed point o + t d lies within
+//   Delta = lam (32 R + 14 E) + 2.3 u R + 5.4 u E
+// of the triangle, where lam = u |d| N / 1e-6 (an accepted |det| is at least
+// 1e-6, and N = |e1||e2| bounds |n| and the rounding of n against
+// cross(e2, e1)), R bounds both |v0 - o| and |t d| (R = |o - box centre| +
+// half diagonal, padded), E = max(|e1|, |e2|), provided lam <= 0.01.  The
+// box is expanded by twice the lam term plus 8 u (R + E) + 4 u (largest box
+// coordinate) + 1e-30, and the slab interval is widened by 8 ulp, which covers
+// the box test's own rounding.  Rays or clusters outside the range where no
+// operation of tri_test can overflow (R < 1e12, 1e-12 < |d| < 1e12) or with
+// lam > 0.01 are never culled.
+// Scale of the culling margin: 1 in the library; the exactness test builds a
+// copy with 0 to show that it detects a missing margin.
+#ifndef XYZ_CULL_MARGIN
+#define XYZ_CULL_MARGIN 1.0f
+#endif
+
+// sqrt and reciprocal within 1 ulp (the hardware instructions on the device);
+// cluster_may_hit's margins cover their error.
+__host__ __device__ __forceinline__ float approx_sqrt(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_sqrtf(x);
+#else
+  return sqrtf(x);
+#endif
+}
+__host__ __device__ __forceinline__ float approx_rcp(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rcpf(x);
+#else
+  return 1.0f / x;
+#endif
+}
+
+__host__ __device__ __forceinline__ bool cluster_may_hit(float4 L, float4 H, float3 o, float3 d,
+                                                         float dlen, float3 inv, float t_min,
+                                                         float t_max) {
+  float const u = 5.9604645e-08f;
+  float const lam = 0.059604645f * dlen * L.w;  // u |d| N / 1e-6
+  float const cx = 0.5f * (L.x + H.x), cy = 0.5f * (L.y + H.y), cz = 0.5f * (L.z + H.z);
+  float const hx = 0.5f * (H.x - L.x), hy = 0.5f * (H.y - L.y), hz = 0.5f * (H.z - L.z);
+  float const ox = o.x - cx, oy = o.y - cy, oz = o.z - cz;
+  float const R = (approx_sqrt(ox * ox + oy * oy + oz * oz) + approx_sqrt(hx * hx + hy * hy + hz * hz)) * 1.000002f;
+  if (!(lam <= 0.01f) || !(R < 1e12f) || !(dlen < 1e12f) || !(dlen > 1e-12f)) return true;
+  float const big = fmaxf(fmaxf(fmaxf(fabsf(L.x), fabsf(H.x)), fmaxf(fabsf(L.y), fabsf(H.y))),
+                          fmaxf(fabsf(L.z), fabsf(H.z)));
+  float const D = XYZ_CULL_MARGIN *
+                      (2.0f * lam * (32.0f * R + 14.0f * H.w) + 8.0f * u * (R + H.w) + 4.0f * u * big) +
+                  1e-30f;
+  if (!(D < 1e30f)) return true;
+  float te = -__builtin_inff(), tx = __builtin_inff();
+  float const lo[3] = {L.x - D, L.y - D, L.z - D};
+  float const hi[3] = {H.x + D, H.y + D, H.z + D};
+  float const oa[3] = {o.x, o.y, o.z};
+  float const da[3] = {d.x, d.y, d.z};
+  float const ia[3] = {inv.x, inv.y, inv.z};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (da[a] == 0.0f) {
+      if ((oa[a] < lo[a]) || (oa[a] > hi[a])) return false;
+    } else {
+      if (!(fabsf(ia[a]) < 3e38f)) return true;
+      float const p = (lo[a] - oa[a]) * ia[a], q = (hi[a] - oa[a]) * ia[a];
+      te = fmaxf(te, fminf(p, q));
+      tx = fminf(tx, fmaxf(p, q));
+    }
+  }
+  te = te - fabsf(te) * 5e-7f;
+  tx = tx + fabsf(tx) * 5e-7f;
+  return !((te > t_max) || (tx < t_min) || (te > tx));
+}
+
+// ---------------------------------------------------------------------------
 // Wave-cooperative variant of trace_step, for persistent kernels in which all
 // 64 lanes of a wave call it together (`active` marks lanes with a query).
 //
 // Within one kd leaf the reference's sequential loop ("accept when t_min <= t
 // <= t_max, then t_max = t"; kdtree.cpp:140-150, triangle.cpp:143) returns the
 // valid triangle with the smallest t, ties going to the later triangle, and
-// that triangle's t -- for any interval bounds that are not NaN and any t that
-// is not NaN.  So the (lane, triangle) pairs of all lanes' current leaf
-// chunks can be tested in any order by any lane: the wave spreads them over
-// its 64 lanes and each ray keeps the minimum of the 64-bit key
-// (bits(t) << 32) | ~index in LDS (t >= t_min > 0, so its bits order like t).
-// A lane whose bounds are NaN runs its chunk sequentially; a NaN t stores key 0
-// and makes its lane redo the chunk sequentially.  f, g and the side are
-// recomputed from the winning triangle (the same operations give the same
-// bits).
-#ifndef XYZ_COOP_CHUNK
-#define XYZ_COOP_CHUNK 128
-#endif
-constexpr uint32_t kCoopChunk = XYZ_COOP_CHUNK;
+// that triangle's t -- for interval bounds that are not NaN and any t that is
+// not NaN.  So the (lane, triangle) pairs can be tested in any order by any
+// lane: each ray keeps the minimum of the 64-bit key (bits(t) << 32) | ~rank
+// in LDS across the steps of one leaf (t >= t_min > 0, so its bits order like
+// t; rank = position in the reference's leaf order).
+//
+// A step takes up to kClustersPerStep clusters of each lane's leaf; the lane
+// drops the clusters that cluster_may_hit rules out, and the wave spreads the
+// remaining clusters' triangles over its 64 lanes.  A lane whose bounds are
+// NaN, or that met a NaN t, runs the whole leaf sequentially in the
+// reference's order.  f, g and the side are recomputed from the winning
+// triangle (the same operations give the same bits).
 
-template <bool kCount>
-__device__ __forceinline__ void seq_chunk(const DevScene& s, TraceState& st, uint32_t stop) {
-  for (uint32_t j = st.lj; j < stop; ++j) {
-    if (kCount) st.nt++;
+#ifndef XYZ_COOP_BOX
+#define XYZ_COOP_BOX 1
+#endif
+
+// Orders this wave's LDS accesses before and after the call.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Exclusive prefix sum of c over the wave and its total.
+__device__ __forceinline__ void wave_prefix(uint32_t c, uint32_t* excl, uint32_t* total) {
+  uint32_t const lane = __lane_id();
+  uint32_t incl = c;
+#pragma unroll
+  for (uint32_t off = 1; off < 64; off <<= 1) {
+    uint32_t const v = __shfl_up(incl, off);
+    if (lane >= off) incl += v;
+  }
+  *excl = incl - c;
+  *total = __shfl(incl, 63);
+}
+
+// The lane that owns item k of a wave_prefix distribution: the last lane whose
+// prefix start is <= k.
+__device__ __forceinline__ uint32_t wave_owner(uint32_t excl, uint32_t k) {
+  uint32_t lo = 0, hi = 64;
+#pragma unroll
+  for (int it = 0; it < 6; ++it) {
+    uint32_t const mid = (lo + hi) >> 1;
+    uint32_t const pm = __shfl(excl, mid);
+    if (pm <= k) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// The current leaf in the reference's order, with the sequential semantics.
+__device__ __forceinline__ void leaf_sequential(const DevScene& s, TraceState& st) {
+  uint32_t const* const pos = s.rank_pos + (size_t(st.lc0) << kClusterShift);
+  for (uint32_t r = 0; r < st.ln; ++r) {
+    uint32_t const j = pos[r];
     float t, f, g, det;
     if (tri_test(st.o, st.d, s.triA[j], s.triB[j], s.triC[j], st.t_min, st.t_max, &t, &f, &g,
                  &det)) {
@@ -267,7 +414,16 @@ __device__ __forceinline__ void seq_chunk(const DevScene& s, TraceState& st, uin
       st.hit = int32_t(j);
     }
   }
-  st.lj = stop;
+  st.lj = st.lend;
+}
+
+// t_max of the current interval: the split distance of the top stack entry.
+__device__ __forceinline__ float interval_t_max(const DevScene& s, const TraceState& st,
+                                                const uint32_t* stk, uint32_t stride) {
+  if (st.sp == 0) return __builtin_inff();
+  uint32_t const pb = stk[(st.sp - 1) * stride];
+  uint32_t f2, n2;
+  return split_distance(s.nodes[pb], st.o, st.d, pb, &f2, &n2);
 }
 
 // Returns true when this lane's (active) query finished in this step.
@@ -280,7 +436,9 @@ __device__ __forceinline__ bool coop_step(const DevScene& s, TraceState& st, boo
   if (active && !st.in_leaf) {
     uint2 w = s.nodes[st.node];
     if (kCount) st.nn++;
-    while ((w.x & 3u) != 3u) {
+    uint32_t budget = XYZ_DESCENT_BUDGET;
+    bool leaf = (w.x & 3u) == 3u;
+    while (!leaf) {
       uint32_t farc, nearc;
       float const e = split_distance(w, st.o, st.d, st.node, &farc, &nearc);
       if ((e > st.t_max) || (e < 0.0f)) {
@@ -293,47 +451,97 @@ __device__ __forceinline__ bool coop_step(const DevScene& s, TraceState& st, boo
         st.t_max = e;
         st.node = nearc;
       }
+      if (XYZ_DESCENT_BUDGET != 0 && --budget == 0) break;  // st.node is loaded next step
       w = s.nodes[st.node];
       if (kCount) st.nn++;
+      leaf = (w.x & 3u) == 3u;
     }
-    st.lj = w.y;
-    st.lend = w.y + (w.x >> 2);
-    st.in_leaf = true;
-    st.hit = -1;
+    if (leaf) {
+      st.lc0 = w.y;
+      st.ln = w.x >> 2;
+      st.lj = st.lc0;
+      st.lend = st.lc0 + ((st.ln + kClusterSize - 1) >> kClusterShift);
+      st.in_leaf = true;
+      st.hit = -1;
+      st.key = ~0ull;
+      if (kCount) st.nt += st.ln;  // the reference tests every triangle of the leaf
+    }
   }
   bool const nan_bounds = (st.t_min != st.t_min) || (st.t_max != st.t_max);
-  bool const coop = active && !nan_bounds;
-  uint32_t const rem = st.lend - st.lj;
-  uint32_t const c = coop ? (rem < kCoopChunk ? rem : kCoopChunk) : 0u;
-  // exclusive prefix of c over the wave
-  uint32_t incl = c;
-#pragma unroll
-  for (uint32_t off = 1; off < 64; off <<= 1) {
-    uint32_t const v = __shfl_up(incl, off);
-    if (lane >= off) incl += v;
+  bool const coop = active && st.in_leaf && !nan_bounds;
+  // this step's clusters of the lane's leaf, and which of them may hold a hit
+  uint32_t const left = st.lend - st.lj;
+  uint32_t const ncl = coop ? (left < kClustersPerStep ? left : kClustersPerStep) : 0u;
+  unsigned long long list = 0;
+  uint32_t nsurv = 0, c = 0;
+#if XYZ_COOP_BOX
+  {
+    // the (lane, cluster) pairs are spread over the wave like the triangle
+    // pairs below; a cluster that may hold a hit sets its bit in the owner's
+    // LDS word (the first half of the owner's key slot)
+    uint32_t bexcl, btotal;
+    wave_prefix(ncl, &bexcl, &btotal);
+    uint32_t* const masks = reinterpret_cast<uint32_t*>(keys);
+    masks[2 * lane] = 0u;
+    wave_sync();
+    for (uint32_t base = 0; base < btotal; base += 64) {
+      uint32_t const k = base + lane;
+      uint32_t const owner = wave_owner(bexcl, k);
+      float3 const o = make_float3(__shfl(st.o.x, owner), __shfl(st.o.y, owner), __shfl(st.o.z, owner));
+      float3 const d = make_float3(__shfl(st.d.x, owner), __shfl(st.d.y, owner), __shfl(st.d.z, owner));
+      float const tmin = __shfl(st.t_min, owner);
+      float const tmax = __shfl(st.t_max, owner);
+      uint32_t const i = k - __shfl(bexcl, owner);
+      uint32_t const ci = __shfl(st.lj, owner) + i;
+      if (k < btotal) {
+        float const dlen = approx_sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
+        float3 const inv = make_float3(approx_rcp(d.x), approx_rcp(d.y), approx_rcp(d.z));
+        if (cluster_may_hit(s.clo[ci], s.chi[ci], o, d, dlen, inv, tmin, tmax))
+          atomicOr(&masks[2 * owner], 1u << i);
+      }
+    }
+    wave_sync();
+    uint32_t const mask = masks[2 * lane];
+    for (uint32_t i = 0; i < ncl; ++i) {
+      if ((mask >> i) & 1u) {
+        uint32_t const ci = st.lj + i;
+        list |= static_cast<unsigned long long>(i) << (4u * nsurv);
+        ++nsurv;
+        c += (ci + 1 == st.lend) ? st.ln - ((ci - st.lc0) << kClusterShift) : kClusterSize;
+      }
+    }
+    wave_sync();
   }
-  uint32_t const excl = incl - c;
-  uint32_t const total = __shfl(incl, 63);
-  keys[lane] = ~0ull;
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#else
+  if (ncl > 0) {
+    float const dlen = approx_sqrt(st.d.x * st.d.x + st.d.y * st.d.y + st.d.z * st.d.z);
+    float3 const inv = make_float3(approx_rcp(st.d.x), approx_rcp(st.d.y), approx_rcp(st.d.z));
+    for (uint32_t i = 0; i < ncl; ++i) {
+      uint32_t const ci = st.lj + i;
+      if (cluster_may_hit(s.clo[ci], s.chi[ci], st.o, st.d, dlen, inv, st.t_min, st.t_max)) {
+        list |= static_cast<unsigned long long>(i) << (4u * nsurv);
+        ++nsurv;
+        c += (ci + 1 == st.lend) ? st.ln - ((ci - st.lc0) << kClusterShift) : kClusterSize;
+      }
+    }
+  }
+#endif
+  uint32_t excl, total;
+  wave_prefix(c, &excl, &total);
+  keys[lane] = coop ? st.key : ~0ull;
+  wave_sync();
   for (uint32_t base = 0; base < total; base += 64) {
     uint32_t const k = base + lane;
-    // owner: the last lane whose prefix start is <= k
-    uint32_t lo = 0, hi = 64;
-#pragma unroll
-    for (int it = 0; it < 6; ++it) {
-      uint32_t const mid = (lo + hi) >> 1;
-      uint32_t const pm = __shfl(excl, mid);
-      if (pm <= k) lo = mid; else hi = mid;
-    }
-    uint32_t const owner = lo;
+    uint32_t const owner = wave_owner(excl, k);
     float3 const o = make_float3(__shfl(st.o.x, owner), __shfl(st.o.y, owner), __shfl(st.o.z, owner));
     float3 const d = make_float3(__shfl(st.d.x, owner), __shfl(st.d.y, owner), __shfl(st.d.z, owner));
     float const tmin = __shfl(st.t_min, owner);
     float const tmax = __shfl(st.t_max, owner);
-    uint32_t const j = __shfl(st.lj, owner) + (k - __shfl(excl, owner));
+    uint32_t const r = k - __shfl(excl, owner);
+    uint32_t const lj_o = __shfl(st.lj, owner);
+    unsigned long long const list_o = __shfl(list, owner);
+    uint32_t const ci = lj_o + uint32_t((list_o >> (4u * (r >> kClusterShift))) & 15ull);
+    uint32_t const j = (ci << kClusterShift) + (r & (kClusterSize - 1u));
     if (k < total) {
       float t, f, g, det;
       bool const ok = tri_test(o, d, s.triA[j], s.triB[j], s.triC[j], tmin, tmax, &t, &f, &g, &det);
@@ -341,44 +549,44 @@ __device__ __forceinline__ bool coop_step(const DevScene& s, TraceState& st, boo
         atomicMin(&keys[owner], 0ull);
       } else if (ok) {
         unsigned long long const key =
-            (static_cast<unsigned long long>(__float_as_uint(t)) << 32) | (0xFFFFFFFFu - j);
+            (static_cast<unsigned long long>(__float_as_uint(t)) << 32) | (0xFFFFFFFFu - s.rec_rank[j]);
         atomicMin(&keys[owner], key);
       }
     }
   }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  if (!active) return false;
+  wave_sync();
+  if (!active || !st.in_leaf) return false;
   if (coop) {
     unsigned long long const key = keys[lane];
     if (key == 0ull) {
-      seq_chunk<kCount>(s, st, st.lj + c);  // a NaN t: the sequential semantics
+      // a NaN t: the sequential semantics over the whole leaf, from its interval
+      st.t_max = interval_t_max(s, st, stk, stride);
+      st.hit = -1;
+      leaf_sequential(s, st);
     } else {
+      st.key = key;
+      st.lj += ncl;
       if (key != ~0ull) {
         st.t_max = __uint_as_float(uint32_t(key >> 32));
-        st.hit = int32_t(0xFFFFFFFFu - uint32_t(key & 0xFFFFFFFFull));
+        if (st.lj == st.lend) {
+          uint32_t const rank = 0xFFFFFFFFu - uint32_t(key & 0xFFFFFFFFull);
+          uint32_t const j = s.rank_pos[(size_t(st.lc0) << kClusterShift) + rank];
+          float t, f, g, det;
+          (void)tri_test(st.o, st.d, s.triA[j], s.triB[j], s.triC[j], st.t_min, st.t_max, &t, &f,
+                         &g, &det);
+          st.hit = int32_t(j);
+          st.f = f;
+          st.g = g;
+          st.back = det > 0.0f;
+        }
       }
-      if (kCount) st.nt += c;
-      st.lj += c;
     }
   } else {
-    uint32_t const stop = (rem > kLeafChunk) ? st.lj + kLeafChunk : st.lend;
-    seq_chunk<kCount>(s, st, stop);
+    leaf_sequential(s, st);
   }
   if (st.lj < st.lend) return false;
   st.in_leaf = false;
-  if (st.hit >= 0) {
-    if (coop) {
-      float t, f, g, det;
-      uint32_t const j = uint32_t(st.hit);
-      (void)tri_test(st.o, st.d, s.triA[j], s.triB[j], s.triC[j], st.t_min, st.t_max, &t, &f, &g, &det);
-      st.f = f;
-      st.g = g;
-      st.back = det > 0.0f;
-    }
-    return true;
-  }
+  if (st.hit >= 0) return true;
   if (st.sp == 0) return true;
   st.sp--;
   uint32_t const p = stk[st.sp * stride];
@@ -386,13 +594,7 @@ __device__ __forceinline__ bool coop_step(const DevScene& s, TraceState& st, boo
   float const e = split_distance(s.nodes[p], st.o, st.d, p, &farc, &nearc);
   st.node = farc;
   st.t_min = e;
-  if (st.sp > 0) {
-    uint32_t const pb = stk[(st.sp - 1) * stride];
-    uint32_t f2, n2;
-    st.t_max = split_distance(s.nodes[pb], st.o, st.d, pb, &f2, &n2);
-  } else {
-    st.t_max = __builtin_inff();
-  }
+  st.t_max = interval_t_max(s, st, stk, stride);
   return false;
 }
 

@@ -10,6 +10,7 @@
 #include <string>
 #include <vector>
 
+#include "cluster_bounds.h"
 #include "device_common.h"
 #include "scene_internal.h"
 #include "xyz_internal.h"
@@ -150,6 +151,93 @@ int upload_field(xyz_scene* sc, const std::vector<T>& h, const T*& field, uint64
 
 inline float f3x(const float* p, size_t i, int k) { return p[3 * i + size_t(k)]; }
 
+uint32_t spread10(uint32_t x) {  // 10 bits -> every third bit
+  x &= 0x3FFu;
+  x = (x | (x << 16)) & 0x030000FFu;
+  x = (x | (x << 8)) & 0x0300F00Fu;
+  x = (x | (x << 4)) & 0x030C30C3u;
+  x = (x | (x << 2)) & 0x09249249u;
+  return x;
+}
+
+// Device triangle layout: each leaf's triangles, in Morton order of their
+// centroids within the leaf, grouped in clusters of kClusterSize records.
+struct Clustered {
+  std::vector<float4> lo, hi;           // per cluster: (box lo, N), (box hi, E)
+  std::vector<float4> A, B, C;          // per record
+  std::vector<int32_t> prim;            // per record
+  std::vector<uint32_t> rank, rank_pos; // per record / per leaf rank
+};
+
+void cluster_leaves(const FlatTree& T, const float* v0, const std::vector<float>& nrm,
+                    const std::vector<float>& e1, const std::vector<float>& e2,
+                    std::vector<uint2>* nodes, Clustered* out) {
+  size_t n_cl = 0;
+  for (const uint2& w : *nodes)
+    if ((w.x & 3u) == 3u) n_cl += ((w.x >> 2) + kClusterSize - 1) >> kClusterShift;
+  size_t const n_rec = n_cl << kClusterShift;
+  out->lo.assign(n_cl, make_float4(0, 0, 0, 0));
+  out->hi.assign(n_cl, make_float4(0, 0, 0, 0));
+  out->A.assign(n_rec, make_float4(0, 0, 0, 0));
+  out->B.assign(n_rec, make_float4(0, 0, 0, 0));
+  out->C.assign(n_rec, make_float4(0, 0, 0, 0));
+  out->prim.assign(n_rec, -1);
+  out->rank.assign(n_rec, 0);
+  out->rank_pos.assign(n_rec, 0);
+  size_t c0 = 0;
+  std::vector<std::pair<uint32_t, uint32_t>> keys;
+  for (uint2& w : *nodes) {
+    if ((w.x & 3u) != 3u) continue;
+    uint32_t const n = w.x >> 2, off = w.y;
+    // Morton keys of the centroids (v0 + (e1 + e2) / 3) in the leaf's centroid box
+    double cmin[3] = {INFINITY, INFINITY, INFINITY}, cmax[3] = {-INFINITY, -INFINITY, -INFINITY};
+    auto centroid = [&](size_t i, int k) {
+      return double(f3x(v0, i, k)) + (double(e1[3 * i + k]) + double(e2[3 * i + k])) / 3.0;
+    };
+    for (uint32_t r = 0; r < n; ++r) {
+      size_t const i = T.leaf_prims[off + r];
+      for (int k = 0; k < 3; ++k) {
+        double const c = centroid(i, k);
+        if (c < cmin[k]) cmin[k] = c;
+        if (c > cmax[k]) cmax[k] = c;
+      }
+    }
+    keys.clear();
+    for (uint32_t r = 0; r < n; ++r) {
+      size_t const i = T.leaf_prims[off + r];
+      uint32_t q[3];
+      for (int k = 0; k < 3; ++k) {
+        double const ext = cmax[k] - cmin[k];
+        double const x = (ext > 0.0) ? (centroid(i, k) - cmin[k]) / ext : 0.0;
+        q[k] = (x >= 0.0 && x <= 1.0) ? uint32_t(x * 1023.0) : 0u;  // NaN -> 0
+      }
+      keys.push_back({spread10(q[0]) | (spread10(q[1]) << 1) | (spread10(q[2]) << 2), r});
+    }
+    std::stable_sort(keys.begin(), keys.end());
+    size_t const p0 = c0 << kClusterShift;
+    for (uint32_t m = 0; m < n; ++m) {
+      uint32_t const r = keys[m].second;
+      size_t const i = T.leaf_prims[off + r];
+      size_t const p = p0 + m;
+      out->A[p] = make_float4(f3x(v0, i, 0), f3x(v0, i, 1), f3x(v0, i, 2), nrm[3 * i]);
+      out->B[p] = make_float4(e1[3 * i], e1[3 * i + 1], e1[3 * i + 2], nrm[3 * i + 1]);
+      out->C[p] = make_float4(e2[3 * i], e2[3 * i + 1], e2[3 * i + 2], nrm[3 * i + 2]);
+      out->prim[p] = int32_t(i);
+      out->rank[p] = r;
+      out->rank_pos[p0 + r] = uint32_t(p);
+    }
+    uint32_t const nc = (n + kClusterSize - 1) >> kClusterShift;
+    for (uint32_t c = 0; c < nc; ++c) {
+      uint32_t const m0 = c << kClusterShift;
+      uint32_t const cnt = std::min<uint32_t>(kClusterSize, n - m0);
+      cluster_bounds(&out->A[p0 + m0], &out->B[p0 + m0], &out->C[p0 + m0], cnt, &out->lo[c0 + c],
+                     &out->hi[c0 + c]);
+    }
+    w.y = uint32_t(c0);
+    c0 += nc;
+  }
+}
+
 }  // namespace
 
 void scene_free(xyz_scene* s) {
@@ -225,17 +313,10 @@ int scene_create(const xyz_scene_desc* desc, xyz_scene** out) {
     ng[3 * i + 1] = ny * r;
     ng[3 * i + 2] = nz * r;
   }
-  std::vector<float4> A(n_refs), B(n_refs), C(n_refs);
-  std::vector<int32_t> rp(n_refs);
-  for (size_t r = 0; r < n_refs; ++r) {
-    size_t const i = T.leaf_prims[r];
-    A[r] = make_float4(f3x(desc->v0, i, 0), f3x(desc->v0, i, 1), f3x(desc->v0, i, 2), nrm[3 * i]);
-    B[r] = make_float4(e1[3 * i], e1[3 * i + 1], e1[3 * i + 2], nrm[3 * i + 1]);
-    C[r] = make_float4(e2[3 * i], e2[3 * i + 1], e2[3 * i + 2], nrm[3 * i + 2]);
-    rp[r] = int32_t(i);
-  }
   std::vector<uint2> nodes(T.nodes.size() / 2);
   for (size_t i = 0; i < nodes.size(); ++i) nodes[i] = make_uint2(T.nodes[2 * i], T.nodes[2 * i + 1]);
+  Clustered CL;
+  cluster_leaves(T, desc->v0, nrm, e1, e2, &nodes, &CL);
 
   // per-primitive shading records
   std::vector<float4> png(n), pv0(n), pe1(n), pe2(n);
@@ -354,10 +435,14 @@ int scene_create(const xyz_scene_desc* desc, xyz_scene** out) {
     }                                                  \
   } while (0)
   UP(nodes, nodes);
-  UP(A, triA);
-  UP(B, triB);
-  UP(C, triC);
-  UP(rp, ref_prim);
+  UP(CL.lo, clo);
+  UP(CL.hi, chi);
+  UP(CL.A, triA);
+  UP(CL.B, triB);
+  UP(CL.C, triC);
+  UP(CL.prim, ref_prim);
+  UP(CL.rank, rec_rank);
+  UP(CL.rank_pos, rank_pos);
   UP(png, prim_ng);
   UP(pn0, prim_n0);
   UP(ps1, prim_s1);
