@@ -92,6 +92,12 @@ def main():
     del cnt
     alg_bytes = 32 * n + 8 * nodes_sum + 40 * tris_sum
     hit_rate = float((prim.numpy() >= 0).mean())
+    # work the kernel executes: culling skips leaf clusters that cannot hold a hit
+    wk = hip.DeviceArray.zeros(4, np.uint64)
+    scene.closest_hit_work(o, d, prim, wk, stream=stream)
+    w_nodes, w_ref, w_tris, w_boxes = (int(x) for x in wk.numpy())
+    assert w_nodes == nodes_sum and w_ref == tris_sum, "counting launches disagree"
+    exec_bytes = 32 * n + 8 * nodes_sum + 40 * w_tris + 32 * w_boxes
 
     if args.copy_calibration:
         import ctypes
@@ -143,8 +149,14 @@ def main():
     log(f"[bench] split: {split}")
 
     achieved = alg_bytes / (ms_avg * 1e-3) / 1e9
+    achieved_x = exec_bytes / (ms_avg * 1e-3) / 1e9
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
-                    unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None)
+                    unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    executed=dict(achieved=round(achieved_x, 1),
+                                  frac=round(achieved_x / HBM_PEAK_GBS, 4),
+                                  bytes_per_launch=int(exec_bytes),
+                                  tris_tested_per_ray=round(w_tris / n, 2),
+                                  boxes_tested_per_ray=round(w_boxes / n, 2)))
     pmc = os.path.join(ROOT, "profiles", "closest_hit_pmc.json")
     if os.path.exists(pmc):
         with open(pmc) as fh:

@@ -128,6 +128,16 @@ int xyz_closest_hit(xyz_scene *scene, uint32_t n, const float *org, const float 
                     int32_t *prim, float *t, float *f, float *g, uint8_t *back,
                     uint32_t *counters, void *stream);
 
+/* Instrumented xyz_closest_hit (no counterpart in the reference; it extends
+ * the query counter of xyz/core/scene.hpp:232-234): the same results in prim
+ * and t (t may be NULL), and work (device, 4 x uint64, ADDED to) receives the
+ * launch's totals: kd nodes visited, triangle tests the reference performs,
+ * triangle tests executed, cluster box tests executed.  The last two differ
+ * from the second because leaf clusters that provably cannot hold an accepted
+ * hit are skipped. */
+int xyz_closest_hit_work(xyz_scene *scene, uint32_t n, const float *org, const float *dir,
+                         int32_t *prim, float *t, unsigned long long *work, void *stream);
+
 /* ---- path tracing -------------------------------------------------------------
  * Camera (reference ThinLensCamera, xyz/core/thinlenscamera.cpp:71-205):
  *   focal_length == 0: pinhole with an explicit vertical field of view, the

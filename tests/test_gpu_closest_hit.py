@@ -157,3 +157,24 @@ def test_full_size_properties(full_mesh_pair, capi_mod):
     assert np.array_equal(back, a["back"][hit])
     assert np.all(a["t"][hit] >= F(1e-5))
     assert np.all((a["f"][hit] >= 0) & (a["g"][hit] >= 0) & (a["f"][hit] + a["g"][hit] <= 1))
+
+
+def test_work_totals_match_counters(full_mesh_pair, capi_mod):
+    """xyz_closest_hit_work returns the same hits and, as launch totals, the sums
+    of the per-ray reference counters; culling executes fewer triangle tests
+    than the reference performs."""
+    from xyz_amd.hip import DeviceArray
+    m, osc, gsc = full_mesh_pair
+    (oc, dc), (oi, di) = S.closest_hit_rays(m, 256, 128, n_incoherent=32768, seed=5)
+    o = np.concatenate([oc, oi])
+    d = np.concatenate([dc, di])
+    a = _run_gpu(capi_mod, gsc, o, d, counters=True)
+    do, dd = DeviceArray.from_numpy(o), DeviceArray.from_numpy(d)
+    prim = DeviceArray(len(o), np.int32)
+    work = DeviceArray.zeros(4, np.uint64)
+    gsc.closest_hit_work(do, dd, prim, work)
+    nodes, ref_tests, tests, boxes = (int(x) for x in work.numpy())
+    assert np.array_equal(prim.numpy(), a["prim"])
+    assert nodes == int(a["nodes"].astype(np.int64).sum())
+    assert ref_tests == int(a["tris"].astype(np.int64).sum())
+    assert 0 < tests < ref_tests and boxes > 0

@@ -82,6 +82,7 @@ class BatchTuning(ctypes.Structure):
 # Every symbol declared in include/xyz_hip.h.
 EXPORTS = ["xyz_kdtree_build", "xyz_kdtree_info", "xyz_kdtree_export", "xyz_kdtree_destroy",
            "xyz_scene_create", "xyz_scene_get_info", "xyz_scene_destroy", "xyz_closest_hit",
+           "xyz_closest_hit_work",
            "xyz_render", "xyz_render_backward", "xyz_render_batch", "xyz_batch_autotune",
            "xyz_last_error", "xyz_version"]
 
@@ -114,6 +115,7 @@ def lib():
         L.xyz_scene_destroy.restype = None
         L.xyz_closest_hit.argtypes = [_vp, ctypes.c_uint32, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                       _vp, _vp]
+        L.xyz_closest_hit_work.argtypes = [_vp, ctypes.c_uint32, _vp, _vp, _vp, _vp, _vp, _vp]
         L.xyz_render.argtypes = [_vp, ctypes.POINTER(Camera), ctypes.POINTER(RenderParams), _vp,
                                  ctypes.POINTER(RenderStats), _vp]
         L.xyz_render_backward.argtypes = [_vp, ctypes.POINTER(Camera),
@@ -278,6 +280,14 @@ class Scene:
         check(lib().xyz_closest_hit(self.h, n, _dptr(org), _dptr(dir), _dptr(prim), _dptr(t),
                                     _dptr(f), _dptr(g), _dptr(back), _dptr(counters),
                                     _stream(stream)))
+
+    def closest_hit_work(self, org, dir, prim, work, t=None, stream=None):
+        """closest_hit that also adds the launch's totals to `work` (device
+        uint64 (4,)): nodes visited, reference triangle tests, triangle tests
+        executed, cluster box tests executed."""
+        n = org.shape[0]
+        check(lib().xyz_closest_hit_work(self.h, n, _dptr(org), _dptr(dir), _dptr(prim), _dptr(t),
+                                         _dptr(work), _stream(stream)))
 
     def render(self, camera, film, spp, max_depth, rr_start=0, rr_threshold=0.2, seed=0,
                sample_offset=0, stream=None):

@@ -153,7 +153,8 @@ struct TraceState {
   int32_t hit;              // record position of the hit
   float f, g;
   bool back, in_leaf;
-  uint32_t nn, nt;
+  uint32_t nn, nt;  // counting mode: nodes visited, reference triangle tests
+  uint32_t xt, xb;  // counting mode: triangle tests and cluster box tests executed
 };
 
 __device__ __forceinline__ void trace_init(TraceState& st, float3 o, float3 d) {
@@ -171,6 +172,7 @@ __device__ __forceinline__ void trace_init(TraceState& st, float3 o, float3 d) {
   st.back = false;
   st.in_leaf = false;
   st.nn = st.nt = 0;
+  st.xt = st.xb = 0;
 }
 
 // One double-faced ray/triangle test, xyz/geom/triangle.cpp:127-165, written
@@ -526,6 +528,10 @@ __device__ __forceinline__ bool coop_step(const DevScene& s, TraceState& st, boo
     }
   }
 #endif
+  if (kCount) {
+    st.xb += ncl;
+    st.xt += c;
+  }
   uint32_t excl, total;
   wave_prefix(c, &excl, &total);
   keys[lane] = coop ? st.key : ~0ull;
@@ -562,6 +568,7 @@ __device__ __forceinline__ bool coop_step(const DevScene& s, TraceState& st, boo
       // a NaN t: the sequential semantics over the whole leaf, from its interval
       st.t_max = interval_t_max(s, st, stk, stride);
       st.hit = -1;
+      if (kCount) st.xt += st.ln;
       leaf_sequential(s, st);
     } else {
       st.key = key;
@@ -582,6 +589,7 @@ __device__ __forceinline__ bool coop_step(const DevScene& s, TraceState& st, boo
       }
     }
   } else {
+    if (kCount) st.xt += st.ln;
     leaf_sequential(s, st);
   }
   if (st.lj < st.lend) return false;

@@ -31,7 +31,10 @@ namespace xyzhip {
 namespace {
 
 constexpr uint32_t kBlock = 256;
-constexpr uint64_t kWaveCap = 1ull << 22;
+#ifndef XYZ_WAVE_CAP_LOG2
+#define XYZ_WAVE_CAP_LOG2 25
+#endif
+constexpr uint64_t kWaveCap = 1ull << XYZ_WAVE_CAP_LOG2;
 
 // A counter alone in its 256-byte line: device-scope atomics on one line are
 // serialised, so counters that many workgroups hit at once do not share one.
@@ -614,8 +617,8 @@ int alloc_wave(uint32_t cap, bool replay, hipStream_t st, DevMem* mem, WaveBufs*
   size_t const u4 = align_up(size_t(cap) * 16);
   size_t const qsz = align_up(size_t(cap) * 8 * 4);
   size_t const total = f4 * (replay ? 10 : 9) + u4 + 2 * qsz + align_up(sizeof(WaveCounters));
-  hipError_t e = hipMallocAsync(&mem->p, total, st);
-  if (e != hipSuccess) return hip_fail(e, "hipMallocAsync(wave)");
+  hipError_t e = pool_alloc(&mem->p, total, st);
+  if (e != hipSuccess) return hip_fail(e, "pool_alloc(wave)");
   mem->st = st;
   char* b = static_cast<char*>(mem->p);
   auto take = [&](size_t n) {
@@ -640,6 +643,48 @@ int alloc_wave(uint32_t cap, bool replay, hipStream_t st, DevMem* mem, WaveBufs*
   wb->cap = cap;
   return XYZ_OK;
 }
+
+// Internal streams (default priority) that alternate the waves of a render,
+// created once per device.
+#ifndef XYZ_RENDER_LANES
+#define XYZ_RENDER_LANES 2
+#endif
+constexpr int kLanes = XYZ_RENDER_LANES;
+std::mutex g_lane_mu;
+std::map<int, std::vector<hipStream_t>> g_lane_streams;
+
+int lane_streams(hipStream_t* out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  std::lock_guard<std::mutex> lock(g_lane_mu);
+  std::vector<hipStream_t>& v = g_lane_streams[dev];
+  while (v.size() < size_t(kLanes)) {
+    hipStream_t s = nullptr;
+    e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamCreateWithFlags");
+    v.push_back(s);
+  }
+  for (int i = 0; i < kLanes; ++i) out[i] = v[size_t(i)];
+  return XYZ_OK;
+}
+
+// Events without timing, destroyed with the set.
+struct EventSet {
+  std::vector<hipEvent_t> e;
+  int create(int n) {
+    for (int i = 0; i < n; ++i) {
+      hipEvent_t x = nullptr;
+      hipError_t const r = hipEventCreateWithFlags(&x, hipEventDisableTiming);
+      if (r != hipSuccess) return hip_fail(r, "hipEventCreateWithFlags");
+      e.push_back(x);
+    }
+    return XYZ_OK;
+  }
+  ~EventSet() {
+    for (hipEvent_t x : e) (void)hipEventDestroy(x);
+  }
+};
 
 // Runs the bounce loop of one wave.  kReplay: adjoint replay pass.
 template <bool kReplay>
@@ -703,40 +748,85 @@ int render_impl(xyz_scene* sc, const xyz_camera* cam, const xyz_render_params* p
   }
   if (total > 0) {
     uint32_t const cap = uint32_t(std::min<uint64_t>(total, kWaveCap));
-    DevMem mem;
-    WaveBufs wb;
-    rc = alloc_wave(cap, backward, st, &mem, &wb);
-    if (rc != XYZ_OK) return rc;
-    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, wb.ctr);
+    uint64_t const n_waves = (total + cap - 1) / cap;
+    // consecutive waves alternate between internal streams, so that the tail
+    // of one wave (a few long paths) overlaps the full bounces of the next;
+    // films still receive the waves in sample order (events below)
+    int const lanes = n_waves > 1 ? kLanes : 1;
+    hipStream_t ls[kLanes];
+    for (int i = 0; i < kLanes; ++i) ls[i] = st;
+    if (lanes > 1) {
+      rc = lane_streams(ls);
+      if (rc != XYZ_OK) return rc;
+    }
+    DevMem mem[kLanes];
+    WaveBufs wb[kLanes];
+    for (int i = 0; i < lanes; ++i) {
+      rc = alloc_wave(cap, backward, st, &mem[i], &wb[i]);
+      if (rc != XYZ_OK) return rc;
+      hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, wb[i].ctr);
+    }
+    EventSet ev;
+    if (lanes > 1) {
+      rc = ev.create(2 * kLanes + 1);
+      if (rc != XYZ_OK) return rc;
+      hipEvent_t const fork = ev.e[2 * kLanes];
+      hipError_t e = hipEventRecord(fork, st);
+      for (int i = 0; i < lanes && e == hipSuccess; ++i) e = hipStreamWaitEvent(ls[i], fork, 0);
+      if (e != hipSuccess) return hip_fail(e, "render fork");
+    }
     GradArgs none{nullptr, nullptr, nullptr};
     GradArgs ga{d_film, d_tex, d_emit};
-    for (uint64_t q0 = 0; q0 < total; q0 += cap) {
+    for (uint64_t k = 0; k < n_waves; ++k) {
+      int const i = int(k % uint64_t(lanes));
+      hipStream_t const s = ls[i];
+      uint64_t const q0 = k * cap;
       uint32_t const n = uint32_t(std::min<uint64_t>(cap, total - q0));
       uint32_t const grid = (n + kBlock - 1) / kBlock;
-      hipLaunchKernelGGL(k_camera, dim3(grid), dim3(kBlock), 0, st, sc->dev, dc, pc, wb, q0, n,
+      hipLaunchKernelGGL(k_camera, dim3(grid), dim3(kBlock), 0, s, sc->dev, dc, pc, wb[i], q0, n,
                          uint32_t(wh), prm->sample_offset);
-      rc = run_bounces<false>(sc->dev, pc, wb, q0, n, uint32_t(wh), prm->sample_offset,
-                              prm->max_depth, none, st);
+      rc = run_bounces<false>(sc->dev, pc, wb[i], q0, n, uint32_t(wh), prm->sample_offset,
+                              prm->max_depth, none, s);
       if (rc != XYZ_OK) return rc;
       if (!backward) {
+        // wave k is added to the film after wave k - 1 (sample order per pixel)
+        if (lanes > 1 && k > 0) {
+          hipError_t const e = hipStreamWaitEvent(s, ev.e[(k - 1) % uint64_t(lanes)], 0);
+          if (e != hipSuccess) return hip_fail(e, "render wait");
+        }
         hipLaunchKernelGGL(k_accumulate, dim3(uint32_t((wh + kBlock - 1) / kBlock)), dim3(kBlock), 0,
-                           st, film, wb.col, q0, n, uint32_t(wh));
+                           s, film, wb[i].col, q0, n, uint32_t(wh));
+        if (lanes > 1) {
+          hipError_t const e = hipEventRecord(ev.e[i], s);
+          if (e != hipSuccess) return hip_fail(e, "render record");
+        }
       } else {
         // replay: same paths from their seeds, now with the primal colours known
-        hipLaunchKernelGGL(k_copy_col, dim3(grid), dim3(kBlock), 0, st, wb.ctot, wb.col, n);
-        hipLaunchKernelGGL(k_camera, dim3(grid), dim3(kBlock), 0, st, sc->dev, dc, pc, wb, q0, n,
+        hipLaunchKernelGGL(k_copy_col, dim3(grid), dim3(kBlock), 0, s, wb[i].ctot, wb[i].col, n);
+        hipLaunchKernelGGL(k_camera, dim3(grid), dim3(kBlock), 0, s, sc->dev, dc, pc, wb[i], q0, n,
                            uint32_t(wh), prm->sample_offset);
-        rc = run_bounces<true>(sc->dev, pc, wb, q0, n, uint32_t(wh), prm->sample_offset,
-                               prm->max_depth, ga, st);
+        rc = run_bounces<true>(sc->dev, pc, wb[i], q0, n, uint32_t(wh), prm->sample_offset,
+                               prm->max_depth, ga, s);
         if (rc != XYZ_OK) return rc;
       }
     }
+    if (lanes > 1) {  // join: the caller's stream waits for every lane
+      for (int i = 0; i < lanes; ++i) {
+        hipEvent_t const done = ev.e[kLanes + i];
+        hipError_t e = hipEventRecord(done, ls[i]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(st, done, 0);
+        if (e != hipSuccess) return hip_fail(e, "render join");
+      }
+    }
     if (stats) {
-      unsigned long long q = 0;
-      hipError_t e = hipMemcpyAsync(&q, &wb.ctr->queries, sizeof(q), hipMemcpyDeviceToHost, st);
+      unsigned long long q[kLanes] = {};
+      hipError_t e = hipSuccess;
+      for (int i = 0; i < lanes && e == hipSuccess; ++i)
+        e = hipMemcpyAsync(&q[i], &wb[i].ctr->queries, sizeof(q[i]), hipMemcpyDeviceToHost, st);
       if (e == hipSuccess) e = hipStreamSynchronize(st);
       if (e != hipSuccess) return hip_fail(e, "render stats");
-      stats->queries = q;
+      stats->queries = 0;
+      for (int i = 0; i < lanes; ++i) stats->queries += q[i];
     }
   }
   hipError_t const e = hipGetLastError();
@@ -1053,8 +1143,8 @@ int render_batch_impl(uint32_t n, xyz_scene* const* scenes, const xyz_camera* ca
     void* jobs_dev = nullptr;
     uint32_t* work = nullptr;
     size_t const jb = plan.jobs.size() * sizeof(BatchJob);
-    hipError_t e = hipMallocAsync(&jobs_dev, jb + 256, st);
-    if (e != hipSuccess) return hip_fail(e, "hipMallocAsync(batch)");
+    hipError_t e = pool_alloc(&jobs_dev, jb + 256, st);
+    if (e != hipSuccess) return hip_fail(e, "pool_alloc(batch)");
     work = reinterpret_cast<uint32_t*>(static_cast<char*>(jobs_dev) + align_up(jb));
     e = hipMemcpyAsync(jobs_dev, plan.jobs.data(), jb, hipMemcpyHostToDevice, st);
     if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(batch)");

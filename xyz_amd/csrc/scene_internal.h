@@ -22,9 +22,16 @@ namespace xyzhip {
 
 int hip_fail(hipError_t e, const char* what);
 
+// Stream-ordered allocation from the library's memory pool on the current
+// device.  The pool keeps freed memory (release threshold: never), so the
+// workspace of one call is reused by the next one on any stream instead of
+// being returned to the driver at every synchronization.  Free with
+// hipFreeAsync.
+hipError_t pool_alloc(void** p, size_t bytes, hipStream_t stream);
+
 int launch_closest_hit(const DevScene& s, uint32_t n, const float* org, const float* dir,
                        int32_t* prim, float* t, float* f, float* g, uint8_t* back,
-                       uint32_t* counters, hipStream_t stream);
+                       uint32_t* counters, unsigned long long* totals, hipStream_t stream);
 
 }  // namespace xyzhip
 

@@ -6,7 +6,9 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -28,6 +30,38 @@ int hip_fail(hipError_t e, const char* what) {
   return XYZ_E_DEVICE;
 }
 
+namespace {
+std::mutex g_pool_mu;
+std::map<int, hipMemPool_t> g_pools;
+}  // namespace
+
+hipError_t pool_alloc(void** p, size_t bytes, hipStream_t stream) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  hipMemPool_t pool = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_pool_mu);
+    auto it = g_pools.find(dev);
+    if (it != g_pools.end()) {
+      pool = it->second;
+    } else {
+      hipMemPoolProps props = {};
+      props.allocType = hipMemAllocationTypePinned;
+      props.handleTypes = hipMemHandleTypeNone;
+      props.location.type = hipMemLocationTypeDevice;
+      props.location.id = dev;
+      e = hipMemPoolCreate(&pool, &props);
+      if (e != hipSuccess) return e;
+      uint64_t keep = UINT64_MAX;
+      e = hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+      if (e != hipSuccess) return e;
+      g_pools[dev] = pool;
+    }
+  }
+  return hipMallocFromPoolAsync(p, bytes, pool, stream);
+}
+
 // ------------------------------------------------------------ closest hit
 constexpr uint32_t kHitBlock = 256;
 #ifndef XYZ_HIT_BLOCKS_PER_CU
@@ -45,7 +79,7 @@ closest_hit_kernel(DevScene s, uint32_t n, const float* __restrict__ org,
                    const float* __restrict__ dir, int32_t* __restrict__ prim,
                    float* __restrict__ tout, float* __restrict__ fout, float* __restrict__ gout,
                    uint8_t* __restrict__ bout, uint32_t* __restrict__ counters,
-                   uint32_t* __restrict__ work) {
+                   unsigned long long* __restrict__ totals, uint32_t* __restrict__ work) {
   extern __shared__ uint32_t lds_stack[];
   __shared__ unsigned long long keys[kHitBlock];
   uint32_t* const stk = lds_stack + threadIdx.x;
@@ -54,6 +88,7 @@ closest_hit_kernel(DevScene s, uint32_t n, const float* __restrict__ org,
   TraceState st;
   uint32_t r = 0xFFFFFFFFu;
   bool exhausted = false;
+  unsigned long long sum[4] = {0, 0, 0, 0};  // counting mode: this lane's work
   for (;;) {
     bool const need = (r == 0xFFFFFFFFu) && !exhausted;
     uint32_t const got = feed.refill(need, work, n, &exhausted);
@@ -77,10 +112,25 @@ closest_hit_kernel(DevScene s, uint32_t n, const float* __restrict__ org,
       if (gout) gout[r] = h.g;
       if (bout) bout[r] = h.back ? 1 : 0;
       if (kCount) {
-        counters[r] = st.nn;
-        counters[size_t(n) + r] = st.nt;
+        if (counters) {
+          counters[r] = st.nn;
+          counters[size_t(n) + r] = st.nt;
+        }
+        sum[0] += st.nn;
+        sum[1] += st.nt;
+        sum[2] += st.xt;
+        sum[3] += st.xb;
       }
       r = 0xFFFFFFFFu;
+    }
+  }
+  if (kCount && totals) {
+    // one atomic per wave and counter
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      unsigned long long v = sum[k];
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+      if (__lane_id() == 0) atomicAdd(&totals[k], v);
     }
   }
 }
@@ -93,7 +143,7 @@ int g_hit_cu = 0;
 
 int launch_closest_hit(const DevScene& s, uint32_t n, const float* org, const float* dir,
                        int32_t* prim, float* t, float* f, float* g, uint8_t* back,
-                       uint32_t* counters, hipStream_t stream) {
+                       uint32_t* counters, unsigned long long* totals, hipStream_t stream) {
   if (n == 0) return XYZ_OK;
   if (g_hit_cu == 0) {
     int dev = 0, v = 0;
@@ -106,16 +156,16 @@ int launch_closest_hit(const DevScene& s, uint32_t n, const float* org, const fl
   uint32_t const need_blocks = (n + kHitBlock - 1) / kHitBlock;
   uint32_t const grid = std::min<uint32_t>(need_blocks, uint32_t(g_hit_cu) * kHitBlocksPerCu);
   void* work = nullptr;
-  hipError_t e = hipMallocAsync(&work, sizeof(uint32_t), stream);
-  if (e != hipSuccess) return hip_fail(e, "hipMallocAsync(work counter)");
+  hipError_t e = pool_alloc(&work, sizeof(uint32_t), stream);
+  if (e != hipSuccess) return hip_fail(e, "pool_alloc(work counter)");
   uint32_t* w = static_cast<uint32_t*>(work);
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, w);
-  if (counters) {
+  if (counters || totals) {
     hipLaunchKernelGGL(closest_hit_kernel<true>, dim3(grid), dim3(kHitBlock), lds, stream, s, n,
-                       org, dir, prim, t, f, g, back, counters, w);
+                       org, dir, prim, t, f, g, back, counters, totals, w);
   } else {
     hipLaunchKernelGGL(closest_hit_kernel<false>, dim3(grid), dim3(kHitBlock), lds, stream, s, n,
-                       org, dir, prim, t, f, g, back, counters, w);
+                       org, dir, prim, t, f, g, back, counters, totals, w);
   }
   e = hipGetLastError();
   (void)hipFreeAsync(work, stream);
@@ -572,8 +622,22 @@ int xyz_closest_hit(xyz_scene* scene, uint32_t n, const float* org, const float*
     set_error("xyz_closest_hit: NULL ray or prim buffer");
     return XYZ_E_INVALID;
   }
-  return launch_closest_hit(scene->dev, n, org, dir, prim, t, f, g, back, counters,
+  return launch_closest_hit(scene->dev, n, org, dir, prim, t, f, g, back, counters, nullptr,
                             static_cast<hipStream_t>(stream));
+}
+
+int xyz_closest_hit_work(xyz_scene* scene, uint32_t n, const float* org, const float* dir,
+                         int32_t* prim, float* t, unsigned long long* work, void* stream) {
+  if (!scene) {
+    set_error("xyz_closest_hit_work: NULL scene");
+    return XYZ_E_INVALID;
+  }
+  if (!work || (n > 0 && (!org || !dir || !prim))) {
+    set_error("xyz_closest_hit_work: NULL ray, prim or work buffer");
+    return XYZ_E_INVALID;
+  }
+  return launch_closest_hit(scene->dev, n, org, dir, prim, t, nullptr, nullptr, nullptr, nullptr,
+                            work, static_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"
