@@ -100,3 +100,25 @@ def test_thin_lens_render_bit_exact(oracle_mod, capi_mod, device):
     capi_mod.render_batch([g], [cam], films, [8], 8, rr_start=3, seeds=[5],
                           tuning=capi_mod.BatchTuning(64, 2, 128, 0))
     assert np.array_equal(films[0].numpy().view(np.uint32), ref.view(np.uint32))
+
+
+@pytest.mark.parametrize("fov", [20.0, 40.0, 75.0])
+def test_pinhole_field_of_view(oracle_mod, fov):
+    # xyz/core/thinlenscamera.cpp:98-124 with V = 0.0125 / tan(fov / 2): the
+    # centre ray is -back, the top edge is fov/2 above it, the right edge is
+    # atan(aspect tan(fov/2)) to its right
+    eye = np.array([0.3, 1.2, -2.0])
+    at = np.array([0.1, 0.9, 0.7])
+    W, H = 64, 48
+    cam = [*eye, *at, 0, 1, 0, fov]
+    back, right, up = _basis(eye, at)
+    o, d = oracle_mod.primary_ray(cam, W, H, 0.5, 0.5)
+    assert np.array_equal(o, np.float32(eye))
+    assert np.allclose(d, -back, atol=2e-7)
+    _, top = oracle_mod.primary_ray(cam, W, H, 0.5, 1.0)
+    _, rgt = oracle_mod.primary_ray(cam, W, H, 1.0, 0.5)
+    half = math.radians(fov) / 2
+    assert abs(math.acos(float(np.clip(top @ -back, -1, 1))) - half) < 2e-6
+    assert float(top @ up) > 0
+    assert abs(math.acos(float(np.clip(rgt @ -back, -1, 1))) - math.atan(W / H * math.tan(half))) < 2e-6
+    assert float(rgt @ right) > 0
