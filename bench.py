@@ -353,7 +353,8 @@ def bench_batch(args, hip, capi, scenes):
                batched_msamples_per_s=round(samples / (bat_ms * 1e-3) / 1e6, 1),
                sequential_msamples_per_s=round(samples / (seq_ms * 1e-3) / 1e6, 1),
                samples=int(samples), films_bit_identical=bool(same),
-               tuning=dict(tile_pixels=best.tile_pixels, blocks_per_cu=best.blocks_per_cu,
+               tuning=dict(engine=["persistent megakernel", "shared wavefront"][best.engine],
+                           tile_pixels=best.tile_pixels, blocks_per_cu=best.blocks_per_cu,
                            block_threads=best.block_threads, use_graph=best.use_graph),
                autotune_s=round(tune_s, 1))
     log(f"[bench] batched: {res}")

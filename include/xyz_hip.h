@@ -198,14 +198,23 @@ int xyz_render_backward(xyz_scene *scene, const xyz_camera *camera,
                         float *d_emission, void *stream);
 
 /* ---- batched rendering (no counterpart in the reference) ---------------------
- * Renders n independent (scene, camera, params, film) jobs in one persistent
- * launch over a global work queue of pixel tiles.  tuning may be NULL (defaults)
- * or come from xyz_batch_autotune. */
+ * Renders n independent (scene, camera, params, film) jobs in one call, with
+ * one of two engines:
+ *   engine 0: one persistent launch over a global work queue of pixel tiles
+ *             (tile_pixels, blocks_per_cu, block_threads; use_graph captures
+ *             the launch in a hipGraph that later calls replay);
+ *   engine 1: the wavefront kernels of xyz_render over all jobs at once: the
+ *             jobs' samples share waves of up to 2^25 paths, so that small jobs
+ *             do not pay per-bounce costs on their own (the other fields are
+ *             unused).
+ * Films are bit-identical to xyz_render's with either engine.  tuning may be
+ * NULL (engine 1) or come from xyz_batch_autotune. */
 typedef struct {
-  uint32_t tile_pixels;    /* pixels per work item */
-  uint32_t blocks_per_cu;  /* persistent workgroups per compute unit */
-  uint32_t block_threads;  /* threads per workgroup */
-  uint32_t use_graph;      /* capture the launch in a hipGraph and replay it */
+  uint32_t tile_pixels;    /* engine 0: pixels per work item */
+  uint32_t blocks_per_cu;  /* engine 0: persistent workgroups per compute unit */
+  uint32_t block_threads;  /* engine 0: threads per workgroup */
+  uint32_t use_graph;      /* engine 0: capture the launch in a hipGraph and replay it */
+  uint32_t engine;         /* 0: persistent megakernel, 1: shared wavefront */
 } xyz_batch_tuning;
 
 int xyz_render_batch(uint32_t n, xyz_scene *const *scenes, const xyz_camera *cameras,

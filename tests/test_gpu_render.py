@@ -142,9 +142,9 @@ def test_batch_equals_sequential(oracle_mod, capi_mod, device):
         f = DeviceArray.zeros((H, W, 3), np.float32)
         sc.render(cam, f, spp, 8, rr_start=3, seed=11)
         seq.append(f.numpy())
-    for use_graph in (0, 1):
+    for engine, use_graph in ((0, 0), (0, 1), (1, 0)):
         films = [DeviceArray.zeros((j[2], j[1], 3), np.float32) for j in jobs]
-        t = capi_mod.BatchTuning(64, 2, 128, use_graph)
+        t = capi_mod.BatchTuning(64, 2, 128, use_graph, engine)
         capi_mod.render_batch(scenes, cams, films, [j[3] for j in jobs], 8, rr_start=3,
                               seeds=[11, 11, 11], tuning=t)
         for f, s in zip(films, seq):

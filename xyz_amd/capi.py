@@ -76,7 +76,8 @@ class RenderStats(ctypes.Structure):
 
 class BatchTuning(ctypes.Structure):
     _fields_ = [("tile_pixels", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32),
-                ("block_threads", ctypes.c_uint32), ("use_graph", ctypes.c_uint32)]
+                ("block_threads", ctypes.c_uint32), ("use_graph", ctypes.c_uint32),
+                ("engine", ctypes.c_uint32)]
 
 
 # Every symbol declared in include/xyz_hip.h.

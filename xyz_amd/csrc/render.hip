@@ -43,13 +43,43 @@ struct alignas(256) LineU32 {
   uint32_t pad[63];
 };
 
-constexpr uint32_t kShadowAppend = 8;  // append[0..7]: octant queues, append[8]: shadow requests
+constexpr uint32_t kMaxWaveJobs = 64;  // jobs one wave may hold
+constexpr uint32_t kSegPad = 256;      // queue segments start at multiples of this position
 
+// One render job -- scene, camera, integrator constants, film -- on the device.
+// Its work items are the global items [base, base + wh * spp); item base + q
+// is sample sample0 + q / wh of pixel q % wh.
+struct RJob {
+  DevScene s;
+  DevCamera cam;
+  PTConst c;
+  float* film;
+  unsigned long long base;
+  uint32_t wh, sample0;
+};
+
+// The jobs of one wave: local job l is global job jlo + l and owns the wave's
+// path slots [start[l], start[l + 1]); slot i holds global item q0 + i.
+struct WaveJobs {
+  unsigned long long q0;
+  uint32_t jlo, nj;
+  uint32_t start[kMaxWaveJobs + 1];
+};
+
+// Queue segments of a wave.  Extension queues: segment g = 8 l + o holds the
+// active paths of local job l whose direction lies in octant o, at entries
+// [8 start[l] + o m_l, ...) of a queue array (m_l = start[l + 1] - start[l]).
+// Shadow requests of job l sit at entries [start[l], ...) of the request
+// arrays.  Persistent kernels address a queue set through virtual positions:
+// the segments end to end, each padded to a multiple of kSegPad, so that an
+// aligned batch of positions always lies in one segment, i.e. one job.
 struct WaveCounters {
-  LineU32 append[9];
-  LineU32 work_extend;  // persistent extend kernel's work counter
-  LineU32 work_shadow;  // persistent shadow kernel's work counter
-  uint32_t cur[8];
+  LineU32 append[kMaxWaveJobs * 9];     // [9 l + o]: o < 8 next octant queue, o = 8 shadow requests
+  LineU32 work_extend;                  // persistent extend kernel's work counter
+  LineU32 work_shadow;                  // persistent shadow kernel's work counter
+  uint32_t cur[kMaxWaveJobs * 8];       // current extension queue sizes
+  uint32_t vext[kMaxWaveJobs * 8 + 1];  // padded prefix of cur
+  uint32_t n_active;                    // sum of cur
   unsigned long long queries;
 };
 
@@ -189,91 +219,227 @@ __device__ __forceinline__ void store_state(const WaveBufs& wb, uint32_t slot, c
   wb.col[slot] = make_float4(ps.color.x, ps.color.y, ps.color.z, 0.0f);
 }
 
-// Total active paths and the queue slot of the j-th one (8 octant regions).
-__device__ __forceinline__ uint32_t queue_total(const uint32_t* cnt) {
-  uint32_t t = 0;
-#pragma unroll
-  for (int o = 0; o < 8; ++o) t += cnt[o];
-  return t;
+__device__ __forceinline__ uint32_t pad_seg(uint32_t c) { return (c + kSegPad - 1u) & ~(kSegPad - 1u); }
+
+// Queue entry of position r of extension segment g.
+__device__ __forceinline__ uint32_t ext_entry(const WaveJobs& wj, uint32_t g, uint32_t r) {
+  uint32_t const l = g >> 3, o = g & 7u;
+  uint32_t const s0 = wj.start[l];
+  return 8u * s0 + o * (wj.start[l + 1] - s0) + r;
 }
-__device__ __forceinline__ uint32_t queue_slot(const uint32_t* cnt, const uint32_t* q, uint32_t cap,
-                                               uint32_t j) {
-  uint32_t base = 0;
-#pragma unroll
-  for (int o = 0; o < 8; ++o) {
-    uint32_t const c = cnt[o];
-    if (j < base + c) return q[size_t(o) * cap + (j - base)];
-    base += c;
+
+// The segment holding virtual position v: the last g with vst[g] <= v, for
+// non-decreasing starts vst[0..nseg] with vst[0] = 0 <= v < vst[nseg].
+__device__ __forceinline__ uint32_t seg_of(const uint32_t* vst, uint32_t nseg, uint32_t v) {
+  uint32_t lo = 0, hi = nseg;
+  while (hi - lo > 1u) {
+    uint32_t const mid = (lo + hi) >> 1;
+    if (vst[mid] <= v) lo = mid; else hi = mid;
   }
-  return 0;
+  return lo;
+}
+
+// Local job of wave slot i.
+__device__ __forceinline__ uint32_t job_of_slot(const WaveJobs& wj, uint32_t i) {
+  uint32_t lo = 0, hi = wj.nj;
+  while (hi - lo > 1u) {
+    uint32_t const mid = (lo + hi) >> 1;
+    if (wj.start[mid] <= i) lo = mid; else hi = mid;
+  }
+  return lo;
 }
 
 // Rays a wave holds at once: 64 while there is work for every wave of the
 // grid; when a late bounce leaves few (and expensive) rays, they are spread
-// over all waves instead of being packed 64 to a wave.
+// over all waves instead of being packed 64 to a wave.  A power of two, so
+// that an aligned batch of this size never crosses a queue segment.
 __device__ __forceinline__ uint32_t wave_ray_limit(uint32_t total) {
   uint32_t const waves = gridDim.x * (blockDim.x / 64u);
-  uint32_t const per = (total + waves - 1) / waves;
-  return per < 1u ? 1u : (per > 64u ? 64u : per);
+  uint32_t const per = (total + waves - 1u) / waves;
+  uint32_t lim = 1u;
+  while (lim < per && lim < 64u) lim <<= 1;
+  return lim;
+}
+
+// Per-wave source of queue positions for the persistent trace kernels.
+// Batches of `lim` consecutive virtual positions come from a global counter,
+// one atomic per batch; padding positions are skipped.  A wave traces the
+// rays of one job at a time: a batch of another job is handed out only once
+// all of the wave's lanes are idle.  All lanes call take() together.
+struct SegFeeder {
+  uint32_t base, count, next, seg;
+  bool done;
+  __device__ __forceinline__ void init() {
+    base = count = next = seg = 0;
+    done = false;
+  }
+  // Lanes with `need` receive the entry r of segment *lane_seg (returned), or
+  // UINT32_MAX.  cnt[g]: real size of segment g; segs_per_job segments per
+  // job; *job: the wave's job, *switched: set when it changed.
+  __device__ __forceinline__ uint32_t take(bool need, uint32_t* counter, uint32_t vtotal,
+                                           uint32_t lim, const uint32_t* vst, uint32_t nseg,
+                                           const uint32_t* cnt, uint32_t segs_per_job,
+                                           uint32_t* job, bool* switched, uint32_t* lane_seg) {
+    uint32_t const lane = __lane_id();
+    unsigned long long const below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint32_t busy = uint32_t(__popcll(__ballot(!need)));
+    uint32_t room = lim > busy ? lim - busy : 0u;
+    uint32_t got = 0xFFFFFFFFu;
+    for (;;) {
+      unsigned long long const m = __ballot(need);
+      if (m == 0 || room == 0 || done) break;
+      if (next == count) {
+        uint32_t b = 0;
+        if (lane == 0) b = atomicAdd(counter, lim);
+        b = __builtin_amdgcn_readfirstlane(__shfl(b, 0));
+        if (b >= vtotal) {
+          done = true;
+          break;
+        }
+        uint32_t const g = seg_of(vst, nseg, b);
+        uint32_t const r0 = b - vst[g];
+        uint32_t const c = cnt[g];
+        base = r0;
+        seg = g;
+        next = 0;
+        count = (r0 < c) ? ((c - r0 < lim) ? c - r0 : lim) : 0u;
+        continue;
+      }
+      uint32_t const j = seg / segs_per_job;
+      if (j != *job) {
+        if (busy > 0) break;  // the current job's rays first
+        *job = j;
+        *switched = true;
+      }
+      uint32_t const avail = count - next;
+      uint32_t const rank = uint32_t(__popcll(m & below));
+      uint32_t take_n = uint32_t(__popcll(m));
+      take_n = take_n < avail ? take_n : avail;
+      take_n = take_n < room ? take_n : room;
+      if (need && rank < take_n) {
+        got = base + next + rank;
+        *lane_seg = seg;
+        need = false;
+      }
+      next += take_n;
+      room -= take_n;
+      busy += take_n;
+    }
+    return got;
+  }
+};
+
+// Exclusive prefix sum over a 512-thread workgroup (sc: 512 LDS words, which
+// hold the exclusive prefix on return); returns the total.
+__device__ __forceinline__ uint32_t block_scan512(uint32_t v, uint32_t* sc) {
+  uint32_t const t = threadIdx.x;
+  sc[t] = v;
+  __syncthreads();
+  for (uint32_t off = 1; off < 512u; off <<= 1) {
+    uint32_t const a = (t >= off) ? sc[t - off] : 0u;
+    __syncthreads();
+    sc[t] += a;
+    __syncthreads();
+  }
+  uint32_t const incl = sc[t];
+  uint32_t const total = sc[511];
+  __syncthreads();
+  sc[t] = incl - v;
+  __syncthreads();
+  return total;
+}
+
+// Device copies of the job and wave tables, written through kernel arguments
+// (stream-ordered; no host buffer has to outlive a copy).
+__global__ void k_put_job(RJob j, RJob* dst) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *dst = j;
+}
+__global__ void k_put_wave(WaveJobs w, WaveJobs* dst) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *dst = w;
+}
+
+// Queue counters of a fresh wave: every path of job l in its octant-0 queue.
+__global__ void __launch_bounds__(512) k_init_wave(const WaveJobs* __restrict__ wjp,
+                                                   WaveCounters* ctr) {
+  __shared__ uint32_t sc[512];
+  WaveJobs const& wj = *wjp;
+  uint32_t const k = threadIdx.x, nseg = wj.nj * 8u;
+  uint32_t c = 0;
+  if (k < nseg) {
+    uint32_t const l = k >> 3;
+    c = ((k & 7u) == 0u) ? wj.start[l + 1] - wj.start[l] : 0u;
+    ctr->cur[k] = c;
+  }
+  for (uint32_t a = k; a < wj.nj * 9u; a += 512u) ctr->append[a].v = 0u;
+  uint32_t const tot = block_scan512(k < nseg ? pad_seg(c) : 0u, sc);
+  if (k < nseg) ctr->vext[k] = sc[k];
+  if (k == 0) {
+    ctr->vext[nseg] = tot;
+    ctr->n_active = wj.start[wj.nj];
+    ctr->work_extend.v = 0u;
+    ctr->work_shadow.v = 0u;
+  }
 }
 
 __global__ void __launch_bounds__(kBlock)
-k_camera(DevScene s, DevCamera cam, PTConst c, WaveBufs wb, uint64_t q0, uint32_t n, uint32_t wh,
-         uint32_t sample0) {
+k_camera(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBufs wb, uint32_t n) {
   uint32_t const i = blockIdx.x * kBlock + threadIdx.x;
-  if (i == 0) {
-    wb.ctr->cur[0] = n;
-    for (int o = 1; o < 8; ++o) wb.ctr->cur[o] = 0;
-    for (int o = 0; o < 9; ++o) wb.ctr->append[o].v = 0;
-    wb.ctr->work_extend.v = 0;
-    wb.ctr->work_shadow.v = 0;
-  }
   if (i >= n) return;
-  uint64_t const q = q0 + i;
-  uint32_t const pixel = uint32_t(q % wh);
-  uint32_t const sample = sample0 + uint32_t(q / wh);
-  float4 const u = draw4(c, pixel, sample, 0);
+  WaveJobs const& wj = *wjp;
+  uint32_t const l = job_of_slot(wj, i);
+  RJob const& J = jobs[wj.jlo + l];
+  unsigned long long const item = wj.q0 + i - J.base;
+  uint32_t const pixel = uint32_t(item % J.wh);
+  uint32_t const sample = J.sample0 + uint32_t(item / J.wh);
+  float4 const u = draw4(J.c, pixel, sample, 0);
   float3 o, d;
-  primary_ray(cam, c, pixel, sample, u.x, u.y, &o, &d);
+  primary_ray(J.cam, J.c, pixel, sample, u.x, u.y, &o, &d);
   wb.pos[i] = make_float4(o.x, o.y, o.z, 0.0f);
   wb.dir[i] = make_float4(d.x, d.y, d.z, __uint_as_float(pack_k(1u, true)));
   wb.pw[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
   wb.col[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  wb.qa[i] = i;
+  wb.qa[8u * wj.start[l] + (i - wj.start[l])] = i;
 }
 
-// Persistent: lanes take active paths from the queues in batches (WaveFeeder)
-// and trace them one step at a time, so a wave never waits for its slowest ray.
+// Persistent: lanes take active paths from the queues (SegFeeder) and trace
+// them one step at a time, so a wave never waits for its slowest ray.
 __global__ void __launch_bounds__(kBlock)
-k_extend(DevScene s, WaveBufs wb, const uint32_t* __restrict__ qcur) {
+k_extend(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBufs wb,
+         const uint32_t* __restrict__ qcur) {
   extern __shared__ uint32_t lds_stack[];
-  __shared__ uint32_t cnt[8];
-  if (threadIdx.x < 8) cnt[threadIdx.x] = wb.ctr->cur[threadIdx.x];
-  __syncthreads();
+  __shared__ uint32_t vst[kMaxWaveJobs * 8 + 1];
+  __shared__ uint32_t cnt[kMaxWaveJobs * 8];
   __shared__ unsigned long long keys[kBlock];
-  uint32_t const total = queue_total(cnt);
-  uint32_t const lim = wave_ray_limit(total);
+  WaveJobs const& wj = *wjp;
+  uint32_t const nseg = wj.nj * 8u;
+  for (uint32_t k = threadIdx.x; k <= nseg; k += kBlock) {
+    vst[k] = wb.ctr->vext[k];
+    if (k < nseg) cnt[k] = wb.ctr->cur[k];
+  }
+  __syncthreads();
+  uint32_t const vtotal = vst[nseg];
+  uint32_t const lim = wave_ray_limit(wb.ctr->n_active);
   uint32_t* const stk = lds_stack + threadIdx.x;
-  WaveFeeder feed;
+  SegFeeder feed;
   feed.init();
+  uint32_t job = 0;
+  const DevScene* sp = &jobs[wj.jlo].s;  // the wave's job (uniform)
   TraceState st;
   uint32_t slot = 0xFFFFFFFFu;
-  bool exhausted = false;
   for (;;) {
-    bool const need = (slot == 0xFFFFFFFFu) && !exhausted;
-    uint32_t const busy = uint32_t(__popcll(__ballot(slot != 0xFFFFFFFFu)));
-    uint32_t const j = feed.refill(need, &wb.ctr->work_extend.v, total, &exhausted, lim - min(lim, busy), lim);
-    if (j != 0xFFFFFFFFu) {
-      slot = queue_slot(cnt, qcur, wb.cap, j);
+    bool switched = false;
+    uint32_t lseg = 0;
+    uint32_t const r = feed.take(slot == 0xFFFFFFFFu, &wb.ctr->work_extend.v, vtotal, lim, vst,
+                                 nseg, cnt, 8u, &job, &switched, &lseg);
+    if (switched) sp = &jobs[wj.jlo + __builtin_amdgcn_readfirstlane(job)].s;
+    DevScene const& s = *sp;
+    if (r != 0xFFFFFFFFu) {
+      slot = qcur[ext_entry(wj, lseg, r)];
       trace_init(st, ld3(wb.pos[slot]), ld3(wb.dir[slot]));
     }
     if (__ballot(slot != 0xFFFFFFFFu) == 0) break;
     bool const act = slot != 0xFFFFFFFFu;
-#if XYZ_COOP
     bool const fin = coop_step<false>(s, st, act, stk, kBlock, keys + (threadIdx.x & ~63u));
-#else
-    bool const fin = act && trace_step<false>(s, st, stk, kBlock);
-#endif
     if (act && fin) {
       HitRec const h = trace_result(s, st);
       wb.hit[slot] = make_float4(__int_as_float(h.prim < 0 ? -1 : (h.prim | (h.back ? 0x80000000 : 0))),
@@ -319,11 +485,15 @@ __device__ __forceinline__ uint32_t lds_wave_append(bool pred, uint32_t key, uin
   return off;
 }
 
+// Shading of the active paths.  A workgroup takes 256 consecutive virtual
+// positions at a time, which lie in one queue segment (one job), so the
+// workgroup's appends go to that job's queues: counted in LDS, then one global
+// atomic per queue.
 template <bool kReplay>
 __global__ void __launch_bounds__(kBlock)
-k_shade(DevScene s, PTConst c, WaveBufs wb, const uint32_t* __restrict__ qcur, uint32_t* qnext,
-        uint64_t q0, uint32_t wh, uint32_t sample0, GradArgs ga) {
-  __shared__ uint32_t cnt[8];
+k_shade(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBufs wb,
+        const uint32_t* __restrict__ qcur, uint32_t* qnext, GradArgs ga) {
+  __shared__ uint32_t vst[kMaxWaveJobs * 8 + 1];
   __shared__ uint32_t bcnt[9], bbase[9];  // this workgroup's appends: 8 octant queues + shadow
   GradLds* glds = nullptr;
   if constexpr (kReplay) {
@@ -331,41 +501,45 @@ k_shade(DevScene s, PTConst c, WaveBufs wb, const uint32_t* __restrict__ qcur, u
     glds = &glds_mem;
     grad_lds_init(glds_mem);
   }
-  if (threadIdx.x < 8) cnt[threadIdx.x] = wb.ctr->cur[threadIdx.x];
+  WaveJobs const& wj = *wjp;
+  uint32_t const nseg = wj.nj * 8u;
+  for (uint32_t k = threadIdx.x; k <= nseg; k += kBlock) vst[k] = wb.ctr->vext[k];
   __syncthreads();
-  uint32_t const total = queue_total(cnt);
-  uint32_t const stride = gridDim.x * kBlock;
-  // every lane runs the same number of iterations so that ballots are convergent
-  for (uint32_t base = blockIdx.x * kBlock; base < total; base += stride) {
-    uint32_t const j = base + threadIdx.x;
-    bool const active = j < total;
+  uint32_t const vtotal = vst[nseg];
+  // every thread of a workgroup runs the same iterations (barriers below)
+  for (uint32_t vb = blockIdx.x * kBlock; vb < vtotal; vb += gridDim.x * kBlock) {
+    uint32_t const g = __builtin_amdgcn_readfirstlane(seg_of(vst, nseg, vb));
+    uint32_t const l = g >> 3;
+    uint32_t const r = vb - vst[g] + threadIdx.x;
+    bool const active = r < wb.ctr->cur[g];
+    RJob const& J = jobs[wj.jlo + l];
     bool cont = false, has_sr = false;
     uint32_t slot = 0, oct = 0;
     ShadowReq sr;
     if (active) {
-      slot = queue_slot(cnt, qcur, wb.cap, j);
-      uint64_t const q = q0 + slot;
-      uint32_t const pixel = uint32_t(q % wh);
-      uint32_t const sample = sample0 + uint32_t(q / wh);
+      slot = qcur[ext_entry(wj, g, r)];
+      unsigned long long const item = wj.q0 + slot - J.base;
+      uint32_t const pixel = uint32_t(item % J.wh);
+      uint32_t const sample = J.sample0 + uint32_t(item / J.wh);
       PathState ps;
       load_state(wb, slot, ps);
       HitRec const h = unpack_hit(wb.hit[slot]);
-      float4 const u0 = draw4(c, pixel, sample, 0);
-      LightVertex const lv = sample_light(s, u0.z, u0.w);
+      float4 const u0 = draw4(J.c, pixel, sample, 0);
+      LightVertex const lv = sample_light(J.s, u0.z, u0.w);
       if (kReplay) {
-        GradReplay g;
-        g.tex = s.tex;
-        g.bsdfs = s.bsdfs;
-        g.w = f3(ga.d_film[3 * size_t(pixel)], ga.d_film[3 * size_t(pixel) + 1],
-                 ga.d_film[3 * size_t(pixel) + 2]);
-        g.ctot = ld3(wb.ctot[slot]);
-        g.d_tex = ga.d_tex;
-        g.d_emit = ga.d_emit;
-        g.lds = glds;
-        cont = shade_vertex(s, c, ps, h, pixel, sample, lv, &sr, &has_sr, g);
+        GradReplay gr;
+        gr.tex = J.s.tex;
+        gr.bsdfs = J.s.bsdfs;
+        gr.w = f3(ga.d_film[3 * size_t(pixel)], ga.d_film[3 * size_t(pixel) + 1],
+                  ga.d_film[3 * size_t(pixel) + 2]);
+        gr.ctot = ld3(wb.ctot[slot]);
+        gr.d_tex = ga.d_tex;
+        gr.d_emit = ga.d_emit;
+        gr.lds = glds;
+        cont = shade_vertex(J.s, J.c, ps, h, pixel, sample, lv, &sr, &has_sr, gr);
       } else {
-        NoGrad g;
-        cont = shade_vertex(s, c, ps, h, pixel, sample, lv, &sr, &has_sr, g);
+        NoGrad ng;
+        cont = shade_vertex(J.s, J.c, ps, h, pixel, sample, lv, &sr, &has_sr, ng);
       }
       store_state(wb, slot, ps);
       oct = (ps.dir.x < 0.0f ? 1u : 0u) | (ps.dir.y < 0.0f ? 2u : 0u) | (ps.dir.z < 0.0f ? 4u : 0u);
@@ -374,30 +548,32 @@ k_shade(DevScene s, PTConst c, WaveBufs wb, const uint32_t* __restrict__ qcur, u
     if (threadIdx.x < 9) bcnt[threadIdx.x] = 0;
     __syncthreads();
     uint32_t const qoff = lds_wave_append(active && cont, oct, 8, 0, bcnt);
-    uint32_t const soff = lds_wave_append(active && has_sr, 0, 1, kShadowAppend, bcnt);
+    uint32_t const soff = lds_wave_append(active && has_sr, 0, 1, 8, bcnt);
     __syncthreads();
     if (threadIdx.x < 9) {
       uint32_t const c = bcnt[threadIdx.x];
-      bbase[threadIdx.x] = c ? atomicAdd(&wb.ctr->append[threadIdx.x].v, c) : 0u;
+      bbase[threadIdx.x] = c ? atomicAdd(&wb.ctr->append[9u * l + threadIdx.x].v, c) : 0u;
     }
     __syncthreads();
-    if (active && cont) qnext[size_t(oct) * wb.cap + bbase[oct] + qoff] = slot;
-    uint32_t const r = bbase[kShadowAppend] + soff;
+    if (active && cont) qnext[ext_entry(wj, 8u * l + oct, bbase[oct] + qoff)] = slot;
+    uint32_t const rq = wj.start[l] + bbase[8] + soff;
     if (active && has_sr) {
-      wb.so[r] = make_float4(sr.o.x, sr.o.y, sr.o.z, sr.nk2);
-      wb.sw[r] = make_float4(sr.w.x, sr.w.y, sr.w.z, sr.nk3);
-      wb.spe[r] = make_float4(sr.pe.x, sr.pe.y, sr.pe.z, __uint_as_float(slot));
-      wb.smeta[r] = make_uint4(uint32_t(sr.expected), sr.type, sr.light_bsdf, 0u);
-      wb.smet[r] = make_float4(sr.fsn, sr.fres, sr.mx, 0.0f);
+      wb.so[rq] = make_float4(sr.o.x, sr.o.y, sr.o.z, sr.nk2);
+      wb.sw[rq] = make_float4(sr.w.x, sr.w.y, sr.w.z, sr.nk3);
+      wb.spe[rq] = make_float4(sr.pe.x, sr.pe.y, sr.pe.z, __uint_as_float(slot));
+      wb.smeta[rq] = make_uint4(uint32_t(sr.expected), sr.type, sr.light_bsdf, 0u);
+      wb.smet[rq] = make_float4(sr.fsn, sr.fres, sr.mx, 0.0f);
     }
   }
-  if constexpr (kReplay) grad_lds_flush(*glds, ga.d_tex, ga.d_emit, 3 * s.n_bsdfs);
+  if constexpr (kReplay) grad_lds_flush(*glds, ga.d_tex, ga.d_emit, 3 * jobs[wj.jlo].s.n_bsdfs);
 }
 
 template <bool kReplay>
 __global__ void __launch_bounds__(kBlock)
-k_shadow(DevScene s, WaveBufs wb, uint64_t q0, uint32_t wh, GradArgs ga) {
+k_shadow(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBufs wb, GradArgs ga) {
   extern __shared__ uint32_t lds_stack[];
+  __shared__ uint32_t vst[kMaxWaveJobs + 1];
+  __shared__ uint32_t cnt[kMaxWaveJobs];
   GradLds* glds = nullptr;
   if constexpr (kReplay) {
     __shared__ GradLds glds_mem;
@@ -405,33 +581,49 @@ k_shadow(DevScene s, WaveBufs wb, uint64_t q0, uint32_t wh, GradArgs ga) {
     grad_lds_init(glds_mem);
   }
   __shared__ unsigned long long keys[kBlock];
-  uint32_t const total = wb.ctr->append[kShadowAppend].v;
-  uint32_t const lim = wave_ray_limit(total);
-  uint32_t* const stk = lds_stack + threadIdx.x;
-  WaveFeeder feed;
-  feed.init();
-  TraceState st;
-  uint32_t r = 0xFFFFFFFFu;
-  bool exhausted = false;
-  for (;;) {
-    bool const need = (r == 0xFFFFFFFFu) && !exhausted;
-    uint32_t const busy = uint32_t(__popcll(__ballot(r != 0xFFFFFFFFu)));
-    uint32_t const got = feed.refill(need, &wb.ctr->work_shadow.v, total, &exhausted, lim - min(lim, busy), lim);
-    if (got != 0xFFFFFFFFu) {
-      r = got;
-      trace_init(st, ld3(wb.so[r]), ld3(wb.sw[r]));
+  WaveJobs const& wj = *wjp;
+  if (threadIdx.x < 64) {  // padded prefix of the jobs' request counts
+    uint32_t const k = threadIdx.x;
+    uint32_t const c = (k < wj.nj) ? wb.ctr->append[9u * k + 8u].v : 0u;
+    uint32_t excl, total;
+    wave_prefix(pad_seg(c), &excl, &total);
+    if (k < wj.nj) {
+      cnt[k] = c;
+      vst[k] = excl;
     }
-    if (__ballot(r != 0xFFFFFFFFu) == 0) break;
-    bool const act = r != 0xFFFFFFFFu;
-#if XYZ_COOP
+    if (k == 0) vst[wj.nj] = total;
+  }
+  __syncthreads();
+  uint32_t const nseg = wj.nj;
+  uint32_t const vtotal = vst[nseg];
+  uint32_t n_req = 0;
+  for (uint32_t k = 0; k < nseg; ++k) n_req += cnt[k];
+  uint32_t const lim = wave_ray_limit(n_req);
+  uint32_t* const stk = lds_stack + threadIdx.x;
+  SegFeeder feed;
+  feed.init();
+  uint32_t job = 0;
+  const DevScene* sp = &jobs[wj.jlo].s;  // the wave's job (uniform)
+  TraceState st;
+  uint32_t rq = 0xFFFFFFFFu;
+  for (;;) {
+    bool switched = false;
+    uint32_t lseg = 0;
+    uint32_t const r = feed.take(rq == 0xFFFFFFFFu, &wb.ctr->work_shadow.v, vtotal, lim, vst, nseg,
+                                 cnt, 1u, &job, &switched, &lseg);
+    if (switched) sp = &jobs[wj.jlo + __builtin_amdgcn_readfirstlane(job)].s;
+    DevScene const& s = *sp;
+    if (r != 0xFFFFFFFFu) {
+      rq = wj.start[lseg] + r;
+      trace_init(st, ld3(wb.so[rq]), ld3(wb.sw[rq]));
+    }
+    if (__ballot(rq != 0xFFFFFFFFu) == 0) break;
+    bool const act = rq != 0xFFFFFFFFu;
     bool const fin = coop_step<false>(s, st, act, stk, kBlock, keys + (threadIdx.x & ~63u));
-#else
-    bool const fin = act && trace_step<false>(s, st, stk, kBlock);
-#endif
     if (!act || !fin) continue;
     HitRec const h = trace_result(s, st);
-    uint32_t const rr = r;
-    r = 0xFFFFFFFFu;
+    uint32_t const rr = rq;
+    rq = 0xFFFFFFFFu;
     uint4 const meta = wb.smeta[rr];
     if (h.prim != int32_t(meta.x)) continue;
     float4 const o = wb.so[rr], w = wb.sw[rr];
@@ -447,52 +639,88 @@ k_shadow(DevScene s, WaveBufs wb, uint64_t q0, uint32_t wh, GradArgs ga) {
     float3 const add = nee_contribution(sr, h.t, s.light_area);
     uint32_t const slot = __float_as_uint(pe.w);
     if constexpr (kReplay) {
-      uint64_t const q = q0 + slot;
-      uint32_t const pixel = uint32_t(q % wh);
-      GradReplay g;
-      g.tex = s.tex;
-      g.bsdfs = s.bsdfs;
-      g.w = f3(ga.d_film[3 * size_t(pixel)], ga.d_film[3 * size_t(pixel) + 1],
-               ga.d_film[3 * size_t(pixel) + 2]);
-      g.d_tex = ga.d_tex;
-      g.d_emit = ga.d_emit;
-      g.lds = glds;
-      g.emission(meta.z, add);
+      RJob const& J = jobs[wj.jlo + job];
+      unsigned long long const item = wj.q0 + slot - J.base;
+      uint32_t const pixel = uint32_t(item % J.wh);
+      GradReplay gr;
+      gr.tex = s.tex;
+      gr.bsdfs = s.bsdfs;
+      gr.w = f3(ga.d_film[3 * size_t(pixel)], ga.d_film[3 * size_t(pixel) + 1],
+                ga.d_film[3 * size_t(pixel) + 2]);
+      gr.d_tex = ga.d_tex;
+      gr.d_emit = ga.d_emit;
+      gr.lds = glds;
+      gr.emission(meta.z, add);
     }
     float4 const cc = wb.col[slot];
     wb.col[slot] = make_float4(cc.x + add.x, cc.y + add.y, cc.z + add.z, 0.0f);
   }
-  if constexpr (kReplay) grad_lds_flush(*glds, ga.d_tex, ga.d_emit, 3 * s.n_bsdfs);
+  if constexpr (kReplay) grad_lds_flush(*glds, ga.d_tex, ga.d_emit, 3 * jobs[wj.jlo].s.n_bsdfs);
 }
 
-__global__ void k_prep(WaveCounters* ctr) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  uint32_t t = ctr->append[kShadowAppend].v;
-  for (int o = 0; o < 8; ++o) {
-    t += ctr->cur[o];
-    ctr->cur[o] = ctr->append[o].v;
-    ctr->append[o].v = 0;
+// End of a bounce: the next queues become current; their padded prefix for
+// the next bounce's virtual positions; counted queries.
+__global__ void __launch_bounds__(512) k_prep(const WaveJobs* __restrict__ wjp, WaveCounters* ctr) {
+  __shared__ uint32_t sc[512];
+  __shared__ unsigned long long qsum;
+  __shared__ uint32_t nsum;
+  WaveJobs const& wj = *wjp;
+  uint32_t const k = threadIdx.x, nseg = wj.nj * 8u;
+  if (k == 0) {
+    qsum = 0;
+    nsum = 0;
   }
-  ctr->append[kShadowAppend].v = 0;
-  ctr->work_extend.v = 0;
-  ctr->work_shadow.v = 0;
-  ctr->queries += t;
+  __syncthreads();
+  uint32_t done = 0, nx = 0;
+  if (k < nseg) {
+    uint32_t const a = 9u * (k >> 3) + (k & 7u);
+    done = ctr->cur[k];
+    nx = ctr->append[a].v;
+    ctr->cur[k] = nx;
+    ctr->append[a].v = 0u;
+  }
+  if (k < wj.nj) {
+    done += ctr->append[9u * k + 8u].v;
+    ctr->append[9u * k + 8u].v = 0u;
+  }
+  if (done) atomicAdd(&qsum, static_cast<unsigned long long>(done));
+  if (nx) atomicAdd(&nsum, nx);
+  uint32_t const tot = block_scan512(k < nseg ? pad_seg(nx) : 0u, sc);
+  if (k < nseg) ctr->vext[k] = sc[k];
+  if (k == 0) {
+    ctr->vext[nseg] = tot;
+    ctr->n_active = nsum;
+    ctr->queries += qsum;
+    ctr->work_extend.v = 0u;
+    ctr->work_shadow.v = 0u;
+  }
 }
 
 __global__ void k_zero_counters(WaveCounters* ctr) {
   if (threadIdx.x == 0 && blockIdx.x == 0) ctr->queries = 0;
 }
 
-// film[p] += colour of each of p's samples in this wave, in sample order.
+// film[p] += colour of each of p's samples in this wave, in sample order;
+// blockIdx.y = local job.
 __global__ void __launch_bounds__(kBlock)
-k_accumulate(float* film, const float4* __restrict__ col, uint64_t q0, uint32_t n, uint32_t wh) {
+k_accumulate(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp,
+             const float4* __restrict__ col) {
+  WaveJobs const& wj = *wjp;
+  uint32_t const l = blockIdx.y;
+  if (l >= wj.nj) return;
+  RJob const& J = jobs[wj.jlo + l];
   uint32_t const p = blockIdx.x * kBlock + threadIdx.x;
+  uint32_t const wh = J.wh;
   if (p >= wh) return;
-  uint64_t q = q0 + (uint64_t(p) + wh - (q0 % wh)) % wh;
+  uint32_t const s0 = wj.start[l];
+  uint32_t const n = wj.start[l + 1] - s0;
+  unsigned long long const q0 = wj.q0 + s0 - J.base;  // job item of slot s0
+  unsigned long long q = q0 + (uint64_t(p) + wh - (q0 % wh)) % wh;
   if (q >= q0 + n) return;
+  float* const film = J.film;
   float fx = film[3 * size_t(p)], fy = film[3 * size_t(p) + 1], fz = film[3 * size_t(p) + 2];
   for (; q < q0 + n; q += wh) {
-    float4 const c = col[q - q0];
+    float4 const c = col[s0 + (q - q0)];
     fx = fx + c.x;
     fy = fy + c.y;
     fz = fz + c.z;
@@ -688,24 +916,171 @@ struct EventSet {
 
 // Runs the bounce loop of one wave.  kReplay: adjoint replay pass.
 template <bool kReplay>
-int run_bounces(const DevScene& s, const PTConst& c, WaveBufs wb, uint64_t q0, uint32_t n,
-                uint32_t wh, uint32_t sample0, uint32_t max_depth, GradArgs ga, hipStream_t st) {
+int run_bounces(const RJob* jobs, const WaveJobs* wj, WaveBufs wb, uint32_t n, uint32_t depth,
+                uint32_t stack_cap, GradArgs ga, hipStream_t st) {
   int const ncu = num_cu();
-  size_t const lds = size_t(s.stack_cap) * kBlock * sizeof(uint32_t);
+  size_t const lds = size_t(stack_cap) * kBlock * sizeof(uint32_t);
   uint32_t const trace_grid = std::max(1u, std::min((n + kBlock - 1) / kBlock, uint32_t(ncu) * 8));
   uint32_t const shade_grid = std::max(1u, std::min((n + kBlock - 1) / kBlock, uint32_t(ncu) * 8));
   uint32_t* qc = wb.qa;
   uint32_t* qn = wb.qb;
-  for (uint32_t b = 0; b < max_depth; ++b) {
-    hipLaunchKernelGGL(k_extend, dim3(trace_grid), dim3(kBlock), lds, st, s, wb, qc);
-    hipLaunchKernelGGL(k_shade<kReplay>, dim3(shade_grid), dim3(kBlock), 0, st, s, c, wb, qc, qn,
-                       q0, wh, sample0, ga);
-    hipLaunchKernelGGL(k_shadow<kReplay>, dim3(trace_grid), dim3(kBlock), lds, st, s, wb, q0, wh, ga);
-    hipLaunchKernelGGL(k_prep, dim3(1), dim3(64), 0, st, wb.ctr);
+  for (uint32_t b = 0; b < depth; ++b) {
+    hipLaunchKernelGGL(k_extend, dim3(trace_grid), dim3(kBlock), lds, st, jobs, wj, wb, qc);
+    hipLaunchKernelGGL(k_shade<kReplay>, dim3(shade_grid), dim3(kBlock), 0, st, jobs, wj, wb, qc, qn,
+                       ga);
+    hipLaunchKernelGGL(k_shadow<kReplay>, dim3(trace_grid), dim3(kBlock), lds, st, jobs, wj, wb, ga);
+    hipLaunchKernelGGL(k_prep, dim3(1), dim3(512), 0, st, wj, wb.ctr);
     std::swap(qc, qn);
   }
   hipError_t const e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "render launch");
+  return XYZ_OK;
+}
+
+// One job of a render call, before planning.
+struct JobSpec {
+  RJob j;
+  unsigned long long items;
+  uint32_t max_depth;
+};
+
+// Renders jobs with the wavefront kernels.  The jobs' work items are laid end
+// to end and cut into waves of up to kWaveCap path slots and kMaxWaveJobs jobs;
+// consecutive waves alternate between internal streams, so that the tail of
+// one wave (a few long paths) overlaps the full bounces of the next.  Each
+// job's film receives its waves in sample order (an event per wave).
+// backward: the adjoint of the single job (ga).  queries: total closest-hit
+// queries (may be NULL; then nothing waits on the device).
+int render_jobs(std::vector<JobSpec> js, hipStream_t st, bool backward, GradArgs ga,
+                unsigned long long* queries) {
+  std::vector<JobSpec> jobs;
+  for (JobSpec& x : js)
+    if (x.items > 0) jobs.push_back(x);
+  if (queries) *queries = 0;
+  if (jobs.empty()) return XYZ_OK;
+  unsigned long long total = 0;
+  for (JobSpec& x : jobs) {
+    x.j.base = total;
+    total += x.items;
+  }
+  uint32_t const cap = uint32_t(std::min<unsigned long long>(total, kWaveCap));
+  // wave plan
+  std::vector<WaveJobs> waves;
+  std::vector<uint32_t> wdepth, wstack, wmaxwh;
+  {
+    unsigned long long q = 0;
+    size_t j = 0;
+    while (q < total) {
+      WaveJobs w;
+      std::memset(&w, 0, sizeof(w));
+      w.q0 = q;
+      w.jlo = uint32_t(j);
+      uint32_t slots = 0, depth = 0, stack = 1, maxwh = 0;
+      while (slots < cap && j < jobs.size() && w.nj < kMaxWaveJobs) {
+        unsigned long long const end = jobs[j].j.base + jobs[j].items;
+        uint32_t const take = uint32_t(std::min<unsigned long long>(cap - slots, end - q));
+        w.start[w.nj++] = slots;
+        slots += take;
+        q += take;
+        depth = std::max(depth, jobs[j].max_depth);
+        stack = std::max(stack, jobs[j].j.s.stack_cap);
+        maxwh = std::max(maxwh, jobs[j].j.wh);
+        if (q == end) ++j; else break;
+      }
+      w.start[w.nj] = slots;
+      waves.push_back(w);
+      wdepth.push_back(depth);
+      wstack.push_back(stack);
+      wmaxwh.push_back(maxwh);
+    }
+  }
+  // device tables, written by kernels from their arguments (stream-ordered)
+  DevMem tables;
+  size_t const jb = align_up(jobs.size() * sizeof(RJob));
+  hipError_t e = pool_alloc(&tables.p, jb + waves.size() * sizeof(WaveJobs), st);
+  if (e != hipSuccess) return hip_fail(e, "pool_alloc(render tables)");
+  tables.st = st;
+  RJob* const jobs_dev = static_cast<RJob*>(tables.p);
+  WaveJobs* const waves_dev = reinterpret_cast<WaveJobs*>(static_cast<char*>(tables.p) + jb);
+  for (size_t i = 0; i < jobs.size(); ++i)
+    hipLaunchKernelGGL(k_put_job, dim3(1), dim3(1), 0, st, jobs[i].j, jobs_dev + i);
+  for (size_t k = 0; k < waves.size(); ++k)
+    hipLaunchKernelGGL(k_put_wave, dim3(1), dim3(1), 0, st, waves[k], waves_dev + k);
+  int const lanes = waves.size() > 1 ? kLanes : 1;
+  hipStream_t ls[kLanes];
+  for (int i = 0; i < kLanes; ++i) ls[i] = st;
+  int rc = XYZ_OK;
+  if (lanes > 1) {
+    rc = lane_streams(ls);
+    if (rc != XYZ_OK) return rc;
+  }
+  DevMem mem[kLanes];
+  WaveBufs wb[kLanes];
+  for (int i = 0; i < lanes; ++i) {
+    rc = alloc_wave(cap, backward, st, &mem[i], &wb[i]);
+    if (rc != XYZ_OK) return rc;
+    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, wb[i].ctr);
+  }
+  EventSet ev;
+  if (lanes > 1) {
+    rc = ev.create(2 * kLanes + 1);
+    if (rc != XYZ_OK) return rc;
+    hipEvent_t const fork = ev.e[2 * kLanes];
+    e = hipEventRecord(fork, st);
+    for (int i = 0; i < lanes && e == hipSuccess; ++i) e = hipStreamWaitEvent(ls[i], fork, 0);
+    if (e != hipSuccess) return hip_fail(e, "render fork");
+  }
+  GradArgs none{nullptr, nullptr, nullptr};
+  for (size_t k = 0; k < waves.size(); ++k) {
+    int const i = int(k % size_t(lanes));
+    hipStream_t const s = ls[i];
+    WaveJobs const* const wj = waves_dev + k;
+    uint32_t const n = waves[k].start[waves[k].nj];
+    uint32_t const grid = (n + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_init_wave, dim3(1), dim3(512), 0, s, wj, wb[i].ctr);
+    hipLaunchKernelGGL(k_camera, dim3(grid), dim3(kBlock), 0, s, jobs_dev, wj, wb[i], n);
+    rc = run_bounces<false>(jobs_dev, wj, wb[i], n, wdepth[k], wstack[k], none, s);
+    if (rc != XYZ_OK) return rc;
+    if (!backward) {
+      // wave k is added to the films after wave k - 1 (sample order per pixel)
+      if (lanes > 1 && k > 0) {
+        e = hipStreamWaitEvent(s, ev.e[(k - 1) % size_t(lanes)], 0);
+        if (e != hipSuccess) return hip_fail(e, "render wait");
+      }
+      hipLaunchKernelGGL(k_accumulate, dim3((wmaxwh[k] + kBlock - 1) / kBlock, waves[k].nj),
+                         dim3(kBlock), 0, s, jobs_dev, wj, wb[i].col);
+      if (lanes > 1) {
+        e = hipEventRecord(ev.e[i], s);
+        if (e != hipSuccess) return hip_fail(e, "render record");
+      }
+    } else {
+      // replay: same paths from their seeds, now with the primal colours known
+      hipLaunchKernelGGL(k_copy_col, dim3(grid), dim3(kBlock), 0, s, wb[i].ctot, wb[i].col, n);
+      hipLaunchKernelGGL(k_init_wave, dim3(1), dim3(512), 0, s, wj, wb[i].ctr);
+      hipLaunchKernelGGL(k_camera, dim3(grid), dim3(kBlock), 0, s, jobs_dev, wj, wb[i], n);
+      rc = run_bounces<true>(jobs_dev, wj, wb[i], n, wdepth[k], wstack[k], ga, s);
+      if (rc != XYZ_OK) return rc;
+    }
+  }
+  if (lanes > 1) {  // join: the caller's stream waits for every lane
+    for (int i = 0; i < lanes; ++i) {
+      hipEvent_t const done = ev.e[kLanes + i];
+      e = hipEventRecord(done, ls[i]);
+      if (e == hipSuccess) e = hipStreamWaitEvent(st, done, 0);
+      if (e != hipSuccess) return hip_fail(e, "render join");
+    }
+  }
+  if (queries) {
+    unsigned long long q[kLanes] = {};
+    e = hipSuccess;
+    for (int i = 0; i < lanes && e == hipSuccess; ++i)
+      e = hipMemcpyAsync(&q[i], &wb[i].ctr->queries, sizeof(q[i]), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "render stats");
+    for (int i = 0; i < lanes; ++i) *queries += q[i];
+  }
+  e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "render");
   return XYZ_OK;
 }
 
@@ -716,11 +1091,11 @@ int render_impl(xyz_scene* sc, const xyz_camera* cam, const xyz_render_params* p
     set_error("render: NULL scene");
     return XYZ_E_INVALID;
   }
-  DevCamera dc;
-  PTConst pc;
-  int rc = make_dev_camera(cam, &dc);
+  JobSpec js;
+  std::memset(&js, 0, sizeof(js));
+  int rc = make_dev_camera(cam, &js.j.cam);
   if (rc != XYZ_OK) return rc;
-  rc = make_pt_const(prm, &pc);
+  rc = make_pt_const(prm, &js.j.c);
   if (rc != XYZ_OK) return rc;
   if (!backward && !film) {
     set_error("render: NULL film");
@@ -735,110 +1110,33 @@ int render_impl(xyz_scene* sc, const xyz_camera* cam, const xyz_render_params* p
     set_error("render: image too large");
     return XYZ_E_UNSUPPORTED;
   }
-  uint64_t const total = wh * prm->spp;
   if (uint64_t(prm->sample_offset) + prm->spp >= (1ull << 32)) {
     set_error("render: sample index overflow");
     return XYZ_E_UNSUPPORTED;
   }
+  js.j.s = sc->dev;
+  js.j.film = film;
+  js.j.wh = uint32_t(wh);
+  js.j.sample0 = prm->sample_offset;
+  js.items = wh * prm->spp;
+  js.max_depth = prm->max_depth;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (stats) {
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
     (void)hipEventRecord(e0, st);
   }
-  if (total > 0) {
-    uint32_t const cap = uint32_t(std::min<uint64_t>(total, kWaveCap));
-    uint64_t const n_waves = (total + cap - 1) / cap;
-    // consecutive waves alternate between internal streams, so that the tail
-    // of one wave (a few long paths) overlaps the full bounces of the next;
-    // films still receive the waves in sample order (events below)
-    int const lanes = n_waves > 1 ? kLanes : 1;
-    hipStream_t ls[kLanes];
-    for (int i = 0; i < kLanes; ++i) ls[i] = st;
-    if (lanes > 1) {
-      rc = lane_streams(ls);
-      if (rc != XYZ_OK) return rc;
-    }
-    DevMem mem[kLanes];
-    WaveBufs wb[kLanes];
-    for (int i = 0; i < lanes; ++i) {
-      rc = alloc_wave(cap, backward, st, &mem[i], &wb[i]);
-      if (rc != XYZ_OK) return rc;
-      hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, wb[i].ctr);
-    }
-    EventSet ev;
-    if (lanes > 1) {
-      rc = ev.create(2 * kLanes + 1);
-      if (rc != XYZ_OK) return rc;
-      hipEvent_t const fork = ev.e[2 * kLanes];
-      hipError_t e = hipEventRecord(fork, st);
-      for (int i = 0; i < lanes && e == hipSuccess; ++i) e = hipStreamWaitEvent(ls[i], fork, 0);
-      if (e != hipSuccess) return hip_fail(e, "render fork");
-    }
-    GradArgs none{nullptr, nullptr, nullptr};
-    GradArgs ga{d_film, d_tex, d_emit};
-    for (uint64_t k = 0; k < n_waves; ++k) {
-      int const i = int(k % uint64_t(lanes));
-      hipStream_t const s = ls[i];
-      uint64_t const q0 = k * cap;
-      uint32_t const n = uint32_t(std::min<uint64_t>(cap, total - q0));
-      uint32_t const grid = (n + kBlock - 1) / kBlock;
-      hipLaunchKernelGGL(k_camera, dim3(grid), dim3(kBlock), 0, s, sc->dev, dc, pc, wb[i], q0, n,
-                         uint32_t(wh), prm->sample_offset);
-      rc = run_bounces<false>(sc->dev, pc, wb[i], q0, n, uint32_t(wh), prm->sample_offset,
-                              prm->max_depth, none, s);
-      if (rc != XYZ_OK) return rc;
-      if (!backward) {
-        // wave k is added to the film after wave k - 1 (sample order per pixel)
-        if (lanes > 1 && k > 0) {
-          hipError_t const e = hipStreamWaitEvent(s, ev.e[(k - 1) % uint64_t(lanes)], 0);
-          if (e != hipSuccess) return hip_fail(e, "render wait");
-        }
-        hipLaunchKernelGGL(k_accumulate, dim3(uint32_t((wh + kBlock - 1) / kBlock)), dim3(kBlock), 0,
-                           s, film, wb[i].col, q0, n, uint32_t(wh));
-        if (lanes > 1) {
-          hipError_t const e = hipEventRecord(ev.e[i], s);
-          if (e != hipSuccess) return hip_fail(e, "render record");
-        }
-      } else {
-        // replay: same paths from their seeds, now with the primal colours known
-        hipLaunchKernelGGL(k_copy_col, dim3(grid), dim3(kBlock), 0, s, wb[i].ctot, wb[i].col, n);
-        hipLaunchKernelGGL(k_camera, dim3(grid), dim3(kBlock), 0, s, sc->dev, dc, pc, wb[i], q0, n,
-                           uint32_t(wh), prm->sample_offset);
-        rc = run_bounces<true>(sc->dev, pc, wb[i], q0, n, uint32_t(wh), prm->sample_offset,
-                               prm->max_depth, ga, s);
-        if (rc != XYZ_OK) return rc;
-      }
-    }
-    if (lanes > 1) {  // join: the caller's stream waits for every lane
-      for (int i = 0; i < lanes; ++i) {
-        hipEvent_t const done = ev.e[kLanes + i];
-        hipError_t e = hipEventRecord(done, ls[i]);
-        if (e == hipSuccess) e = hipStreamWaitEvent(st, done, 0);
-        if (e != hipSuccess) return hip_fail(e, "render join");
-      }
-    }
-    if (stats) {
-      unsigned long long q[kLanes] = {};
-      hipError_t e = hipSuccess;
-      for (int i = 0; i < lanes && e == hipSuccess; ++i)
-        e = hipMemcpyAsync(&q[i], &wb[i].ctr->queries, sizeof(q[i]), hipMemcpyDeviceToHost, st);
-      if (e == hipSuccess) e = hipStreamSynchronize(st);
-      if (e != hipSuccess) return hip_fail(e, "render stats");
-      stats->queries = 0;
-      for (int i = 0; i < lanes; ++i) stats->queries += q[i];
-    }
-  }
-  hipError_t const e = hipGetLastError();
-  if (e != hipSuccess) return hip_fail(e, "render");
+  unsigned long long q = 0;
+  rc = render_jobs({js}, st, backward, GradArgs{d_film, d_tex, d_emit}, stats ? &q : nullptr);
+  if (rc != XYZ_OK) return rc;
   if (stats) {
     (void)hipEventRecord(e1, st);
     (void)hipEventSynchronize(e1);
     float ms = 0.0f;
     (void)hipEventElapsedTime(&ms, e0, e1);
     stats->seconds = double(ms) * 1e-3;
-    stats->paths = total;
-    if (total == 0) stats->queries = 0;
+    stats->paths = js.items;
+    stats->queries = q;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
   }
@@ -1040,6 +1338,7 @@ xyz_batch_tuning default_tuning() {
   t.blocks_per_cu = 4;
   t.block_threads = 128;
   t.use_graph = 1;
+  t.engine = 1;
   return t;
 }
 
@@ -1069,10 +1368,69 @@ int launch_batch_raw(const BatchPlan& plan, const xyz_batch_tuning& t, void* job
 }
 
 int check_tuning(const xyz_batch_tuning& t) {
+  if (t.engine > 1) {
+    set_error("render_batch: unknown engine");
+    return XYZ_E_INVALID;
+  }
+  if (t.engine == 1) return XYZ_OK;
   if (t.tile_pixels == 0 || t.block_threads == 0 || t.block_threads > 1024 ||
       (t.block_threads % 64) != 0 || t.blocks_per_cu == 0 || t.blocks_per_cu > 64) {
     set_error("render_batch: invalid tuning");
     return XYZ_E_INVALID;
+  }
+  return XYZ_OK;
+}
+
+// Engine 1: all jobs through the wavefront kernels in shared waves.
+int render_batch_wavefront(uint32_t n, xyz_scene* const* scenes, const xyz_camera* cams,
+                           const xyz_render_params* prm, float* const* films,
+                           xyz_render_stats* stats, hipStream_t st) {
+  std::vector<JobSpec> js(n);
+  unsigned long long paths = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    JobSpec& x = js[i];
+    std::memset(&x, 0, sizeof(x));
+    if (!scenes[i] || !films[i]) {
+      set_error("render_batch: NULL scene or film in job " + std::to_string(i));
+      return XYZ_E_INVALID;
+    }
+    int rc = make_dev_camera(&cams[i], &x.j.cam);
+    if (rc != XYZ_OK) return rc;
+    rc = make_pt_const(&prm[i], &x.j.c);
+    if (rc != XYZ_OK) return rc;
+    uint64_t const wh = uint64_t(cams[i].width) * cams[i].height;
+    if (wh >= (1ull << 32) || uint64_t(prm[i].sample_offset) + prm[i].spp >= (1ull << 32)) {
+      set_error("render_batch: image too large or sample index overflow in job " + std::to_string(i));
+      return XYZ_E_UNSUPPORTED;
+    }
+    x.j.s = scenes[i]->dev;
+    x.j.film = films[i];
+    x.j.wh = uint32_t(wh);
+    x.j.sample0 = prm[i].sample_offset;
+    x.items = wh * prm[i].spp;
+    x.max_depth = prm[i].max_depth;
+    paths += x.items;
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (stats) {
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    (void)hipEventRecord(e0, st);
+  }
+  unsigned long long q = 0;
+  int const rc = render_jobs(std::move(js), st, false, GradArgs{nullptr, nullptr, nullptr},
+                             stats ? &q : nullptr);
+  if (rc != XYZ_OK) return rc;
+  if (stats) {
+    (void)hipEventRecord(e1, st);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    stats->seconds = double(ms) * 1e-3;
+    stats->paths = paths;
+    stats->queries = q;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
   }
   return XYZ_OK;
 }
@@ -1088,6 +1446,7 @@ int render_batch_impl(uint32_t n, xyz_scene* const* scenes, const xyz_camera* ca
   xyz_batch_tuning const t = tuning ? *tuning : default_tuning();
   int rc = check_tuning(t);
   if (rc != XYZ_OK) return rc;
+  if (t.engine == 1) return render_batch_wavefront(n, scenes, cams, prm, films, stats, st);
   BatchPlan plan;
   rc = plan_batch(n, scenes, cams, prm, films, t.tile_pixels, &plan);
   if (rc != XYZ_OK) return rc;
@@ -1192,6 +1551,7 @@ int autotune_impl(uint32_t n, xyz_scene* const* scenes, const xyz_camera* cams,
   *best = default_tuning();
   if (n == 0) return XYZ_OK;
   // candidates are timed on a reduced sample count of the same jobs
+  const xyz_render_params* const full = prm;
   std::vector<xyz_render_params> trial(prm, prm + n);
   for (auto& p : trial) p.spp = std::max(1u, std::min(p.spp, 4u));
   prm = trial.data();
@@ -1208,6 +1568,7 @@ int autotune_impl(uint32_t n, xyz_scene* const* scenes, const xyz_camera* cams,
         t.blocks_per_cu = bc;
         t.block_threads = th;
         t.use_graph = 0;
+        t.engine = 0;
         uint32_t max_stack = 1;
         for (uint32_t i = 0; i < n; ++i) max_stack = std::max(max_stack, scenes[i]->dev.stack_cap);
         if (size_t(max_stack) * th * sizeof(uint32_t) > 65536) continue;
@@ -1226,6 +1587,23 @@ int autotune_impl(uint32_t n, xyz_scene* const* scenes, const xyz_camera* cams,
       }
     }
   }
+  // the best engine-0 tuning against engine 1, on more samples: engine 1 gains
+  // from large waves, which a 4-sample trial does not form
+  std::vector<xyz_render_params> trial16(full, full + n);
+  for (auto& p : trial16) p.spp = std::max(1u, std::min(p.spp, 16u));
+  xyz_batch_tuning wave = *best;
+  wave.engine = 1;
+  double secs[2] = {0.0, 0.0};
+  for (int k = 0; k < 2; ++k) {
+    xyz_batch_tuning const t = k == 0 ? *best : wave;
+    int rc = render_batch_impl(n, scenes, cams, trial16.data(), films, &t, nullptr, st);  // warm-up
+    if (rc != XYZ_OK) return rc;
+    xyz_render_stats s;
+    rc = render_batch_impl(n, scenes, cams, trial16.data(), films, &t, &s, st);
+    if (rc != XYZ_OK) return rc;
+    secs[k] = s.seconds;
+  }
+  if (secs[1] < secs[0]) best->engine = 1;
   best->use_graph = 1;
   return zero_films(n, cams, films, st);
 }
