@@ -5,7 +5,8 @@
 //   xyz::KDTree::Build           (reference xyz/geom/kdtree.hpp:31)
 //   xyz::Scene::FindIntersection (reference xyz/core/scene.hpp:236-238,277-295)
 //   xyz::PathTracer::run body    (reference xyz/integrator/integrator.cpp:119-128)
-//   ImageFilm::SaveAsPfm         (reference xyz/core/imagefilm.cpp:153-206)
+//   ImageFilm::SaveAsPfm/PfmRgb/Float/Hdr (reference xyz/core/imagefilm.cpp:108-241,
+//                                 xyz/core/imagefilm_hdr.cpp)
 //
 // The reference's code calls these one ray or one pixel at a time; over a GPU
 // boundary the useful unit is a batch, so the batched forms are primary and the
@@ -16,6 +17,7 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -179,8 +181,22 @@ class Scene {
   DeviceBuffer<uint8_t> d_back_;
 };
 
-// ImageFilm::SaveAsPfm: "PF\nW H\n-1\n", little-endian RGB floats, rows bottom
-// up, value = sum / spp, non-finite pixels written as 0.
+// ---- film files (xyz/core/imagefilm.cpp:108-206, xyz/core/imagefilm_hdr.cpp) --
+// A film holds per-pixel sums, row 0 = bottom; every writer divides by the
+// sample count first (value = sum * (1 / spp)).  Like ImageFilm, the .float
+// and .hdr writers treat the film as CIE XYZ and convert it to RGB with the
+// CCIR 601-1 matrix; the PFM writers write the values as they are.  For
+// byte-identical files build with -ffp-contract=off.
+
+// hi::CCIR601_1_XYZ2RGB (libhi/hi/image/color_space.hpp:70-79), in float.
+inline void Ccir601XyzToRgb(const float xyz[3], float rgb[3]) {
+  rgb[0] = 1.9104f * xyz[0] - 0.5338f * xyz[1] - 0.2891f * xyz[2];
+  rgb[1] = -0.9844f * xyz[0] + 1.9985f * xyz[1] - 0.0279f * xyz[2];
+  rgb[2] = 0.0585f * xyz[0] - 0.1187f * xyz[1] + 0.9017f * xyz[2];
+}
+
+// ImageFilm::SaveAsPfm: "PF\nW H\n-1\n", little-endian floats, rows bottom up,
+// non-finite pixels written as 0.
 inline bool SaveAsPfm(const char* path, uint32_t w, uint32_t h, const std::vector<float>& film_sum,
                       float spp) {
   FILE* f = std::fopen(path, "wb");
@@ -194,6 +210,147 @@ inline bool SaveAsPfm(const char* path, uint32_t w, uint32_t h, const std::vecto
     std::fwrite(rgb, sizeof(float), 3, f);
   }
   return std::fclose(f) == 0;
+}
+
+// ImageFilm::SaveAsPfmRgb: the PFM layout, values unchecked.
+inline bool SaveAsPfmRgb(const char* path, uint32_t w, uint32_t h,
+                         const std::vector<float>& film_sum, float spp) {
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return false;
+  std::fprintf(f, "PF\n%u %u\n-1\n", w, h);
+  float const s = 1.0f / spp;
+  for (size_t p = 0; p < size_t(w) * h; ++p) {
+    float const v[3] = {film_sum[3 * p] * s, film_sum[3 * p + 1] * s, film_sum[3 * p + 2] * s};
+    std::fwrite(v, sizeof(float), 3, f);
+  }
+  return std::fclose(f) == 0;
+}
+
+// ImageFilm::SaveAsFloat: headerless RGB floats, rows bottom up, negative
+// components clamped to 0.
+inline bool SaveAsFloat(const char* path, uint32_t w, uint32_t h,
+                        const std::vector<float>& film_sum, float spp) {
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return false;
+  float const s = 1.0f / spp;
+  for (size_t p = 0; p < size_t(w) * h; ++p) {
+    float const xyz[3] = {film_sum[3 * p] * s, film_sum[3 * p + 1] * s, film_sum[3 * p + 2] * s};
+    float rgb[3];
+    Ccir601XyzToRgb(xyz, rgb);
+    for (float& c : rgb)
+      if (c < 0) c = 0;
+    std::fwrite(rgb, sizeof(float), 3, f);
+  }
+  return std::fclose(f) == 0;
+}
+
+namespace detail {
+
+// RGBE pixel (xyz/core/imagefilm_hdr.cpp:12-28): the largest component d maps
+// to a mantissa byte by frexp(d) * 255.9999 / d.
+inline void RgbeEncode(const float rgb[3], uint8_t out[4]) {
+  float const d = std::max(std::max(rgb[0], rgb[1]), rgb[2]);
+  if (d <= 1e-32) {
+    out[0] = out[1] = out[2] = out[3] = 0;
+    return;
+  }
+  int e = 0;
+  float const f = float(double(std::frexp(d, &e)) * 255.9999 / double(d));
+  // a negative component (XYZ -> RGB can give one) is undefined in the
+  // reference's cast; here it truncates to int and wraps modulo 256
+  out[0] = uint8_t(int(rgb[0] * f));
+  out[1] = uint8_t(int(rgb[1] * f));
+  out[2] = uint8_t(int(rgb[2] * f));
+  out[3] = uint8_t(e + 128);
+}
+
+// One scanline, run-length encoded per component (xyz/core/imagefilm_hdr.cpp:
+// 44-109): lines shorter than 8 or longer than 32767 pixels are written flat;
+// otherwise "2 2 hi lo", then for each component a sequence of literal packets
+// (count <= 128, then the bytes) and run packets (128 + count <= 127, then the
+// byte); runs shorter than 4 stay literal.
+inline void RgbeWriteLine(const std::vector<uint8_t>& line, std::string* out) {
+  int const size = int(line.size() / 4);
+  if (size < 8 || size > 0x7fff) {
+    out->append(reinterpret_cast<const char*>(line.data()), line.size());
+    return;
+  }
+  out->push_back(char(2));
+  out->push_back(char(2));
+  out->push_back(char((size >> 8) & 0xff));
+  out->push_back(char(size & 0xff));
+  auto at = [&](int i, int c) { return line[size_t(4 * i + c)]; };
+  for (int c = 0; c < 4; ++c) {
+    for (int j = 0, count = 0; j < size; j += count) {
+      // the next run of at least 4 equal bytes, from j on
+      int begin = j;
+      for (; begin < size; begin += count) {
+        for (count = 1; count < 127 && begin + count < size && at(begin + count, c) == at(begin, c);
+             ++count) {
+        }
+        if (count >= 4) break;
+      }
+      // a short run (2 or 3 equal bytes) right before it goes out as a run
+      if (begin - j > 1 && begin - j < 4) {
+        int k = j + 1;
+        while (at(k++, c) == at(j, c)) {
+          if (k == begin) {
+            out->push_back(char(begin - j + 128));
+            out->push_back(char(at(j, c)));
+            j = begin;
+            break;
+          }
+        }
+      }
+      // literals up to the run
+      while (j < begin) {
+        int n = begin - j;
+        if (n > 128) n = 128;
+        out->push_back(char(n));
+        while (n--) out->push_back(char(at(j++, c)));
+      }
+      if (count >= 4) {
+        out->push_back(char(count + 128));
+        out->push_back(char(at(begin, c)));
+      } else {
+        count = 0;
+      }
+    }
+  }
+}
+
+}  // namespace detail
+
+// ImageFilm::SaveAsHdr (xyz/core/imagefilm_hdr.cpp:114-187): Radiance RGBE,
+// "-Y H +X W", top row first, XYZ -> RGB, non-finite pixels as 0.
+inline bool SaveAsHdr(const char* path, uint32_t w, uint32_t h, const std::vector<float>& film_sum,
+                      float spp) {
+  std::string out =
+      "#?RADIANCE\n"
+      "# Made with Mikado Renderer\n"
+      "FORMAT=32-bit_rle_rgbe\n"
+      "EXPOSURE=1.0000000000000\n"
+      "\n";
+  out += "-Y " + std::to_string(h) + " +X " + std::to_string(w) + "\n";
+  float const s = 1.0f / spp;
+  std::vector<uint8_t> line(size_t(w) * 4);
+  for (uint32_t y = 0; y < h; ++y) {
+    size_t const row = size_t(h - 1 - y) * w;
+    for (uint32_t x = 0; x < w; ++x) {
+      size_t const p = row + x;
+      float const xyz[3] = {film_sum[3 * p] * s, film_sum[3 * p + 1] * s, film_sum[3 * p + 2] * s};
+      float rgb[3];
+      Ccir601XyzToRgb(xyz, rgb);
+      if (!std::isfinite(rgb[0]) || !std::isfinite(rgb[1]) || !std::isfinite(rgb[2]))
+        rgb[0] = rgb[1] = rgb[2] = 0.0f;
+      detail::RgbeEncode(rgb, &line[4 * size_t(x)]);
+    }
+    detail::RgbeWriteLine(line, &out);
+  }
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return false;
+  size_t const wrote = std::fwrite(out.data(), 1, out.size(), f);
+  return std::fclose(f) == 0 && wrote == out.size();
 }
 
 }  // namespace xyz_hip
