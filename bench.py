@@ -35,6 +35,8 @@ def parse():
     ap.add_argument("--skip-pt", action="store_true", help="skip the 1080p path-tracing leg")
     ap.add_argument("--skip-adjoint", action="store_true", help="skip the adjoint/primal leg")
     ap.add_argument("--skip-batch", action="store_true", help="skip the batched/sequential leg")
+    ap.add_argument("--skip-kd-build", action="store_true",
+                    help="skip the device kd-tree build timing")
     ap.add_argument("--pt-spp", type=int, default=16,
                     help="samples per pixel timed in the 1080p leg (the workload renders 256)")
     ap.add_argument("--pt-cpu-rows", type=int, default=128,
@@ -192,6 +194,7 @@ def main():
                                       sample=f"every {s1}th ray of both batches ({n1} rays)"))
         log(f"[bench] cpu baseline: {cpu}")
 
+    kdb = None if args.skip_kd_build else bench_kd_build(capi, mesh, info)
     pt = None if args.skip_pt else bench_pt(args, hip, capi, scenes)
     adj = None if args.skip_adjoint else bench_adjoint(args, hip, capi, scenes)
     bat = None if args.skip_batch else bench_batch(args, hip, capi, scenes)
@@ -223,8 +226,38 @@ def main():
         "pt_1080p": pt,
         "adjoint": adj,
         "batched": bat,
+        "kd_build": kdb,
     }
     print(json.dumps(out), flush=True)
+
+
+def bench_kd_build(capi, mesh, info):
+    """SAH kd-tree build of the workload mesh on the device vs the host builder
+    (the scene above was built on the host); the trees must be identical."""
+    host = capi.KDTree(mesh["v0"], mesh["v1"], mesh["v2"])
+    hn, hp = host.export()
+    host_s = host.info()["build_seconds"]
+    del host
+    secs = []
+    same = True
+    for _ in range(3):  # the first call also loads the build kernels
+        dev = capi.KDTree(mesh["v0"], mesh["v1"], mesh["v2"], device=True)
+        secs.append(dev.info()["build_seconds"])
+        dn, dp = dev.export()
+        leaf = (hn[:, 0] & 3) == 3
+        same = same and hn.shape == dn.shape and bool(
+            np.array_equal(hn[:, 0], dn[:, 0]) and np.array_equal(hn[leaf, 1], dn[leaf, 1])
+            and np.array_equal(hn[~leaf, 1].view(np.float32), dn[~leaf, 1].view(np.float32))
+            and np.array_equal(hp, dp))
+        del dev
+    dev_s = min(secs[1:])
+    r = dict(n_tris=int(info["n_tris"]), host_seconds=round(host_s, 4),
+             device_seconds=round(dev_s, 4), first_device_call_seconds=round(secs[0], 4),
+             speedup=round(host_s / dev_s, 2), identical_trees=same,
+             note="host arrays in, host tree out; device time includes upload, root box and "
+                  "pre-order layout on the host")
+    log(f"[bench] kd build: {r}")
+    return r
 
 
 def bench_pt(args, hip, capi, scenes):

@@ -66,7 +66,9 @@ typedef struct {
  *   textures: tex_wh[2*i] = width, tex_wh[2*i+1] = height, texels are the
  *   textures' RGB floats concatenated, row 0 first.
  *   kd_max_bucket / kd_max_depth: KDTree(max_bucket=16, max_depth=16)
- *   (reference xyz/geom/kdtree.hpp:19-20); 0 selects the default 16. */
+ *   (reference xyz/geom/kdtree.hpp:19-20); 0 selects the default 16.
+ *   kd_builder: XYZ_KD_HOST (0) builds the tree on the host, XYZ_KD_DEVICE (1)
+ *   on the current GPU (xyz_kdtree_build_device); both give the same tree. */
 typedef struct {
   uint32_t n_tris;
   const float *v0, *v1, *v2;
@@ -80,7 +82,10 @@ typedef struct {
   const float *texels;
   uint32_t kd_max_bucket;
   uint32_t kd_max_depth;
+  uint32_t kd_builder;
 } xyz_scene_desc;
+
+enum { XYZ_KD_HOST = 0, XYZ_KD_DEVICE = 1 };
 
 typedef struct {
   uint64_t n_tris;
@@ -90,7 +95,7 @@ typedef struct {
   uint32_t depth;         /* deepest interior level + 1 */
   uint32_t max_leaf;      /* largest leaf */
   float bmin[3], bmax[3]; /* root box (bmax is the builder's upper bound) */
-  double build_seconds;   /* host SAH build */
+  double build_seconds;   /* SAH build, host or device */
   uint64_t device_bytes;  /* HBM held by the scene (0 for a host tree) */
 } xyz_scene_info;
 
@@ -104,6 +109,13 @@ typedef struct {
  *   keeps caller order in leaves). */
 int xyz_kdtree_build(uint32_t n_tris, const float *v0, const float *v1, const float *v2,
                      uint32_t max_bucket, uint32_t max_depth, xyz_kdtree **out);
+/* The same build on the current GPU (level by level: sorted events, segmented
+ * scans, an atomic first-minimum per node and axis), returning the same tree
+ * as xyz_kdtree_build for the same input; a split of +-0 may differ in the
+ * sign bit only.  Host arrays in, host tree out; synchronous.  Replaces the
+ * same KDTree::Build call (xyz/geom/kdtree.cpp:207-357). */
+int xyz_kdtree_build_device(uint32_t n_tris, const float *v0, const float *v1, const float *v2,
+                            uint32_t max_bucket, uint32_t max_depth, xyz_kdtree **out);
 int xyz_kdtree_info(const xyz_kdtree *tree, xyz_scene_info *info);
 int xyz_kdtree_export(const xyz_kdtree *tree, uint32_t *nodes, uint32_t *leaf_prims);
 void xyz_kdtree_destroy(xyz_kdtree *tree);

@@ -38,7 +38,10 @@ class SceneDesc(ctypes.Structure):
                 ("bsdf", _u32p), ("n_bsdfs", ctypes.c_uint32),
                 ("bsdfs", ctypes.POINTER(BsdfDesc)), ("n_textures", ctypes.c_uint32),
                 ("tex_wh", _u32p), ("texels", _f32p), ("kd_max_bucket", ctypes.c_uint32),
-                ("kd_max_depth", ctypes.c_uint32)]
+                ("kd_max_depth", ctypes.c_uint32), ("kd_builder", ctypes.c_uint32)]
+
+
+KD_HOST, KD_DEVICE = 0, 1
 
 
 class SceneInfo(ctypes.Structure):
@@ -81,7 +84,7 @@ class BatchTuning(ctypes.Structure):
 
 
 # Every symbol declared in include/xyz_hip.h.
-EXPORTS = ["xyz_kdtree_build", "xyz_kdtree_info", "xyz_kdtree_export", "xyz_kdtree_destroy",
+EXPORTS = ["xyz_kdtree_build", "xyz_kdtree_build_device", "xyz_kdtree_info", "xyz_kdtree_export", "xyz_kdtree_destroy",
            "xyz_scene_create", "xyz_scene_get_info", "xyz_scene_destroy", "xyz_closest_hit",
            "xyz_closest_hit_work",
            "xyz_render", "xyz_render_backward", "xyz_render_batch", "xyz_batch_autotune",
@@ -106,6 +109,7 @@ def lib():
         L = ctypes.CDLL(path)
         L.xyz_kdtree_build.argtypes = [ctypes.c_uint32, _f32p, _f32p, _f32p, ctypes.c_uint32,
                                        ctypes.c_uint32, ctypes.POINTER(_vp)]
+        L.xyz_kdtree_build_device.argtypes = L.xyz_kdtree_build.argtypes
         L.xyz_kdtree_info.argtypes = [_vp, ctypes.POINTER(SceneInfo)]
         L.xyz_kdtree_export.argtypes = [_vp, _u32p, _u32p]
         L.xyz_kdtree_destroy.argtypes = [_vp]
@@ -165,13 +169,17 @@ def _stream(stream):
 
 
 class KDTree:
-    """Host SAH kd-tree (reference KDTree::Build), flattened to the device layout."""
+    """SAH kd-tree (reference KDTree::Build), flattened to the device layout.
 
-    def __init__(self, v0, v1, v2, max_bucket=16, max_depth=16):
+    device=False builds on the host (no GPU needed), device=True on the current
+    GPU; both give the same tree."""
+
+    def __init__(self, v0, v1, v2, max_bucket=16, max_depth=16, device=False):
         v0, v1, v2 = (_np(a, np.float32).reshape(-1, 3) for a in (v0, v1, v2))
         h = _vp()
-        check(lib().xyz_kdtree_build(v0.shape[0], _ptr(v0, _f32p), _ptr(v1, _f32p),
-                                     _ptr(v2, _f32p), max_bucket, max_depth, ctypes.byref(h)))
+        fn = lib().xyz_kdtree_build_device if device else lib().xyz_kdtree_build
+        check(fn(v0.shape[0], _ptr(v0, _f32p), _ptr(v1, _f32p), _ptr(v2, _f32p), max_bucket,
+                 max_depth, ctypes.byref(h)))
         self.h = h
 
     def __del__(self):
@@ -214,7 +222,7 @@ class Scene:
     """
 
     def __init__(self, v0, v1, v2, tri_bsdf=None, bsdfs=None, normals=None, uv=None,
-                 textures=None, max_bucket=16, max_depth=16):
+                 textures=None, max_bucket=16, max_depth=16, kd_builder=KD_HOST):
         v0, v1, v2 = (_np(a, np.float32).reshape(-1, 3) for a in (v0, v1, v2))
         n = v0.shape[0]
         if bsdfs is None:
@@ -250,6 +258,7 @@ class Scene:
             self.n_texels = tx.size
         d.kd_max_bucket = max_bucket
         d.kd_max_depth = max_depth
+        d.kd_builder = kd_builder
         h = _vp()
         check(lib().xyz_scene_create(ctypes.byref(d), ctypes.byref(h)))
         self.h = h

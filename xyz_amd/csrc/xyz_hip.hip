@@ -73,8 +73,14 @@ constexpr uint32_t kHitBlocksPerCu = XYZ_HIT_BLOCKS_PER_CU;
 // query finishes the lane takes the next ray of its wave's batch (batches of 64
 // consecutive rays from a global counter), so a wave's SIMD lanes stay busy
 // whatever the spread of per-ray cost.
+#ifndef XYZ_HIT_WAVES_PER_EU
+#define XYZ_HIT_WAVES_PER_EU 0
+#endif
 template <bool kCount>
 __global__ void __launch_bounds__(kHitBlock)
+#if XYZ_HIT_WAVES_PER_EU
+__attribute__((amdgpu_waves_per_eu(XYZ_HIT_WAVES_PER_EU, XYZ_HIT_WAVES_PER_EU)))
+#endif
 closest_hit_kernel(DevScene s, uint32_t n, const float* __restrict__ org,
                    const float* __restrict__ dir, int32_t* __restrict__ prim,
                    float* __restrict__ tout, float* __restrict__ fout, float* __restrict__ gout,
@@ -329,8 +335,15 @@ int scene_create(const xyz_scene_desc* desc, xyz_scene** out) {
     return XYZ_E_UNSUPPORTED;
   }
   std::unique_ptr<xyz_scene> sc(new xyz_scene());
-  std::string err = build_flat_tree(n, desc->v0, desc->v1, desc->v2, desc->kd_max_bucket,
-                                    desc->kd_max_depth, &sc->tree);
+  if (desc->kd_builder != XYZ_KD_HOST && desc->kd_builder != XYZ_KD_DEVICE) {
+    set_error("xyz_scene_create: kd_builder must be XYZ_KD_HOST or XYZ_KD_DEVICE");
+    return XYZ_E_INVALID;
+  }
+  std::string err = desc->kd_builder == XYZ_KD_DEVICE
+                        ? build_flat_tree_device(n, desc->v0, desc->v1, desc->v2, desc->kd_max_bucket,
+                                                 desc->kd_max_depth, &sc->tree)
+                        : build_flat_tree(n, desc->v0, desc->v1, desc->v2, desc->kd_max_bucket,
+                                          desc->kd_max_depth, &sc->tree);
   if (!err.empty()) {
     set_error("xyz_scene_create: " + err);
     return XYZ_E_INVALID;
@@ -556,6 +569,24 @@ int xyz_kdtree_build(uint32_t n_tris, const float* v0, const float* v1, const fl
   if (!err.empty()) {
     set_error("xyz_kdtree_build: " + err);
     return XYZ_E_INVALID;
+  }
+  t->n_tris = n_tris;
+  *out = t.release();
+  return XYZ_OK;
+}
+
+int xyz_kdtree_build_device(uint32_t n_tris, const float* v0, const float* v1, const float* v2,
+                            uint32_t max_bucket, uint32_t max_depth, xyz_kdtree** out) {
+  if (!out) {
+    set_error("xyz_kdtree_build_device: NULL out");
+    return XYZ_E_INVALID;
+  }
+  *out = nullptr;
+  std::unique_ptr<xyz_kdtree> t(new xyz_kdtree());
+  std::string err = build_flat_tree_device(n_tris, v0, v1, v2, max_bucket, max_depth, &t->t);
+  if (!err.empty()) {
+    set_error("xyz_kdtree_build_device: " + err);
+    return err.rfind("device kd build", 0) == 0 ? XYZ_E_DEVICE : XYZ_E_INVALID;
   }
   t->n_tris = n_tris;
   *out = t.release();

@@ -28,6 +28,10 @@ struct FlatTree {
 std::string build_flat_tree(uint32_t n, const float* v0, const float* v1, const float* v2,
                             uint32_t max_bucket, uint32_t max_depth, FlatTree* out);
 
+// The same build on the current GPU (kdtree_gpu.hip); the same trees.
+std::string build_flat_tree_device(uint32_t n, const float* v0, const float* v1, const float* v2,
+                                   uint32_t max_bucket, uint32_t max_depth, FlatTree* out);
+
 void set_error(const std::string& msg);
 
 }  // namespace xyzhip
