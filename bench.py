@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--skip-batch", action="store_true", help="skip the batched/sequential leg")
     ap.add_argument("--skip-kd-build", action="store_true",
                     help="skip the device kd-tree build timing")
+    ap.add_argument("--skip-mqo", action="store_true",
+                    help="skip the Metasequoia scene load timing")
     ap.add_argument("--pt-spp", type=int, default=16,
                     help="samples per pixel timed in the 1080p leg (the workload renders 256)")
     ap.add_argument("--pt-cpu-rows", type=int, default=128,
@@ -195,6 +197,7 @@ def main():
         log(f"[bench] cpu baseline: {cpu}")
 
     kdb = None if args.skip_kd_build else bench_kd_build(capi, mesh, info)
+    mqo_leg = None if args.skip_mqo else bench_mqo(capi, scenes)
     pt = None if args.skip_pt else bench_pt(args, hip, capi, scenes)
     adj = None if args.skip_adjoint else bench_adjoint(args, hip, capi, scenes)
     bat = None if args.skip_batch else bench_batch(args, hip, capi, scenes)
@@ -227,6 +230,7 @@ def main():
         "adjoint": adj,
         "batched": bat,
         "kd_build": kdb,
+        "mqo_load": mqo_leg,
     }
     print(json.dumps(out), flush=True)
 
@@ -258,6 +262,47 @@ def bench_kd_build(capi, mesh, info):
                   "pre-order layout on the host")
     log(f"[bench] kd build: {r}")
     return r
+
+
+def mqo_document(v0, v1, v2):
+    """Metasequoia text of a triangle soup (centimetres, one Lambert material)."""
+    import io
+    n = len(v0)
+    verts = np.stack([v0, v1, v2], 1).reshape(-1, 3) * 100.0
+    buf = io.StringIO()
+    buf.write("Metasequoia Document\nFormat Text Ver 1.0\n\nMaterial 1 {\n"
+              '\t"mcc-white" shader(3) col(1.000 1.000 1.000 1.000) dif(0.800) amb(0.600) '
+              "emi(0.000) spc(0.000) power(5.00)\n}\n")
+    buf.write(f'Object "mesh" {{\n\tvisible 15\n\tshading 0\n\tvertex {3 * n} {{\n')
+    np.savetxt(buf, verts, fmt="\t\t%.4f %.4f %.4f")
+    buf.write(f"\t}}\n\tface {n} {{\n")
+    idx = np.arange(3 * n, dtype=np.int64).reshape(-1, 3)
+    np.savetxt(buf, idx, fmt="\t\t3 V(%d %d %d) M(0)")
+    buf.write("\t}\n}\nEof\n")
+    return buf.getvalue()
+
+
+def bench_mqo(capi, scenes):
+    """Metasequoia text -> device scene: parse (xyz_mqo_load) and scene creation
+    with the host and the device kd-tree builders, on a displaced icosphere."""
+    from xyz_amd import mqo
+    m = scenes.closest_hit_mesh(6, 64, seed=3)
+    text = mqo_document(m["v0"], m["v1"], m["v2"])
+    t0 = time.perf_counter()
+    doc = mqo.loads(text)
+    t_parse = time.perf_counter() - t0
+    out = dict(n_tris=doc.n_tris, megabytes=round(len(text) / 1e6, 2),
+               parse_seconds=round(t_parse, 4),
+               parse_mb_per_s=round(len(text) / 1e6 / t_parse, 1))
+    for name, builder in (("host", capi.KD_HOST), ("device", capi.KD_DEVICE)):
+        doc.scene(kd_builder=builder)  # warm-up (first call loads kernels)
+        t0 = time.perf_counter()
+        sc = doc.scene(kd_builder=builder)
+        out[f"scene_create_seconds_{name}_kd"] = round(time.perf_counter() - t0, 4)
+        out[f"kd_build_seconds_{name}"] = round(sc.info()["build_seconds"], 4)
+        del sc
+    log(f"[bench] mqo load: {out}")
+    return out
 
 
 def bench_pt(args, hip, capi, scenes):

@@ -125,6 +125,41 @@ int xyz_scene_create(const xyz_scene_desc *desc, xyz_scene **out);
 int xyz_scene_get_info(const xyz_scene *scene, xyz_scene_info *info);
 void xyz_scene_destroy(xyz_scene *scene);
 
+/* ---- Metasequoia scenes -----------------------------------------------------
+ * Replaces Scene::Load (reference xyz/core/scene_mqo.cpp:547-777): parses a
+ * Metasequoia "Format Text Ver 1.0" document with the reference's rules.
+ *   materials -> BSDFs (scene_mqo.cpp:126-235): emi(1) -> light; col alpha
+ *     != 1 -> glass; spc(1) with power(100) -> mirror, spc(1) with another
+ *     positive power -> metal (shininess = power); otherwise Lambert.  Lambert
+ *     and metal colours are a Macbeth patch named in the material name
+ *     ("mcc-white", ...), else pow(col * spc, 2.2) through the CCIR 601-1
+ *     RGB -> XYZ matrix, clamped to [0, 1]: film channels are CIE XYZ, as in
+ *     the reference;
+ *   vertices are scaled from centimetres to metres; objects with "shading 1"
+ *     get angle-weighted smooth vertex normals, flat ones n0 = n1 = n2 = ng;
+ *   every light emits the reference's global power Phi (mercury arc lamp,
+ *     xyz/core/config.hpp:81-94) as L_e = Phi / (pi * total light area);
+ *   the Scene chunk's pos / lookat / head / pich give the camera eye and at
+ *     (default eye (0,1,4), at (0,1,0)).
+ * Triangles keep file order (primitive id = order of faces in the file; the
+ * reference's final Morton sort is not reproduced).  xyz_mqo_describe fills a
+ * scene description whose arrays belong to the xyz_mqo handle (valid until
+ * xyz_mqo_destroy), ready for xyz_scene_create. */
+typedef struct xyz_mqo xyz_mqo;
+typedef struct {
+  uint32_t n_tris, n_bsdfs;
+  uint32_t n_objects, n_smooth_objects, n_lights;
+  uint32_t has_camera;    /* 1 if the document had a Scene chunk */
+  float eye[3], at[3];    /* camera position and gazing point, metres */
+  float light_power[3];   /* Phi, CIE XYZ */
+  float surface_area;     /* of all visible triangles, m^2 */
+  float light_area;
+} xyz_mqo_info;
+int xyz_mqo_load(const char *text, uint64_t len, xyz_mqo **out);
+int xyz_mqo_load_file(const char *path, xyz_mqo **out);
+int xyz_mqo_describe(const xyz_mqo *mqo, xyz_scene_desc *desc, xyz_mqo_info *info);
+void xyz_mqo_destroy(xyz_mqo *mqo);
+
 /* ---- closest hit -------------------------------------------------------------
  * Replaces Scene::FindIntersection / KDTree::FindIntersection
  * (reference xyz/core/scene.hpp:236-238,277-295; xyz/geom/kdtree.cpp:366-372)

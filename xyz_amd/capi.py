@@ -77,6 +77,21 @@ class RenderStats(ctypes.Structure):
                 ("seconds", ctypes.c_double)]
 
 
+class MqoInfo(ctypes.Structure):
+    _fields_ = [("n_tris", ctypes.c_uint32), ("n_bsdfs", ctypes.c_uint32),
+                ("n_objects", ctypes.c_uint32), ("n_smooth_objects", ctypes.c_uint32),
+                ("n_lights", ctypes.c_uint32), ("has_camera", ctypes.c_uint32),
+                ("eye", ctypes.c_float * 3), ("at", ctypes.c_float * 3),
+                ("light_power", ctypes.c_float * 3), ("surface_area", ctypes.c_float),
+                ("light_area", ctypes.c_float)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        for k in ("eye", "at", "light_power"):
+            d[k] = list(d[k])
+        return d
+
+
 class BatchTuning(ctypes.Structure):
     _fields_ = [("tile_pixels", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32),
                 ("block_threads", ctypes.c_uint32), ("use_graph", ctypes.c_uint32),
@@ -85,6 +100,7 @@ class BatchTuning(ctypes.Structure):
 
 # Every symbol declared in include/xyz_hip.h.
 EXPORTS = ["xyz_kdtree_build", "xyz_kdtree_build_device", "xyz_kdtree_info", "xyz_kdtree_export", "xyz_kdtree_destroy",
+           "xyz_mqo_load", "xyz_mqo_load_file", "xyz_mqo_describe", "xyz_mqo_destroy",
            "xyz_scene_create", "xyz_scene_get_info", "xyz_scene_destroy", "xyz_closest_hit",
            "xyz_closest_hit_work",
            "xyz_render", "xyz_render_backward", "xyz_render_batch", "xyz_batch_autotune",
@@ -114,6 +130,11 @@ def lib():
         L.xyz_kdtree_export.argtypes = [_vp, _u32p, _u32p]
         L.xyz_kdtree_destroy.argtypes = [_vp]
         L.xyz_kdtree_destroy.restype = None
+        L.xyz_mqo_load.argtypes = [ctypes.c_char_p, ctypes.c_uint64, ctypes.POINTER(_vp)]
+        L.xyz_mqo_load_file.argtypes = [ctypes.c_char_p, ctypes.POINTER(_vp)]
+        L.xyz_mqo_describe.argtypes = [_vp, ctypes.POINTER(SceneDesc), ctypes.POINTER(MqoInfo)]
+        L.xyz_mqo_destroy.argtypes = [_vp]
+        L.xyz_mqo_destroy.restype = None
         L.xyz_scene_create.argtypes = [ctypes.POINTER(SceneDesc), ctypes.POINTER(_vp)]
         L.xyz_scene_get_info.argtypes = [_vp, ctypes.POINTER(SceneInfo)]
         L.xyz_scene_destroy.argtypes = [_vp]
