@@ -235,6 +235,25 @@ typedef struct {
 int xyz_render(xyz_scene *scene, const xyz_camera *camera, const xyz_render_params *params,
                float *film, xyz_render_stats *stats, void *stream);
 
+/* Bidirectional path tracing: replaces BidirectionalPathTracer::run's pixel loop
+ * around Scene::EvaluateBidirectionalPathTracing (reference
+ * xyz/integrator/integrator.cpp:1025-1081,
+ * xyz/core/scene_bidirectionalpathtracing.cpp:8-522), on the same hit kernel.
+ * Each sample of a pixel adds its own colour to film; its light-tracing
+ * (t = 1) contributions go to the pixels they land on, in splat_film, or in
+ * film when splat_film is NULL (both device float[width*height*3], row 0 =
+ * bottom, SUMS: divide by the total spp).  The estimator is the reference's,
+ * with its multiple-importance weights as it computes them (a connection's
+ * density written into a path vertex stays there for later strategies), its
+ * Lambertian connection factors and its pinhole eye path (a thin-lens camera
+ * contributes its lens-film distance and focal length only).  Russian
+ * roulette is the BSDFs' (albedo), so rr_start and rr_threshold are unused;
+ * max_depth <= 16.  Random numbers: Philox streams 2 (light path) and 3 (eye
+ * path), keyed like xyz_render's.  Splats are float atomics, so splat sums
+ * may differ in the last bits between runs; own colours do not. */
+int xyz_render_bdpt(xyz_scene *scene, const xyz_camera *camera, const xyz_render_params *params,
+                    float *film, float *splat_film, xyz_render_stats *stats, void *stream);
+
 /* Path-replay adjoint of xyz_render (no counterpart in the reference).  Given
  * d_film = dLoss/d(film SUM) (device float[w*h*3]), adds dLoss/d(texel) to
  * d_texels (device, laid out like the scene's texels) and dLoss/dL_e to

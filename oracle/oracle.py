@@ -61,6 +61,10 @@ def lib():
         L.orc_primary_ray.restype = None
         L.orc_primary_dir.argtypes = [_f32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float,
                                       ctypes.c_float, _f32p]
+        L.orc_render_bdpt.restype = ctypes.c_double
+        L.orc_render_bdpt.argtypes = [_vp, _f32p, ctypes.c_uint32, ctypes.c_uint32,
+                                      ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                      ctypes.c_uint64, _f32p, _f32p, _u64p, ctypes.c_int]
         L.orc_render.restype = ctypes.c_double
         L.orc_render.argtypes = [_vp, _f32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                  ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float,
@@ -217,6 +221,23 @@ class OracleScene:
         if stats:
             return film, dict(seconds=secs, queries=int(st[0]), nodes=int(st[1]), tris=int(st[2]))
         return film
+
+    def render_bdpt(self, camera, W, H, spp, max_depth, seed=0, sample0=0, threads=0,
+                    stats=False):
+        """Bidirectional path tracing: (own, splat) films of per-pixel sums, (H, W, 3)
+        float32, row 0 = bottom; the image is (own + splat) / spp."""
+        cam = camera12(camera)
+        own = np.zeros((H, W, 3), np.float32)
+        splat = np.zeros((H, W, 3), np.float32)
+        st = np.zeros(3, np.uint64)
+        secs = lib().orc_render_bdpt(self.h, _p(cam, _f32p), W, H, spp, sample0, max_depth, seed,
+                                     _p(own, _f32p), _p(splat, _f32p), _p(st, _u64p), threads)
+        if secs < 0:
+            raise ValueError("max_depth above 16")
+        if stats:
+            return own, splat, dict(seconds=secs, queries=int(st[0]), nodes=int(st[1]),
+                                    tris=int(st[2]))
+        return own, splat
 
     def render_backward(self, camera, W, H, spp, max_depth, d_film, rr_start=0,
                         rr_threshold=0.2, seed=0, sample0=0, threads=0):

@@ -20,6 +20,9 @@
 //            Glass    xyz/bsdf/glass.cpp:96-165,    Metal  xyz/bsdf/metal.cpp:68-153,
 //            Light    xyz/bsdf/light.cpp:18-33, helpers xyz/stdafx.hpp:44-168
 //   * progressive driver + film                        xyz/integrator/integrator.cpp:88-162
+//   * bidirectional path tracing (light tracing, connections, the reference's
+//     MIS weights)    xyz/core/scene_bidirectionalpathtracing.cpp:8-522,
+//                     xyz/integrator/integrator.cpp:1025-1081 (see bdpt_sample())
 //
 // Arithmetic contract: IEEE binary32, expression order exactly as the reference
 // sources (dot = (a0*b0 + a1*b1) + a2*b2, libhi/hi/math/vector3.hpp:121-122),
@@ -862,6 +865,497 @@ V3 pt_path(Scene const& sc, Camera const& cam, PTParams const& pp, uint32_t W, u
   return color;
 }
 
+// ------------------------------------------------------------------ BDPT
+// Scene::EvaluateBidirectionalPathTracing (xyz/core/scene_bidirectionalpathtracing.cpp:8-522)
+// with the bidirectional BSDF methods (xyz/bsdf/{lambert,mirror,glass,metal,light}.cpp
+// LightsToCamera/CameraToLightsScatteringDirection) and the pinhole camera's
+// film mapping (xyz/core/thinlenscamera.hpp:38-90, thinlenscamera.cpp:125-157).
+// Reference behaviour kept as it is:
+//   * hi::single_stack (libhi/hi/util.hpp:24-36) assigns its value for good (its
+//     destructor writes the new value back), so each connection's temporary
+//     density stays in the vertex and later strategies read it;
+//   * Mirror::CameraToLightsScatteringDirection ends the eye path when the
+//     reflected direction is ABOVE the geometric surface (mirror.cpp:235-239);
+//   * connections weigh every non-specular vertex as Lambertian (M_1_PI).
+// Vertex fields the reference would read before any write in the same sample
+// (it reuses the vertex arrays across samples) start from zero / false here.
+struct BVert {
+  int tri = -1;
+  bool back = false, specular = false;
+  V3 pos, win, wout, ng, ns, t, b, power;
+  real bsdf_x_ipdf = 0, s_prev = 0, s_next = 0, cos_in_sh = 0, cos_out_geo = 0, g = 0;
+};
+
+constexpr uint32_t kBdptStreamLight = 2, kBdptStreamEye = 3;
+
+// xyz/stdafx.hpp:50-63 (cos/sin as the path tracer's)
+real diffused(real mu_t, real mu_p, V3 const& t, V3 const& n, V3 const& b, V3* out) {
+  real const phi = real(2 * M_PI) * mu_p;
+  real const ct = std::sqrt(mu_t);
+  real const st = std::sqrt(1 - mu_t);
+  *out = (t * (st * xyz_cos(phi)) + b * (st * xyz_sin(phi))) + n * ct;
+  return ct;
+}
+
+// xyz/stdafx.hpp:154-168
+real glossy_half(real mu_t, real mu_p, real shin, V3 const& t, V3 const& n, V3 const& b, V3* h) {
+  real const phi = real(2 * M_PI) * mu_p;
+  real const ct = std::pow(mu_t, rcp(shin));
+  real const st = std::sqrt(real(1) - ct * ct);
+  *h = (t * (st * xyz_cos(phi)) + b * (st * xyz_sin(phi))) + n * ct;
+  return ct;
+}
+
+// Triangle::GeometricBasis (xyz/geom/triangle.cpp:167-192)
+void geometric_basis(Tri const& tr, Hit const& h, BVert* v) {
+  V3 ns = normalize((tr.n0 + h.f * tr.s1) + h.g * tr.s2);
+  V3 ng = tr.ng;
+  if (h.back) { ng = -ng; ns = -ns; }
+  V3 t(-ng[2], 0, ng[0]);
+  real const l2 = length_squared(t);
+  V3 b;
+  if (l2 > 0) {
+    t = t * rsqrt(l2);
+    b = cross(t, ng);
+  } else {
+    b = V3(0, 0, (ng[1] > 0) ? 1 : -1);
+    t = cross(ng, b);
+  }
+  v->ns = ns; v->ng = ng; v->t = t; v->b = b;
+}
+
+real glass_ior_sq(Bsdf const& b, bool back) {
+  real const r = back ? rcp(b.ior) : b.ior;
+  return r * r;
+}
+
+real min1(real x) { return (x < real(1)) ? x : real(1); }  // std::min(float_t(1), x)
+
+// Bsdf::LightsToCameraScatteringDirection; u = (dir u1, dir u2, choice)
+bool lights_to_camera(Scene const& sc, Bsdf const& bs, Tri const& tr, Hit const& h, BVert* v,
+                      Draw const& u) {
+  switch (bs.type) {
+    case LAMBERT: {  // lambert.cpp:314-365
+      int texel;
+      V3 const alb = albedo_at(sc, bs, tr, h, &texel);
+      real const cig = dot(v->win, v->ng);
+      real const alpha = v->cos_in_sh / cig;
+      real const pp = min1(alb[1]), pn = min1(alb[1] * alpha);
+      v->specular = false;
+      v->s_prev = real(M_1_PI * double(pp));
+      v->s_next = real(M_1_PI * double(pn));
+      v->power = v->power * alb;
+      v->power = v->power * alpha;
+      v->bsdf_x_ipdf = rcp(pn);
+      if (u.u[2] >= pn) return false;
+      v->cos_out_geo = diffused(u.u[0], u.u[1], v->t, v->ng, v->b, &v->wout);
+      if (dot(v->wout, v->ns) <= 0) return false;
+      return true;
+    }
+    case MIRROR: {  // mirror.cpp:172-203
+      v->specular = true;
+      v->wout = -v->win;
+      v->wout = v->wout + (real(2) * v->cos_in_sh) * v->ns;
+      v->cos_out_geo = dot(v->wout, v->ng);
+      if (v->cos_out_geo <= 0) return false;
+      v->s_prev = rcp(v->cos_in_sh);
+      v->s_next = rcp(v->cos_out_geo);
+      real const cig = dot(v->win, v->ng);
+      v->bsdf_x_ipdf = v->cos_out_geo / cig;
+      return true;
+    }
+    case GLASS: {  // glass.cpp:239-323
+      real const ior_sq = glass_ior_sq(bs, v->back);
+      real const nk = dot(v->win, v->ns);
+      real const bb = ior_sq + nk * nk - 1;
+      v->wout = -v->win;
+      v->specular = true;
+      if (bb > 0) {
+        real const g = std::sqrt(bb);
+        real const gpc = g + nk, gmc = g - nk;
+        real const alpha = gmc / gpc;
+        real const beta = (nk * gpc - 1) / (nk * gmc + 1);
+        real const fr = real(0.5) * alpha * alpha * (beta * beta + 1);
+        if (u.u[2] >= fr) {
+          v->s_prev = v->s_next = 1 - fr;
+          v->wout = (v->wout - gmc * v->ns) / std::sqrt(ior_sq);
+          v->cos_out_geo = -dot(v->wout, v->ng);
+          if (v->cos_out_geo <= 0) return false;
+          real const cig = dot(v->win, v->ng);
+          v->bsdf_x_ipdf = v->cos_out_geo / cig;
+          return true;
+        }
+        v->s_prev = v->s_next = fr;
+      } else {
+        v->s_prev = v->s_next = 1;
+      }
+      v->wout = v->wout + (real(2) * nk) * v->ns;
+      v->cos_out_geo = dot(v->wout, v->ng);
+      if (v->cos_out_geo <= 0) return false;
+      real const cig = dot(v->win, v->ng);
+      v->bsdf_x_ipdf = v->cos_out_geo / cig;
+      return true;
+    }
+    case METAL: {  // metal.cpp:227-278
+      v->specular = false;
+      V3 hv;
+      real const nh = glossy_half(u.u[0], u.u[1], bs.shininess, v->t, v->ns, v->b, &hv);
+      real const hk = dot(hv, v->win);
+      v->wout = (2 * hk) * hv - v->win;
+      v->cos_out_geo = dot(v->ng, v->wout);
+      if (v->cos_out_geo <= 0) return false;
+      real const cos_sh = dot(v->ns, v->wout);
+      if (cos_sh <= 0) return false;
+      real const cig = dot(v->ng, v->win);
+      v->bsdf_x_ipdf = fake_fresnel(hk) * v->cos_in_sh / hmax(cos_sh, v->cos_in_sh) *
+                       v->cos_out_geo / cig;
+      v->s_prev = v->s_next = real(double(bs.shininess + 1) * (M_1_PI / 8) *
+                                   double(std::pow(nh, bs.shininess)) / double(hk));
+      v->s_prev *= rcp(v->cos_in_sh);
+      v->s_next *= rcp(v->cos_out_geo);
+      return true;
+    }
+    default:
+      return false;  // light.cpp:44-51
+  }
+}
+
+// Bsdf::CameraToLightsScatteringDirection; u = (dir u1, dir u2, choice)
+bool camera_to_lights(Scene const& sc, Bsdf const& bs, Tri const& tr, Hit const& h, BVert* v,
+                      Draw const& u) {
+  switch (bs.type) {
+    case LAMBERT: {  // lambert.cpp:380-424
+      v->specular = false;
+      v->cos_in_sh = diffused(u.u[0], u.u[1], v->t, v->ns, v->b, &v->win);
+      if (dot(v->win, v->ng) <= 0) return false;
+      real const cog = dot(v->wout, v->ng);
+      int texel;
+      V3 const alb = albedo_at(sc, bs, tr, h, &texel);
+      real const alpha = v->cos_in_sh / cog;
+      real const pp = min1(alb[1]), pn = min1(alb[1] * alpha);
+      v->s_prev = real(M_1_PI * double(pp));
+      v->s_next = real(M_1_PI * double(pn));
+      v->power = v->power * alb;
+      v->bsdf_x_ipdf = rcp(pp);
+      if (u.u[2] >= pp) return false;
+      return true;
+    }
+    case MIRROR: {  // mirror.cpp:217-248
+      v->specular = true;
+      v->cos_in_sh = dot(v->wout, v->ns);
+      v->win = -v->wout;
+      v->win = v->win + (real(2) * v->cos_in_sh) * v->ns;
+      if (dot(v->win, v->ng) > 0) return false;
+      v->s_prev = rcp(v->cos_in_sh);
+      v->s_next = rcp(v->cos_out_geo);
+      v->bsdf_x_ipdf = 1;
+      return true;
+    }
+    case GLASS: {  // glass.cpp:339-405
+      real const ior_sq = glass_ior_sq(bs, v->back);
+      real const nk = dot(v->wout, v->ns);
+      real const bb = ior_sq + nk * nk - 1;
+      v->win = -v->wout;
+      v->specular = true;
+      if (bb > 0) {
+        real const g = std::sqrt(bb);
+        real const gpc = g + nk, gmc = g - nk;
+        real const alpha = gmc / gpc;
+        real const beta = (nk * gpc - 1) / (nk * gmc + 1);
+        real const fr = real(0.5) * alpha * alpha * (beta * beta + 1);
+        if (u.u[2] >= fr) {
+          v->s_prev = v->s_next = 1 - fr;
+          v->win = (v->win - gmc * v->ns) / std::sqrt(ior_sq);
+          v->cos_in_sh = -dot(v->win, v->ng);
+          if (v->cos_in_sh <= 0) return false;
+          v->bsdf_x_ipdf = 1;
+          return true;
+        }
+        v->s_prev = v->s_next = fr;
+      } else {
+        v->s_prev = v->s_next = 1;
+      }
+      v->win = v->win + (real(2) * nk) * v->ns;
+      v->cos_in_sh = dot(v->win, v->ng);
+      if (v->cos_in_sh <= 0) return false;
+      v->bsdf_x_ipdf = 1;
+      return true;
+    }
+    case METAL: {  // metal.cpp:292-340
+      v->specular = false;
+      V3 hv;
+      real const nh = glossy_half(u.u[0], u.u[1], bs.shininess, v->t, v->ns, v->b, &hv);
+      real const hk = dot(hv, v->wout);
+      v->win = (2 * hk) * hv - v->wout;
+      if (dot(v->ng, v->win) <= 0) return false;
+      v->cos_in_sh = dot(v->ns, v->win);
+      if (v->cos_in_sh <= 0) return false;
+      real const cos_sh = dot(v->ns, v->wout);
+      v->bsdf_x_ipdf = fake_fresnel(hk) * v->cos_in_sh / hmax(cos_sh, v->cos_in_sh);
+      v->s_prev = v->s_next = real(double(bs.shininess + 1) * (M_1_PI / 8) *
+                                   double(std::pow(nh, bs.shininess)) / double(hk));
+      v->s_prev *= rcp(v->cos_in_sh);
+      v->s_next *= rcp(v->cos_out_geo);
+      return true;
+    }
+    default:
+      return false;
+  }
+}
+
+// GetConstFactor() = (f / F)^2 / A (thinlenscamera.hpp:83-85); a pinhole given by
+// its field of view uses its lens-film distance V for f.
+real bdpt_const_factor(Camera const& c, float focal_length) {
+  real const f = (focal_length > 0) ? real(focal_length) : c.v;
+  real const r = f / c.film;
+  return (r * r) / c.aspect;
+}
+
+// Pinhole GetFilmPosition (thinlenscamera.cpp:125-157)
+bool film_position(Camera const& c, V3 const& d, real* fx, real* fy) {
+  real const lx = dot(d, c.right), ly = dot(d, c.up), lz = dot(d, c.back);
+  real const k = (c.v / -lz) / c.film;
+  *fx = real(0.5) + lx * k / c.aspect;
+  *fy = real(0.5) + ly * k;
+  return (0 <= *fx) && (*fx < 1) && (0 <= *fy) && (*fy < 1);
+}
+
+struct Splat { uint32_t pixel; V3 value; };
+
+inline real fourth(real x) { real const s = x * x; return s * s; }
+
+// One sample of pixel (px, py): returns the pixel's own colour; light-tracing
+// (t = 1) contributions to other pixels are appended to *splats in s order.
+V3 bdpt_sample(Scene const& sc, Camera const& cam, real C, uint32_t kmax, uint64_t seed,
+               uint32_t W, uint32_t H, uint32_t px, uint32_t py, uint32_t smp,
+               std::vector<Splat>* splats, PathStats* st) {
+  uint32_t const pixel = py * W + px;
+  BVert L[20], E[20];
+  real const A = sc.light_area();
+  auto find = [&](V3 const& o, V3 const& d, Hit* h) {
+    if (st) st->queries++;
+    Counters c;
+    int const r = scene_find(sc, o, d, h, &c);
+    if (st) { st->c.nodes += c.nodes; st->c.tris += c.tris; }
+    return r;
+  };
+  Draw const de = draw(pixel, seed, smp, 0, kBdptStreamEye);
+  real const x = (real(int(px)) + de.u[0]) / real(int(W));
+  real const y = (real(int(py)) + de.u[1]) / real(int(H));
+  Draw const dl = draw(pixel, seed, smp, 0, kBdptStreamLight);
+  // SampleLightPosition (scene.cpp:42-78) + Triangle::Sample's basis (triangle.cpp:232-263)
+  {
+    Vertex lv;
+    int n = -1;
+    sample_light(sc, dl.u[0], dl.u[1], &lv, &n);
+    L[1].tri = lv.tri;
+    if (lv.tri >= 0) {
+      L[1].pos = lv.pos;
+      L[1].ns = lv.ns;
+      L[1].ng = lv.ng;
+      V3 t(-lv.ng[2], 0, lv.ng[0]);
+      real const l2 = length_squared(t);
+      if (l2 > 0) {
+        t = t * rsqrt(l2);
+        L[1].b = cross(t, lv.ng);
+      } else {
+        L[1].b = V3(0, 0, (lv.ng[1] > 0) ? 1 : -1);
+        t = cross(lv.ng, L[1].b);
+      }
+      L[1].t = t;
+      L[1].s_next = rcp(A);
+      L[1].specular = false;
+      L[1].power = sc.bsdfs[sc.tris[size_t(lv.tri)].bsdf].phi;
+    }
+  }
+  L[2].tri = -1;
+  // SampleLensPosition (scene.cpp:80-98): eye, camera basis, normal = -back
+  E[1].tri = -1;
+  E[1].pos = cam.eye;
+  E[1].t = cam.right;
+  E[1].b = cam.up;
+  E[1].ng = E[1].ns = -cam.back;
+  E[1].s_next = 0;
+  E[1].power = V3(1, 1, 1);
+
+  // BuildLightsPath
+  if (L[1].tri >= 0) {
+    L[0].s_next = rcp(A);
+    L[1].cos_in_sh = 1;
+    L[1].cos_out_geo = diffused(dl.u[2], dl.u[3], L[1].t, L[1].ng, L[1].b, &L[1].wout);
+    L[1].s_prev = 1;
+    L[1].s_next = real(M_1_PI);
+    L[1].g = 1;
+    L[1].bsdf_x_ipdf = 1;
+    for (uint32_t s = 1;;) {
+      BVert const& o = L[s];
+      BVert& nw = L[s + 1];
+      Hit h;
+      nw.tri = find(o.pos, o.wout, &h);
+      if (nw.tri < 0) break;
+      Tri const& tr = sc.tris[size_t(nw.tri)];
+      Bsdf const& bs = sc.bsdfs[tr.bsdf];
+      if (bs.type == LIGHT) { nw.tri = -1; break; }
+      geometric_basis(tr, h, &nw);
+      nw.back = h.back;
+      nw.win = -o.wout;
+      nw.cos_in_sh = dot(nw.win, nw.ns);
+      if (nw.cos_in_sh <= 0) { nw.tri = -1; break; }
+      nw.power = o.power * o.bsdf_x_ipdf;
+      nw.pos = o.wout * h.t_max + o.pos;
+      nw.g = o.cos_out_geo * nw.cos_in_sh / (h.t_max * h.t_max);
+      if (++s >= kmax) { L[s + 1].tri = -1; break; }
+      if (!lights_to_camera(sc, bs, tr, h, &L[s], draw(pixel, seed, smp, s, kBdptStreamLight))) {
+        L[s + 1].tri = -1;
+        break;
+      }
+    }
+  }
+
+  // EvaluatePathS1: t = 1, light vertices seen by the camera
+  for (uint32_t s = 2; s < kmax; ++s) {
+    BVert const& lv = L[s];
+    if (lv.tri < 0) break;
+    uint32_t const ty = sc.bsdfs[sc.tris[size_t(lv.tri)].bsdf].type;
+    if (ty == MIRROR || ty == GLASS) continue;
+    V3 const d = normalize(lv.pos - E[1].pos);
+    real const cog = -dot(d, lv.ng);
+    if (cog <= 0) continue;
+    if (-dot(d, lv.ns) <= 0) continue;
+    real const cis = dot(d, E[1].ns);
+    if (cis <= 0) continue;
+    real fx, fy;
+    if (!film_position(cam, lv.pos - E[1].pos, &fx, &fy)) continue;
+    Hit h;
+    if (find(E[1].pos, d, &h) != lv.tri) continue;
+    // fx < 1 can still round fx * W up to W (one past the row in the
+    // reference): keep the last column / row
+    uint32_t const jx = std::min(uint32_t(fx * real(int(W))), W - 1u);
+    uint32_t const jy = std::min(uint32_t(fy * real(int(H))), H - 1u);
+    real const flux = C / fourth(cis);
+    real const G = cis * cog / (h.t_max * h.t_max);
+    real wst = 1;
+    L[s + 1].s_prev = flux;
+    real pst = 1;
+    for (uint32_t i = s; i > 0; --i) {
+      pst *= L[i + 1].s_prev / L[i - 1].s_next;
+      if (!L[i].specular && !L[i - 1].specular) {
+        real const q = pst * G / L[i].g;
+        wst += q * q;
+      }
+    }
+    wst = rcp(wst);
+    double const sc1 = double(flux * G) * M_1_PI * double(wst);
+    V3 const pw = lv.power * E[1].power;
+    if (splats) splats->push_back(Splat{jy * W + jx, pw * real(sc1)});
+  }
+
+  // BuildTheEyePath (pinhole)
+  E[1].win = primary_dir(cam, x, y);
+  E[1].power = V3(1, 1, 1);
+  E[1].cos_in_sh = dot(E[1].win, E[1].ns);
+  E[1].cos_out_geo = 1;
+  E[1].s_prev = C / fourth(E[1].cos_in_sh);
+  E[1].s_next = 1;
+  E[1].g = 1;
+  E[1].bsdf_x_ipdf = 1;
+  for (uint32_t t = 1;;) {
+    BVert const& o = E[t];
+    BVert& nw = E[t + 1];
+    Hit h;
+    nw.tri = find(o.pos, o.win, &h);
+    if (nw.tri < 0) break;
+    Tri const& tr = sc.tris[size_t(nw.tri)];
+    Vertex sv;
+    shading_basis(tr, h, &sv);
+    nw.ns = sv.ns; nw.ng = sv.ng; nw.t = sv.t; nw.b = sv.b;
+    nw.back = h.back;
+    nw.wout = -o.win;
+    if (dot(nw.wout, nw.ns) <= 0) { nw.tri = -1; break; }
+    nw.power = o.power;
+    nw.power = nw.power * o.bsdf_x_ipdf;
+    nw.pos = o.win * h.t_max + o.pos;
+    nw.cos_out_geo = dot(nw.wout, nw.ng);
+    nw.g = nw.cos_out_geo * o.cos_in_sh / (h.t_max * h.t_max);
+    Bsdf const& bs = sc.bsdfs[tr.bsdf];
+    if (++t >= kmax) { E[t + 1].tri = -1; break; }
+    if (bs.type == LIGHT) { E[t + 1].tri = -1; break; }
+    if (!camera_to_lights(sc, bs, tr, h, &E[t], draw(pixel, seed, smp, t, kBdptStreamEye))) {
+      E[t + 1].tri = -1;
+      break;
+    }
+  }
+
+  // EvaluatePathST
+  V3 color(0, 0, 0);
+  for (uint32_t t = 2; t < kmax; ++t) {
+    BVert& ev = E[t];
+    if (ev.tri < 0) break;
+    Bsdf const& eb = sc.bsdfs[sc.tris[size_t(ev.tri)].bsdf];
+    if (eb.type == LIGHT) {
+      real wst = 1;
+      if (t > 2) {
+        E[t + 1].s_next = L[0].s_next;
+        ev.s_next = real(M_1_PI);
+        real pst = 1;
+        for (uint32_t i = t; i > 1; --i) {
+          pst *= E[i + 1].s_next / E[i - 1].s_prev;
+          if (!E[i].specular && !E[i - 1].specular) {
+            real const q = pst / E[i].g;
+            wst += q * q;
+          }
+        }
+        wst = rcp(wst);
+      }
+      double const sc1 = double(rcp(A)) * M_1_PI * double(wst);
+      V3 const pw = eb.phi * ev.power;  // the power of the light that was hit
+      color = color + pw * real(sc1);
+      break;
+    }
+    if (eb.type == MIRROR || eb.type == GLASS) continue;
+    for (uint32_t s = 1; (s + t - 1) < kmax; ++s) {
+      BVert const& lv = L[s];
+      if (lv.tri < 0) break;
+      uint32_t const lt = sc.bsdfs[sc.tris[size_t(lv.tri)].bsdf].type;
+      if (lt == MIRROR || lt == GLASS) continue;
+      V3 const d = normalize(ev.pos - lv.pos);
+      real const cog = dot(d, lv.ng);
+      if (cog <= 0) continue;
+      if (dot(d, lv.ns) <= 0) continue;
+      if (-dot(d, ev.ng) <= 0) continue;
+      real const cis = -dot(d, ev.ns);
+      if (cis <= 0) continue;
+      Hit h;
+      if (find(lv.pos, d, &h) != ev.tri) continue;
+      real const G = cis * cog / (h.t_max * h.t_max);
+      real wst = 1;
+      L[s + 1].s_prev = real(M_1_PI);
+      real pst = 1;
+      for (uint32_t i = s; i > 0; --i) {
+        pst *= L[i + 1].s_prev / L[i - 1].s_next;
+        if (!L[i].specular && !L[i - 1].specular) {
+          real const q = pst * G / L[i].g;
+          wst += q * q;
+        }
+      }
+      E[t + 1].s_next = real(M_1_PI);
+      pst = 1;
+      for (uint32_t i = t; i > 1; --i) {
+        pst *= E[i + 1].s_next / E[i - 1].s_prev;
+        if (!E[i].specular && !E[i - 1].specular) {
+          real const q = pst * G / E[i].g;
+          wst += q * q;
+        }
+      }
+      wst = rcp(wst);
+      double const sc1 = double(G) * (M_1_PI * M_1_PI) * double(wst);
+      V3 const pw = lv.power * ev.power;
+      color = color + pw * real(sc1);
+    }
+  }
+  return color;
+}
+
 int n_threads(int req) {
 #ifdef _OPENMP
   return req > 0 ? req : omp_get_max_threads();
@@ -1120,6 +1614,46 @@ double orc_render(void* s, const float* cam, uint32_t W, uint32_t H, uint32_t sp
       }
     }
     if (stats) { q += ps.queries; nn += ps.c.nodes; ntr += ps.c.tris; }
+  }
+  if (stats) { stats[0] = q; stats[1] = nn; stats[2] = ntr; }
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// Bidirectional path tracing (BidirectionalPathTracer::run,
+// xyz/integrator/integrator.cpp:1025-1081): for each pass (sample index), every
+// pixel's sample adds its own colour to own_film and its light-tracing
+// contributions to splat_film, in the reference's pass / row / column / s
+// order.  Films are W*H*3 sums (row 0 = bottom).  stats: queries, nodes, tris.
+double orc_render_bdpt(void* s, const float* cam, uint32_t W, uint32_t H, uint32_t spp,
+                       uint32_t sample0, uint32_t max_depth, uint64_t seed, float* own_film,
+                       float* splat_film, uint64_t* stats, int threads) {
+  Scene const& sc = *static_cast<Scene*>(s);
+  Camera const c = make_camera(cam, W, H);
+  real const C = bdpt_const_factor(c, cam[10]);
+  uint32_t const kmax = max_depth + 1;
+  if (kmax > 17) return -1.0;  // max_depth <= 16, as the device
+  int const nt = n_threads(threads);
+  std::atomic<uint64_t> q(0), nn(0), ntr(0);
+  std::vector<std::vector<Splat>> rows(H);
+  auto t0 = std::chrono::steady_clock::now();
+  for (uint32_t smp = sample0; smp < sample0 + spp; ++smp) {
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
+    for (int64_t y = 0; y < int64_t(H); ++y) {
+      PathStats ps;
+      rows[size_t(y)].clear();
+      for (uint32_t x = 0; x < W; ++x) {
+        V3 const col = bdpt_sample(sc, c, C, kmax, seed, W, H, x, uint32_t(y), smp,
+                                   &rows[size_t(y)], stats ? &ps : nullptr);
+        float* px = own_film + (size_t(y) * W + x) * 3;
+        px[0] += col[0]; px[1] += col[1]; px[2] += col[2];
+      }
+      if (stats) { q += ps.queries; nn += ps.c.nodes; ntr += ps.c.tris; }
+    }
+    for (uint32_t y = 0; y < H; ++y)
+      for (Splat const& sp : rows[y]) {
+        float* px = splat_film + size_t(sp.pixel) * 3;
+        px[0] += sp.value[0]; px[1] += sp.value[1]; px[2] += sp.value[2];
+      }
   }
   if (stats) { stats[0] = q; stats[1] = nn; stats[2] = ntr; }
   return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

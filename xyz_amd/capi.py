@@ -103,7 +103,7 @@ EXPORTS = ["xyz_kdtree_build", "xyz_kdtree_build_device", "xyz_kdtree_info", "xy
            "xyz_mqo_load", "xyz_mqo_load_file", "xyz_mqo_describe", "xyz_mqo_destroy",
            "xyz_scene_create", "xyz_scene_get_info", "xyz_scene_destroy", "xyz_closest_hit",
            "xyz_closest_hit_work",
-           "xyz_render", "xyz_render_backward", "xyz_render_batch", "xyz_batch_autotune",
+           "xyz_render", "xyz_render_bdpt", "xyz_render_backward", "xyz_render_batch", "xyz_batch_autotune",
            "xyz_last_error", "xyz_version"]
 
 _lib = None
@@ -144,6 +144,8 @@ def lib():
         L.xyz_closest_hit_work.argtypes = [_vp, ctypes.c_uint32, _vp, _vp, _vp, _vp, _vp, _vp]
         L.xyz_render.argtypes = [_vp, ctypes.POINTER(Camera), ctypes.POINTER(RenderParams), _vp,
                                  ctypes.POINTER(RenderStats), _vp]
+        L.xyz_render_bdpt.argtypes = [_vp, ctypes.POINTER(Camera), ctypes.POINTER(RenderParams),
+                                      _vp, _vp, ctypes.POINTER(RenderStats), _vp]
         L.xyz_render_backward.argtypes = [_vp, ctypes.POINTER(Camera),
                                           ctypes.POINTER(RenderParams), _vp, _vp, _vp, _vp]
         L.xyz_render_batch.argtypes = [ctypes.c_uint32, ctypes.POINTER(_vp),
@@ -329,6 +331,18 @@ class Scene:
         st = RenderStats()
         check(lib().xyz_render(self.h, ctypes.byref(cam), ctypes.byref(p), _dptr(film),
                                ctypes.byref(st), _stream(stream)))
+        return dict(paths=st.paths, queries=st.queries, seconds=st.seconds)
+
+    def render_bdpt(self, camera, film, spp, max_depth, seed=0, sample_offset=0, splat=None,
+                    stream=None):
+        """Bidirectional path tracing: adds the per-pixel SUM of `spp` samples' own
+        colours to `film` and their light-tracing contributions to `splat` (or to
+        `film` when splat is None); device float32 (H, W, 3), row 0 = bottom."""
+        cam = make_camera(camera, film.shape[1], film.shape[0])
+        p = make_params(spp, max_depth, 0, 0.2, seed, sample_offset)
+        st = RenderStats()
+        check(lib().xyz_render_bdpt(self.h, ctypes.byref(cam), ctypes.byref(p), _dptr(film),
+                                    _dptr(splat), ctypes.byref(st), _stream(stream)))
         return dict(paths=st.paths, queries=st.queries, seconds=st.seconds)
 
     def render_backward(self, camera, d_film, d_texels, d_emission, spp, max_depth, rr_start=0,

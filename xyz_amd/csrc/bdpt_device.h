@@ -1,0 +1,542 @@
+// Device code of the bidirectional path tracer: one sample of one pixel per
+// call, Scene::EvaluateBidirectionalPathTracing
+// (xyz/core/scene_bidirectionalpathtracing.cpp:8-522) with the bidirectional
+// BSDF methods (LightsToCamera / CameraToLightsScatteringDirection of
+// xyz/bsdf/{lambert,mirror,glass,metal,light}.cpp) and the pinhole camera's
+// film mapping and constant factor (xyz/core/thinlenscamera.hpp:38-90,
+// thinlenscamera.cpp:125-157), in the reference's binary32 operation order.
+//
+// The reference's own behaviour is kept:
+//   * hi::single_stack (libhi/hi/util.hpp:24-36) keeps the value it assigns
+//     (its destructor writes the new value back), so each connection's
+//     temporary density stays in the vertex for the strategies after it;
+//   * Mirror::CameraToLightsScatteringDirection ends the eye path when the
+//     mirrored direction lies above the geometric surface (mirror.cpp:235-239);
+//   * connections and light tracing weigh every non-specular vertex as
+//     Lambertian (the M_1_PI factors);
+//   * the eye path starts at the pinhole (the lens-film distance V of the
+//     camera, also for a thin-lens camera).
+// Vertex fields the reference reads before writing them within a sample (it
+// reuses its vertex arrays across samples) start from zero / false here.
+// Random numbers: Philox streams 2 (light path) and 3 (eye path); draw (pixel,
+// sample, vertex index, stream) gives (direction u1, u2, choice / Russian
+// roulette) at a vertex, vertex 0 of stream 2 the light sample and its
+// emission direction, vertex 0 of stream 3 the pixel jitter.
+#ifndef XYZ_BDPT_DEVICE_H_
+#define XYZ_BDPT_DEVICE_H_
+
+#include "pt_device.h"
+
+namespace xyzhip {
+
+constexpr uint32_t kBdptMaxDepth = 16;  // vertex arrays hold kmax + 2 <= 19 entries
+constexpr uint32_t kBdptStreamLight = 2, kBdptStreamEye = 3;
+
+struct BdptConst {
+  PTConst c;   // kmax, seeds (Russian roulette settings unused)
+  float C;     // GetConstFactor() = (f / F)^2 / A
+  float inv_area;
+};
+
+struct BVert {
+  int32_t tri;
+  bool back, specular;
+  float3 pos, win, wout, ng, ns, t, b, power;
+  float bsdf_x_ipdf, s_prev, s_next, cos_in_sh, cos_out_geo, g;
+};
+
+__device__ __forceinline__ void bvert_clear(BVert& v) {
+  float3 const z = f3(0.f, 0.f, 0.f);
+  v.tri = -1;
+  v.back = v.specular = false;
+  v.pos = v.win = v.wout = v.ng = v.ns = v.t = v.b = v.power = z;
+  v.bsdf_x_ipdf = v.s_prev = v.s_next = v.cos_in_sh = v.cos_out_geo = v.g = 0.0f;
+}
+
+// xyz/stdafx.hpp:50-63
+__device__ __forceinline__ float bd_diffused(float mu_t, float mu_p, float3 t, float3 n, float3 b,
+                                             float3* out) {
+  float const phi = 6.28318530717958647692f * mu_p;
+  float const ct = sqrtf(mu_t);
+  float const st = sqrtf(1.0f - mu_t);
+  float cp, sp;
+  cos_sin(phi, &cp, &sp);
+  *out = add3(add3(scl3(t, st * cp), scl3(b, st * sp)), scl3(n, ct));
+  return ct;
+}
+
+// xyz/stdafx.hpp:154-168
+__device__ __forceinline__ float bd_glossy_half(float mu_t, float mu_p, float shin, float3 t,
+                                                float3 n, float3 b, float3* h) {
+  float const phi = 6.28318530717958647692f * mu_p;
+  float const ct = powf(mu_t, rcpf(shin));
+  float const st = sqrtf(1.0f - ct * ct);
+  float cp, sp;
+  cos_sin(phi, &cp, &sp);
+  *h = add3(add3(scl3(t, st * cp), scl3(b, st * sp)), scl3(n, ct));
+  return ct;
+}
+
+// tangent and binormal about a normal (xyz/geom/triangle.cpp:181-191, 232-263)
+__device__ __forceinline__ void basis_about(float3 n, float3* t, float3* b) {
+  float3 tt = f3(-n.z, 0.0f, n.x);
+  float const l2 = dot3(tt, tt);
+  if (l2 > 0.0f) {
+    tt = scl3(tt, 1.0f / sqrtf(l2));
+    *b = cross3(tt, n);
+  } else {
+    *b = f3(0.0f, 0.0f, (n.y > 0.0f) ? 1.0f : -1.0f);
+    tt = cross3(n, *b);
+  }
+  *t = tt;
+}
+
+// Triangle::GeometricBasis (xyz/geom/triangle.cpp:167-192)
+__device__ __forceinline__ void geometric_basis(const DevScene& s, int32_t prim, float f, float g,
+                                                bool back, BVert& v) {
+  uint32_t bsdf;
+  Basis const B = shading_basis(s, prim, f, g, back, &bsdf);  // ns, ng as ShadingBasis
+  v.ns = B.ns;
+  v.ng = B.ng;
+  basis_about(B.ng, &v.t, &v.b);
+}
+
+__device__ __forceinline__ float min1(float x) { return (x < 1.0f) ? x : 1.0f; }
+__device__ __forceinline__ float hmaxf(float x, float y) { return (x < y) ? y : x; }
+__device__ __forceinline__ float fourth(float x) {
+  float const s = x * x;
+  return s * s;
+}
+
+__device__ __forceinline__ float glass_ior_sq(const DevBsdf& b, bool back) {
+  float const r = back ? rcpf(b.ior) : b.ior;
+  return r * r;
+}
+
+// Bsdf::LightsToCameraScatteringDirection
+__device__ __forceinline__ bool lights_to_camera(const DevScene& s, const DevBsdf& bs, int32_t prim,
+                                                 float hf, float hg, BVert& v, float4 u) {
+  switch (bs.type) {
+    case XYZ_LAMBERT: {  // lambert.cpp:314-365
+      int32_t texel;
+      float3 const alb = albedo_at(s, bs, prim, hf, hg, &texel);
+      float const cig = dot3(v.win, v.ng);
+      float const alpha = v.cos_in_sh / cig;
+      float const pp = min1(alb.y), pn = min1(alb.y * alpha);
+      v.specular = false;
+      v.s_prev = float(kInvPi * double(pp));
+      v.s_next = float(kInvPi * double(pn));
+      v.power = mul3(v.power, alb);
+      v.power = scl3(v.power, alpha);
+      v.bsdf_x_ipdf = rcpf(pn);
+      if (u.z >= pn) return false;
+      v.cos_out_geo = bd_diffused(u.x, u.y, v.t, v.ng, v.b, &v.wout);
+      return dot3(v.wout, v.ns) > 0.0f;
+    }
+    case XYZ_MIRROR: {  // mirror.cpp:172-203
+      v.specular = true;
+      v.wout = neg3(v.win);
+      v.wout = add3(v.wout, scl3l(2.0f * v.cos_in_sh, v.ns));
+      v.cos_out_geo = dot3(v.wout, v.ng);
+      if (v.cos_out_geo <= 0.0f) return false;
+      v.s_prev = rcpf(v.cos_in_sh);
+      v.s_next = rcpf(v.cos_out_geo);
+      float const cig = dot3(v.win, v.ng);
+      v.bsdf_x_ipdf = v.cos_out_geo / cig;
+      return true;
+    }
+    case XYZ_GLASS: {  // glass.cpp:239-323
+      float const ior_sq = glass_ior_sq(bs, v.back);
+      float const nk = dot3(v.win, v.ns);
+      float const bb = ior_sq + nk * nk - 1.0f;
+      v.wout = neg3(v.win);
+      v.specular = true;
+      if (bb > 0.0f) {
+        float const g = sqrtf(bb);
+        float const gpc = g + nk, gmc = g - nk;
+        float const alpha = gmc / gpc;
+        float const beta = (nk * gpc - 1.0f) / (nk * gmc + 1.0f);
+        float const fr = 0.5f * alpha * alpha * (beta * beta + 1.0f);
+        if (u.z >= fr) {
+          v.s_prev = v.s_next = 1.0f - fr;
+          float const q = sqrtf(ior_sq);
+          float3 const w = sub3(v.wout, scl3l(gmc, v.ns));
+          v.wout = f3(w.x / q, w.y / q, w.z / q);
+          v.cos_out_geo = -dot3(v.wout, v.ng);
+          if (v.cos_out_geo <= 0.0f) return false;
+          float const cig = dot3(v.win, v.ng);
+          v.bsdf_x_ipdf = v.cos_out_geo / cig;
+          return true;
+        }
+        v.s_prev = v.s_next = fr;
+      } else {
+        v.s_prev = v.s_next = 1.0f;
+      }
+      v.wout = add3(v.wout, scl3l(2.0f * nk, v.ns));
+      v.cos_out_geo = dot3(v.wout, v.ng);
+      if (v.cos_out_geo <= 0.0f) return false;
+      float const cig = dot3(v.win, v.ng);
+      v.bsdf_x_ipdf = v.cos_out_geo / cig;
+      return true;
+    }
+    case XYZ_METAL: {  // metal.cpp:227-278
+      v.specular = false;
+      float3 hv;
+      float const nh = bd_glossy_half(u.x, u.y, bs.shininess, v.t, v.ns, v.b, &hv);
+      float const hk = dot3(hv, v.win);
+      v.wout = sub3(scl3l(2.0f * hk, hv), v.win);
+      v.cos_out_geo = dot3(v.ng, v.wout);
+      if (v.cos_out_geo <= 0.0f) return false;
+      float const cos_sh = dot3(v.ns, v.wout);
+      if (cos_sh <= 0.0f) return false;
+      float const cig = dot3(v.ng, v.win);
+      v.bsdf_x_ipdf = fake_fresnel(hk) * v.cos_in_sh / hmaxf(cos_sh, v.cos_in_sh) *
+                      v.cos_out_geo / cig;
+      v.s_prev = v.s_next = float(double(bs.shininess + 1.0f) * (kInvPi / 8) *
+                                  double(powf(nh, bs.shininess)) / double(hk));
+      v.s_prev *= rcpf(v.cos_in_sh);
+      v.s_next *= rcpf(v.cos_out_geo);
+      return true;
+    }
+    default:
+      return false;  // light.cpp:44-51
+  }
+}
+
+// Bsdf::CameraToLightsScatteringDirection
+__device__ __forceinline__ bool camera_to_lights(const DevScene& s, const DevBsdf& bs, int32_t prim,
+                                                 float hf, float hg, BVert& v, float4 u) {
+  switch (bs.type) {
+    case XYZ_LAMBERT: {  // lambert.cpp:380-424
+      v.specular = false;
+      v.cos_in_sh = bd_diffused(u.x, u.y, v.t, v.ns, v.b, &v.win);
+      if (dot3(v.win, v.ng) <= 0.0f) return false;
+      float const cog = dot3(v.wout, v.ng);
+      int32_t texel;
+      float3 const alb = albedo_at(s, bs, prim, hf, hg, &texel);
+      float const alpha = v.cos_in_sh / cog;
+      float const pp = min1(alb.y), pn = min1(alb.y * alpha);
+      v.s_prev = float(kInvPi * double(pp));
+      v.s_next = float(kInvPi * double(pn));
+      v.power = mul3(v.power, alb);
+      v.bsdf_x_ipdf = rcpf(pp);
+      return !(u.z >= pp);
+    }
+    case XYZ_MIRROR: {  // mirror.cpp:217-248
+      v.specular = true;
+      v.cos_in_sh = dot3(v.wout, v.ns);
+      v.win = neg3(v.wout);
+      v.win = add3(v.win, scl3l(2.0f * v.cos_in_sh, v.ns));
+      if (dot3(v.win, v.ng) > 0.0f) return false;
+      v.s_prev = rcpf(v.cos_in_sh);
+      v.s_next = rcpf(v.cos_out_geo);
+      v.bsdf_x_ipdf = 1.0f;
+      return true;
+    }
+    case XYZ_GLASS: {  // glass.cpp:339-405
+      float const ior_sq = glass_ior_sq(bs, v.back);
+      float const nk = dot3(v.wout, v.ns);
+      float const bb = ior_sq + nk * nk - 1.0f;
+      v.win = neg3(v.wout);
+      v.specular = true;
+      if (bb > 0.0f) {
+        float const g = sqrtf(bb);
+        float const gpc = g + nk, gmc = g - nk;
+        float const alpha = gmc / gpc;
+        float const beta = (nk * gpc - 1.0f) / (nk * gmc + 1.0f);
+        float const fr = 0.5f * alpha * alpha * (beta * beta + 1.0f);
+        if (u.z >= fr) {
+          v.s_prev = v.s_next = 1.0f - fr;
+          float const q = sqrtf(ior_sq);
+          float3 const w = sub3(v.win, scl3l(gmc, v.ns));
+          v.win = f3(w.x / q, w.y / q, w.z / q);
+          v.cos_in_sh = -dot3(v.win, v.ng);
+          if (v.cos_in_sh <= 0.0f) return false;
+          v.bsdf_x_ipdf = 1.0f;
+          return true;
+        }
+        v.s_prev = v.s_next = fr;
+      } else {
+        v.s_prev = v.s_next = 1.0f;
+      }
+      v.win = add3(v.win, scl3l(2.0f * nk, v.ns));
+      v.cos_in_sh = dot3(v.win, v.ng);
+      if (v.cos_in_sh <= 0.0f) return false;
+      v.bsdf_x_ipdf = 1.0f;
+      return true;
+    }
+    case XYZ_METAL: {  // metal.cpp:292-340
+      v.specular = false;
+      float3 hv;
+      float const nh = bd_glossy_half(u.x, u.y, bs.shininess, v.t, v.ns, v.b, &hv);
+      float const hk = dot3(hv, v.wout);
+      v.win = sub3(scl3l(2.0f * hk, hv), v.wout);
+      if (dot3(v.ng, v.win) <= 0.0f) return false;
+      v.cos_in_sh = dot3(v.ns, v.win);
+      if (v.cos_in_sh <= 0.0f) return false;
+      float const cos_sh = dot3(v.ns, v.wout);
+      v.bsdf_x_ipdf = fake_fresnel(hk) * v.cos_in_sh / hmaxf(cos_sh, v.cos_in_sh);
+      v.s_prev = v.s_next = float(double(bs.shininess + 1.0f) * (kInvPi / 8) *
+                                  double(powf(nh, bs.shininess)) / double(hk));
+      v.s_prev *= rcpf(v.cos_in_sh);
+      v.s_next *= rcpf(v.cos_out_geo);
+      return true;
+    }
+    default:
+      return false;
+  }
+}
+
+// Pinhole GetFilmPosition (thinlenscamera.cpp:125-157)
+__device__ __forceinline__ bool film_position(const DevCamera& cam, float3 d, float* fx, float* fy) {
+  float const lx = dot3(d, f3(cam.right[0], cam.right[1], cam.right[2]));
+  float const ly = dot3(d, f3(cam.up[0], cam.up[1], cam.up[2]));
+  float const lz = dot3(d, f3(cam.back[0], cam.back[1], cam.back[2]));
+  float const k = (cam.v / -lz) / cam.film;
+  *fx = 0.5f + lx * k / cam.aspect;
+  *fy = 0.5f + ly * k;
+  return (0.0f <= *fx) && (*fx < 1.0f) && (0.0f <= *fy) && (*fy < 1.0f);
+}
+
+// The strategies' weights: one vertex array side of the MIS sum
+// (scene_bidirectionalpathtracing.cpp:174-193, 460-507).  `G` is the
+// connection's geometric factor (1 for s = 0, where it does not appear).
+__device__ __forceinline__ float mis_light_side(const BVert* L, uint32_t s, float G, float wst) {
+  float pst = 1.0f;
+  for (uint32_t i = s; i > 0; --i) {
+    pst *= L[i + 1].s_prev / L[i - 1].s_next;
+    if (!L[i].specular && !L[i - 1].specular) {
+      float const q = pst * G / L[i].g;
+      wst += q * q;
+    }
+  }
+  return wst;
+}
+__device__ __forceinline__ float mis_eye_side(const BVert* E, uint32_t t, float G, float wst) {
+  float pst = 1.0f;
+  for (uint32_t i = t; i > 1; --i) {
+    pst *= E[i + 1].s_next / E[i - 1].s_prev;
+    if (!E[i].specular && !E[i - 1].specular) {
+      float const q = pst * G / E[i].g;
+      wst += q * q;
+    }
+  }
+  return wst;
+}
+
+// One BDPT sample of pixel (px, py): returns the pixel's own colour and adds
+// the light-tracing (t = 1) contributions to `splat` (atomically).
+template <bool kCount>
+__device__ __forceinline__ float3 bdpt_sample(const DevScene& s, const DevCamera& cam,
+                                              const BdptConst& bc, uint32_t px, uint32_t py,
+                                              uint32_t smp, uint32_t* stk, uint32_t stride,
+                                              float* splat, uint32_t* queries) {
+  uint32_t const W = cam.width, H = cam.height;
+  uint32_t const pixel = py * W + px;
+  uint32_t const kmax = bc.c.kmax;
+  BVert L[kBdptMaxDepth + 3], E[kBdptMaxDepth + 3];
+  for (uint32_t i = 0; i < kmax + 2; ++i) {
+    bvert_clear(L[i]);
+    bvert_clear(E[i]);
+  }
+  auto find = [&](float3 o, float3 d) {
+    if (kCount) ++*queries;
+    return trace_closest<false>(s, o, d, stk, stride, nullptr, nullptr);
+  };
+  float4 const de = draw4(bc.c, pixel, smp, 0u, kBdptStreamEye);
+  float4 const dl = draw4(bc.c, pixel, smp, 0u, kBdptStreamLight);
+  // SampleLightPosition and Triangle::Sample's basis
+  {
+    LightVertex const lv = sample_light(s, dl.x, dl.y);
+    L[1].tri = lv.prim;
+    if (lv.prim >= 0) {
+      L[1].pos = lv.pos;
+      L[1].ns = lv.ns;
+      L[1].ng = lv.ng;
+      basis_about(lv.ng, &L[1].t, &L[1].b);
+      L[1].s_next = bc.inv_area;
+      L[1].specular = false;
+      L[1].power = lv.power;
+    }
+  }
+  L[2].tri = -1;
+  // SampleLensPosition: the pinhole, normal = -back
+  float3 const eye = f3(cam.eye[0], cam.eye[1], cam.eye[2]);
+  float3 const fwd = neg3(f3(cam.back[0], cam.back[1], cam.back[2]));
+  E[1].pos = eye;
+  E[1].t = f3(cam.right[0], cam.right[1], cam.right[2]);
+  E[1].b = f3(cam.up[0], cam.up[1], cam.up[2]);
+  E[1].ng = E[1].ns = fwd;
+  E[1].power = f3(1.f, 1.f, 1.f);
+
+  // BuildLightsPath
+  if (L[1].tri >= 0) {
+    L[0].s_next = bc.inv_area;
+    L[1].cos_in_sh = 1.0f;
+    L[1].cos_out_geo = bd_diffused(dl.z, dl.w, L[1].t, L[1].ng, L[1].b, &L[1].wout);
+    L[1].s_prev = 1.0f;
+    L[1].s_next = float(kInvPi);
+    L[1].g = 1.0f;
+    L[1].bsdf_x_ipdf = 1.0f;
+    for (uint32_t k = 1;;) {
+      BVert& nw = L[k + 1];
+      HitRec const h = find(L[k].pos, L[k].wout);
+      nw.tri = h.prim;
+      if (h.prim < 0) break;
+      uint32_t bsdf;
+      {
+        float4 const ngb = s.prim_ng[h.prim];
+        bsdf = __float_as_uint(ngb.w);
+      }
+      DevBsdf const bs = s.bsdfs[bsdf];
+      if (bs.type == XYZ_LIGHT) {
+        nw.tri = -1;
+        break;
+      }
+      geometric_basis(s, h.prim, h.f, h.g, h.back, nw);
+      nw.back = h.back;
+      nw.win = neg3(L[k].wout);
+      nw.cos_in_sh = dot3(nw.win, nw.ns);
+      if (nw.cos_in_sh <= 0.0f) {
+        nw.tri = -1;
+        break;
+      }
+      nw.power = scl3(L[k].power, L[k].bsdf_x_ipdf);
+      nw.pos = add3(scl3(L[k].wout, h.t), L[k].pos);
+      nw.g = L[k].cos_out_geo * nw.cos_in_sh / (h.t * h.t);
+      if (++k >= kmax) {
+        L[k + 1].tri = -1;
+        break;
+      }
+      if (!lights_to_camera(s, bs, h.prim, h.f, h.g, L[k], draw4(bc.c, pixel, smp, k, kBdptStreamLight))) {
+        L[k + 1].tri = -1;
+        break;
+      }
+    }
+  }
+
+  // EvaluatePathS1 (t = 1)
+  for (uint32_t k = 2; k < kmax; ++k) {
+    BVert const& lv = L[k];
+    if (lv.tri < 0) break;
+    uint32_t const ty = s.bsdfs[__float_as_uint(s.prim_ng[lv.tri].w)].type;
+    if (ty == XYZ_MIRROR || ty == XYZ_GLASS) continue;
+    float3 const d = normalize3(sub3(lv.pos, E[1].pos));
+    float const cog = -dot3(d, lv.ng);
+    if (cog <= 0.0f) continue;
+    if (-dot3(d, lv.ns) <= 0.0f) continue;
+    float const cis = dot3(d, E[1].ns);
+    if (cis <= 0.0f) continue;
+    float fx, fy;
+    if (!film_position(cam, sub3(lv.pos, E[1].pos), &fx, &fy)) continue;
+    HitRec const h = find(E[1].pos, d);
+    if (h.prim != lv.tri) continue;
+    // fx < 1 can still round fx * W up to W: keep the last column / row
+    uint32_t const jx = min(uint32_t(fx * float(int(W))), W - 1u);
+    uint32_t const jy = min(uint32_t(fy * float(int(H))), H - 1u);
+    float const flux = bc.C / fourth(cis);
+    float const G = cis * cog / (h.t * h.t);
+    L[k + 1].s_prev = flux;
+    float const wst = rcpf(mis_light_side(L, k, G, 1.0f));
+    double const sc1 = double(flux * G) * kInvPi * double(wst);
+    float3 const pw = mul3(lv.power, E[1].power);
+    float const w = float(sc1);
+    float* dst = splat + 3 * size_t(jy * W + jx);
+    atomicAdd(dst + 0, pw.x * w);
+    atomicAdd(dst + 1, pw.y * w);
+    atomicAdd(dst + 2, pw.z * w);
+  }
+
+  // BuildTheEyePath
+  E[1].win = primary_dir(cam, px, py, de.x, de.y);
+  E[1].cos_in_sh = dot3(E[1].win, E[1].ns);
+  E[1].cos_out_geo = 1.0f;
+  E[1].s_prev = bc.C / fourth(E[1].cos_in_sh);
+  E[1].s_next = 1.0f;
+  E[1].g = 1.0f;
+  E[1].bsdf_x_ipdf = 1.0f;
+  for (uint32_t k = 1;;) {
+    BVert& nw = E[k + 1];
+    HitRec const h = find(E[k].pos, E[k].win);
+    nw.tri = h.prim;
+    if (h.prim < 0) break;
+    uint32_t bsdf;
+    Basis const B = shading_basis(s, h.prim, h.f, h.g, h.back, &bsdf);
+    nw.ns = B.ns;
+    nw.ng = B.ng;
+    nw.t = B.t;
+    nw.b = B.b;
+    nw.back = h.back;
+    nw.wout = neg3(E[k].win);
+    if (dot3(nw.wout, nw.ns) <= 0.0f) {
+      nw.tri = -1;
+      break;
+    }
+    nw.power = scl3(E[k].power, E[k].bsdf_x_ipdf);
+    nw.pos = add3(scl3(E[k].win, h.t), E[k].pos);
+    nw.cos_out_geo = dot3(nw.wout, nw.ng);
+    nw.g = nw.cos_out_geo * E[k].cos_in_sh / (h.t * h.t);
+    DevBsdf const bs = s.bsdfs[bsdf];
+    if (++k >= kmax) {
+      E[k + 1].tri = -1;
+      break;
+    }
+    if (bs.type == XYZ_LIGHT) {
+      E[k + 1].tri = -1;
+      break;
+    }
+    if (!camera_to_lights(s, bs, h.prim, h.f, h.g, E[k], draw4(bc.c, pixel, smp, k, kBdptStreamEye))) {
+      E[k + 1].tri = -1;
+      break;
+    }
+  }
+
+  // EvaluatePathST
+  float3 color = f3(0.f, 0.f, 0.f);
+  for (uint32_t t = 2; t < kmax; ++t) {
+    BVert& ev = E[t];
+    if (ev.tri < 0) break;
+    DevBsdf const eb = s.bsdfs[__float_as_uint(s.prim_ng[ev.tri].w)];
+    if (eb.type == XYZ_LIGHT) {
+      float wst = 1.0f;
+      if (t > 2) {
+        E[t + 1].s_next = L[0].s_next;
+        ev.s_next = float(kInvPi);
+        wst = rcpf(mis_eye_side(E, t, 1.0f, 1.0f));
+      }
+      double const sc1 = double(bc.inv_area) * kInvPi * double(wst);
+      float3 const pw = mul3(f3(eb.phi[0], eb.phi[1], eb.phi[2]), ev.power);
+      color = add3(color, scl3(pw, float(sc1)));
+      break;
+    }
+    if (eb.type == XYZ_MIRROR || eb.type == XYZ_GLASS) continue;
+    for (uint32_t k = 1; (k + t - 1) < kmax; ++k) {
+      BVert const& lv = L[k];
+      if (lv.tri < 0) break;
+      uint32_t const lt = s.bsdfs[__float_as_uint(s.prim_ng[lv.tri].w)].type;
+      if (lt == XYZ_MIRROR || lt == XYZ_GLASS) continue;
+      float3 const d = normalize3(sub3(ev.pos, lv.pos));
+      float const cog = dot3(d, lv.ng);
+      if (cog <= 0.0f) continue;
+      if (dot3(d, lv.ns) <= 0.0f) continue;
+      if (-dot3(d, ev.ng) <= 0.0f) continue;
+      float const cis = -dot3(d, ev.ns);
+      if (cis <= 0.0f) continue;
+      HitRec const h = find(lv.pos, d);
+      if (h.prim != ev.tri) continue;
+      float const G = cis * cog / (h.t * h.t);
+      L[k + 1].s_prev = float(kInvPi);
+      float wst = mis_light_side(L, k, G, 1.0f);
+      E[t + 1].s_next = float(kInvPi);
+      wst = rcpf(mis_eye_side(E, t, G, wst));
+      double const sc1 = double(G) * (kInvPi * kInvPi) * double(wst);
+      float3 const pw = mul3(lv.power, ev.power);
+      color = add3(color, scl3(pw, float(sc1)));
+    }
+  }
+  return color;
+}
+
+}  // namespace xyzhip
+
+#endif  // XYZ_BDPT_DEVICE_H_

@@ -23,6 +23,7 @@
 #include <string>
 #include <vector>
 
+#include "bdpt_device.h"
 #include "pt_device.h"
 #include "scene_internal.h"
 
@@ -1608,6 +1609,131 @@ int autotune_impl(uint32_t n, xyz_scene* const* scenes, const xyz_camera* cams,
   return zero_films(n, cams, films, st);
 }
 
+// ================================================================ BDPT
+// One thread per pixel, its samples in sample order (the pixel's own colour is
+// added to the film as BidirectionalPathTracer::run does, integrator.cpp:1059-1081);
+// light-tracing contributions land on other pixels through float atomics.
+template <bool kCount>
+__global__ void __launch_bounds__(kBlock) k_bdpt(DevScene s, DevCamera cam, BdptConst bc,
+                                                 float* film, float* splat, uint32_t sample0,
+                                                 uint32_t spp, unsigned long long* queries) {
+  extern __shared__ uint32_t lds_stack[];
+  uint32_t* const stk = lds_stack + threadIdx.x;
+  uint32_t const n = cam.width * cam.height;
+  uint32_t q = 0;
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+    uint32_t const px = p % cam.width, py = p / cam.width;
+    float* const dst = film + 3 * size_t(p);
+    float3 acc = f3(dst[0], dst[1], dst[2]);
+    for (uint32_t smp = sample0; smp < sample0 + spp; ++smp) {
+      float3 const c = bdpt_sample<kCount>(s, cam, bc, px, py, smp, stk, blockDim.x, splat, &q);
+      acc = add3(acc, c);
+    }
+    dst[0] = acc.x;
+    dst[1] = acc.y;
+    dst[2] = acc.z;
+  }
+  if (kCount && q) atomicAdd(queries, static_cast<unsigned long long>(q));
+}
+
+__global__ void __launch_bounds__(kBlock) k_add_into(float* dst, const float* src, size_t n) {
+  size_t const i = size_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i < n) dst[i] = dst[i] + src[i];
+}
+
+__global__ void k_zero_u64(unsigned long long* p) {
+  if (threadIdx.x == 0) *p = 0ull;
+}
+
+int render_bdpt_impl(xyz_scene* sc, const xyz_camera* cam, const xyz_render_params* prm,
+                     float* film, float* splat, xyz_render_stats* stats, hipStream_t st) {
+  if (!sc || !film) {
+    set_error("render_bdpt: NULL scene or film");
+    return XYZ_E_INVALID;
+  }
+  DevCamera dc;
+  int rc = make_dev_camera(cam, &dc);
+  if (rc != XYZ_OK) return rc;
+  BdptConst bc;
+  rc = make_pt_const(prm, &bc.c);
+  if (rc != XYZ_OK) return rc;
+  if (prm->max_depth > kBdptMaxDepth) {
+    set_error("render_bdpt: max_depth must be <= 16");
+    return XYZ_E_UNSUPPORTED;
+  }
+  uint64_t const wh = uint64_t(cam->width) * cam->height;
+  if (wh >= (1ull << 31)) {
+    set_error("render_bdpt: image too large");
+    return XYZ_E_UNSUPPORTED;
+  }
+  if (uint64_t(prm->sample_offset) + prm->spp >= (1ull << 32)) {
+    set_error("render_bdpt: sample index overflow");
+    return XYZ_E_UNSUPPORTED;
+  }
+  // GetConstFactor() = (f / F)^2 / A (thinlenscamera.hpp:83-85); a pinhole given
+  // by its field of view takes its lens-film distance V as f
+  float const f = (cam->focal_length > 0.0f) ? cam->focal_length : dc.v;
+  float const r = f / dc.film;
+  bc.C = (r * r) / dc.aspect;
+  bc.inv_area = sc->dev.n_lights ? 1.0f / sc->dev.light_area : 0.0f;
+  if (prm->spp == 0) return XYZ_OK;
+  DevMem own_splat, qbuf;
+  own_splat.st = qbuf.st = st;
+  float* sp = splat;
+  hipError_t e = hipSuccess;
+  if (!sp) {
+    e = pool_alloc(&own_splat.p, wh * 3 * sizeof(float), st);
+    if (e != hipSuccess) return hip_fail(e, "render_bdpt: splat buffer");
+    sp = static_cast<float*>(own_splat.p);
+    hipLaunchKernelGGL(k_fill_zero, dim3(uint32_t((wh * 3 + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       st, sp, size_t(wh * 3));
+  }
+  unsigned long long* qd = nullptr;
+  if (stats) {
+    e = pool_alloc(&qbuf.p, sizeof(unsigned long long), st);
+    if (e != hipSuccess) return hip_fail(e, "render_bdpt: counter");
+    qd = static_cast<unsigned long long*>(qbuf.p);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, qd);
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (stats) {
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    (void)hipEventRecord(e0, st);
+  }
+  uint32_t const grid = uint32_t((wh + kBlock - 1) / kBlock);
+  size_t const lds = size_t(sc->dev.stack_cap) * kBlock * sizeof(uint32_t);
+  if (stats) {
+    hipLaunchKernelGGL(k_bdpt<true>, dim3(grid), dim3(kBlock), lds, st, sc->dev, dc, bc, film, sp,
+                       prm->sample_offset, prm->spp, qd);
+  } else {
+    hipLaunchKernelGGL(k_bdpt<false>, dim3(grid), dim3(kBlock), lds, st, sc->dev, dc, bc, film, sp,
+                       prm->sample_offset, prm->spp, qd);
+  }
+  if (!splat) {
+    hipLaunchKernelGGL(k_add_into, dim3(uint32_t((wh * 3 + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       st, film, sp, size_t(wh * 3));
+  }
+  e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "render_bdpt launch");
+  if (stats) {
+    (void)hipEventRecord(e1, st);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    unsigned long long q = 0;
+    e = hipMemcpyAsync(&q, qd, sizeof(q), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "render_bdpt stats");
+    stats->seconds = double(ms) * 1e-3;
+    stats->paths = wh * prm->spp;
+    stats->queries = q;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  return XYZ_OK;
+}
+
 }  // namespace
 
 }  // namespace xyzhip
@@ -1620,6 +1746,12 @@ int xyz_render(xyz_scene* scene, const xyz_camera* camera, const xyz_render_para
                float* film, xyz_render_stats* stats, void* stream) {
   return render_impl(scene, camera, params, film, stats, static_cast<hipStream_t>(stream), false,
                      nullptr, nullptr, nullptr);
+}
+
+int xyz_render_bdpt(xyz_scene* scene, const xyz_camera* camera, const xyz_render_params* params,
+                    float* film, float* splat_film, xyz_render_stats* stats, void* stream) {
+  return render_bdpt_impl(scene, camera, params, film, splat_film, stats,
+                          static_cast<hipStream_t>(stream));
 }
 
 int xyz_render_backward(xyz_scene* scene, const xyz_camera* camera, const xyz_render_params* params,
