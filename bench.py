@@ -39,6 +39,8 @@ def parse():
                     help="skip the device kd-tree build timing")
     ap.add_argument("--skip-mqo", action="store_true",
                     help="skip the Metasequoia scene load timing")
+    ap.add_argument("--skip-bdpt", action="store_true",
+                    help="skip the bidirectional path-tracing leg")
     ap.add_argument("--pt-spp", type=int, default=16,
                     help="samples per pixel timed in the 1080p leg (the workload renders 256)")
     ap.add_argument("--pt-cpu-rows", type=int, default=128,
@@ -198,6 +200,7 @@ def main():
 
     kdb = None if args.skip_kd_build else bench_kd_build(capi, mesh, info)
     mqo_leg = None if args.skip_mqo else bench_mqo(capi, scenes)
+    bdpt = None if args.skip_bdpt else bench_bdpt(args, hip, capi, scenes)
     pt = None if args.skip_pt else bench_pt(args, hip, capi, scenes)
     adj = None if args.skip_adjoint else bench_adjoint(args, hip, capi, scenes)
     bat = None if args.skip_batch else bench_batch(args, hip, capi, scenes)
@@ -231,6 +234,7 @@ def main():
         "batched": bat,
         "kd_build": kdb,
         "mqo_load": mqo_leg,
+        "bdpt_cornell": bdpt,
     }
     print(json.dumps(out), flush=True)
 
@@ -302,6 +306,43 @@ def bench_mqo(capi, scenes):
         out[f"kd_build_seconds_{name}"] = round(sc.info()["build_seconds"], 4)
         del sc
     log(f"[bench] mqo load: {out}")
+    return out
+
+
+def bench_bdpt(args, hip, capi, scenes):
+    """Bidirectional path tracing of the Cornell box at 256x256, depth 8, 64 spp
+    (xyz_render_bdpt) vs the oracle's BDPT on the host cores (16 spp) and on one
+    thread (2 spp); the own films of the CPU sample are checked bit-identical."""
+    sc = scenes.cornell_box()
+    g = capi.Scene.from_dict(sc)
+    W = H = 256
+    D, spp = 8, 64
+    warm = hip.DeviceArray.zeros((H, W, 3), np.float32)
+    g.render_bdpt(sc["camera"], warm, 1, D, seed=1)
+    film = hip.DeviceArray.zeros((H, W, 3), np.float32)
+    st = g.render_bdpt(sc["camera"], film, spp, D, seed=1)
+    ms = st["seconds"] * 1e3
+    gpu = W * H * spp / st["seconds"] / 1e6
+    out = dict(msamples_per_s=round(gpu, 2), ms=round(ms, 2), spp=spp, width=W, height=H,
+               max_depth=D, queries_per_sample=round(st["queries"] / (W * H * spp), 2))
+    if not args.no_cpu_baseline:
+        from oracle import oracle as orc
+        o = orc.OracleScene(sc["v0"], sc["v1"], sc["v2"], tri_bsdf=sc["tri_bsdf"], bsdfs=sc["bsdfs"])
+        threads = args.cpu_threads or orc.max_threads()
+        own, spl, cst = o.render_bdpt(sc["camera"], W, H, 16, D, seed=1, threads=threads, stats=True)
+        gown = hip.DeviceArray.zeros((H, W, 3), np.float32)
+        gspl = hip.DeviceArray.zeros((H, W, 3), np.float32)
+        g.render_bdpt(sc["camera"], gown, 16, D, seed=1, splat=gspl)
+        same = bool(np.array_equal(gown.numpy().view(np.uint32), own.view(np.uint32)))
+        _, _, c1 = o.render_bdpt(sc["camera"], W, H, 2, D, seed=1, threads=1, stats=True)
+        cpu = W * H * 16 / cst["seconds"] / 1e6
+        out["cpu_baseline"] = dict(value=round(cpu, 3), unit="Msamples/s", cores=threads,
+                                   kind="port", sample=f"16 spp of the same frame; own film "
+                                   f"identical to the GPU's: {same}",
+                                   single_thread=dict(value=round(W * H * 2 / c1["seconds"] / 1e6, 4),
+                                                      cores=1, sample="2 spp"))
+        out["speedup_vs_cpu"] = round(gpu / cpu, 1)
+    log(f"[bench] bdpt: {out}")
     return out
 
 

@@ -1610,30 +1610,46 @@ int autotune_impl(uint32_t n, xyz_scene* const* scenes, const xyz_camera* cams,
 }
 
 // ================================================================ BDPT
-// One thread per pixel, its samples in sample order (the pixel's own colour is
-// added to the film as BidirectionalPathTracer::run does, integrator.cpp:1059-1081);
+// Work items are (sample, pixel) pairs, pixel-major within a sample so that
+// neighbouring threads trace neighbouring primary rays.  A chunk of samples
+// writes each item's own colour to a buffer, then one thread per pixel adds
+// them to the film in sample order (the order of BidirectionalPathTracer::run,
+// integrator.cpp:1059-1081), so own colours are summed deterministically;
 // light-tracing contributions land on other pixels through float atomics.
 template <bool kCount>
 __global__ void __launch_bounds__(kBlock) k_bdpt(DevScene s, DevCamera cam, BdptConst bc,
-                                                 float* film, float* splat, uint32_t sample0,
-                                                 uint32_t spp, unsigned long long* queries) {
+                                                 float4* colours, float* splat, uint32_t n,
+                                                 uint32_t sample0, uint64_t items,
+                                                 unsigned long long* queries) {
   extern __shared__ uint32_t lds_stack[];
   uint32_t* const stk = lds_stack + threadIdx.x;
-  uint32_t const n = cam.width * cam.height;
   uint32_t q = 0;
-  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < items;
+       i += uint64_t(gridDim.x) * blockDim.x) {
+    uint32_t const smp = sample0 + uint32_t(i / n);
+    uint32_t const p = uint32_t(i % n);
     uint32_t const px = p % cam.width, py = p / cam.width;
-    float* const dst = film + 3 * size_t(p);
-    float3 acc = f3(dst[0], dst[1], dst[2]);
-    for (uint32_t smp = sample0; smp < sample0 + spp; ++smp) {
-      float3 const c = bdpt_sample<kCount>(s, cam, bc, px, py, smp, stk, blockDim.x, splat, &q);
-      acc = add3(acc, c);
-    }
-    dst[0] = acc.x;
-    dst[1] = acc.y;
-    dst[2] = acc.z;
+    float3 const c = bdpt_sample<kCount>(s, cam, bc, px, py, smp, stk, blockDim.x, splat, &q);
+    colours[i] = make_float4(c.x, c.y, c.z, 0.0f);
   }
   if (kCount && q) atomicAdd(queries, static_cast<unsigned long long>(q));
+}
+
+__global__ void __launch_bounds__(kBlock) k_bdpt_sum(float* film, const float4* colours,
+                                                     uint32_t n, uint32_t count) {
+  uint32_t const p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= n) return;
+  float* const dst = film + 3 * size_t(p);
+  float x = dst[0], y = dst[1], z = dst[2];
+  for (uint32_t k = 0; k < count; ++k) {
+    float4 const c = colours[size_t(k) * n + p];
+    x = x + c.x;
+    y = y + c.y;
+    z = z + c.z;
+  }
+  dst[0] = x;
+  dst[1] = y;
+  dst[2] = z;
 }
 
 __global__ void __launch_bounds__(kBlock) k_add_into(float* dst, const float* src, size_t n) {
@@ -1701,14 +1717,28 @@ int render_bdpt_impl(xyz_scene* sc, const xyz_camera* cam, const xyz_render_para
     (void)hipEventCreate(&e1);
     (void)hipEventRecord(e0, st);
   }
-  uint32_t const grid = uint32_t((wh + kBlock - 1) / kBlock);
+  // samples in chunks of at most 2^24 (sample, pixel) items (256 MiB of colours)
+  uint32_t const n = uint32_t(wh);
+  uint32_t const per_chunk = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(prm->spp, (1ull << 24) / wh)));
+  DevMem colbuf;
+  colbuf.st = st;
+  e = pool_alloc(&colbuf.p, size_t(per_chunk) * wh * sizeof(float4), st);
+  if (e != hipSuccess) return hip_fail(e, "render_bdpt: colour buffer");
+  float4* const cols = static_cast<float4*>(colbuf.p);
   size_t const lds = size_t(sc->dev.stack_cap) * kBlock * sizeof(uint32_t);
-  if (stats) {
-    hipLaunchKernelGGL(k_bdpt<true>, dim3(grid), dim3(kBlock), lds, st, sc->dev, dc, bc, film, sp,
-                       prm->sample_offset, prm->spp, qd);
-  } else {
-    hipLaunchKernelGGL(k_bdpt<false>, dim3(grid), dim3(kBlock), lds, st, sc->dev, dc, bc, film, sp,
-                       prm->sample_offset, prm->spp, qd);
+  for (uint32_t done = 0; done < prm->spp; done += per_chunk) {
+    uint32_t const count = std::min(per_chunk, prm->spp - done);
+    uint64_t const items = uint64_t(count) * wh;
+    uint32_t const grid = uint32_t(std::min<uint64_t>((items + kBlock - 1) / kBlock, 1u << 20));
+    if (stats) {
+      hipLaunchKernelGGL(k_bdpt<true>, dim3(grid), dim3(kBlock), lds, st, sc->dev, dc, bc, cols, sp,
+                         n, prm->sample_offset + done, items, qd);
+    } else {
+      hipLaunchKernelGGL(k_bdpt<false>, dim3(grid), dim3(kBlock), lds, st, sc->dev, dc, bc, cols,
+                         sp, n, prm->sample_offset + done, items, qd);
+    }
+    hipLaunchKernelGGL(k_bdpt_sum, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, film, cols,
+                       n, count);
   }
   if (!splat) {
     hipLaunchKernelGGL(k_add_into, dim3(uint32_t((wh * 3 + kBlock - 1) / kBlock)), dim3(kBlock), 0,
