@@ -45,12 +45,13 @@ struct BVert {
   float bsdf_x_ipdf, s_prev, s_next, cos_in_sh, cos_out_geo, g;
 };
 
+// The fields a sample can read before writing them (the flags of the dummy
+// vertices and of a path's last vertex, and densities of vertices whose BSDF
+// returned early); every other field is written before it is read.
 __device__ __forceinline__ void bvert_clear(BVert& v) {
-  float3 const z = f3(0.f, 0.f, 0.f);
   v.tri = -1;
   v.back = v.specular = false;
-  v.pos = v.win = v.wout = v.ng = v.ns = v.t = v.b = v.power = z;
-  v.bsdf_x_ipdf = v.s_prev = v.s_next = v.cos_in_sh = v.cos_out_geo = v.g = 0.0f;
+  v.s_prev = v.s_next = 0.0f;
 }
 
 // xyz/stdafx.hpp:50-63

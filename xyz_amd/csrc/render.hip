@@ -1616,8 +1616,15 @@ int autotune_impl(uint32_t n, xyz_scene* const* scenes, const xyz_camera* cams,
 // them to the film in sample order (the order of BidirectionalPathTracer::run,
 // integrator.cpp:1059-1081), so own colours are summed deterministically;
 // light-tracing contributions land on other pixels through float atomics.
+#ifndef XYZ_BDPT_WAVES_PER_EU
+#define XYZ_BDPT_WAVES_PER_EU 8
+#endif
 template <bool kCount>
-__global__ void __launch_bounds__(kBlock) k_bdpt(DevScene s, DevCamera cam, BdptConst bc,
+__global__ void __launch_bounds__(kBlock)
+#if XYZ_BDPT_WAVES_PER_EU > 0
+    __attribute__((amdgpu_waves_per_eu(XYZ_BDPT_WAVES_PER_EU)))
+#endif
+    k_bdpt(DevScene s, DevCamera cam, BdptConst bc,
                                                  float4* colours, float* splat, uint32_t n,
                                                  uint32_t sample0, uint64_t items,
                                                  unsigned long long* queries) {
