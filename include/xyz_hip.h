@@ -254,6 +254,63 @@ int xyz_render(xyz_scene *scene, const xyz_camera *camera, const xyz_render_para
 int xyz_render_bdpt(xyz_scene *scene, const xyz_camera *camera, const xyz_render_params *params,
                     float *film, float *splat_film, xyz_render_stats *stats, void *stream);
 
+/* ------------------------------------------- importance-driven path tracing */
+typedef struct xyz_importon_map xyz_importon_map;
+
+typedef struct {
+  uint64_t n_importons;
+  uint32_t n_emit;       /* light paths traced (0 for a map of caller importons) */
+  uint32_t grid[3];      /* cells per axis of the query grid */
+  float cell;            /* cell width (>= radius) */
+  float radius;          /* search radius: 0.1 (reference CONFIG_SEARCH_RADIUS) */
+  double seconds;        /* wall time of the build */
+} xyz_importon_info;
+
+/* Replaces Scene::CreateLightImportanceMap (reference
+ * xyz/core/scene_photonmap.cpp:272-334) with Bsdf::BoundsImportance
+ * (xyz/bsdf/lambert.cpp:16-40, mirror.cpp:18-48, glass.cpp:16-91,
+ * metal.cpp:22-66, light.cpp:11-16): n_emit light paths (reference
+ * CONFIG_EMITED_SIZE = 100000) of at most max_depth + 1 segments (the
+ * reference's XYZ_CONFIG_kMaxRandomWalkDepth) leave an importon -- hit
+ * position and incoming direction -- at every Lambert hit reached through a
+ * specular bounce after the first segment.  Light paths are traced on the
+ * device; random numbers come from Philox stream 4 keyed by (path index,
+ * seed).  The importons keep path order, then vertex order, and are indexed
+ * by a uniform grid for the fixed-radius query.  Synchronous. */
+int xyz_importon_map_create(xyz_scene *scene, uint32_t n_emit, uint32_t max_depth, uint64_t seed,
+                            xyz_importon_map **out);
+/* A map of caller importons: host float[6*n], position then incoming
+ * direction per importon (no counterpart in the reference, which only builds
+ * its map from light paths). */
+int xyz_importon_map_from_host(xyz_scene *scene, uint64_t n, const float *importons,
+                               xyz_importon_map **out);
+int xyz_importon_map_info(const xyz_importon_map *map, xyz_importon_info *info);
+/* host float[6 * n_importons], in map order */
+int xyz_importon_map_export(const xyz_importon_map *map, float *importons);
+/* ImportanceQuery (reference xyz/core/photonmap.cpp:321-363) at n vertices:
+ * basis = device float[12*n] (position, shading normal, tangent, binormal);
+ * cdf = device float[17*n], the cumulative 2 x 8 histogram of cos^2 theta and
+ * azimuth octant over the importons within the radius, 16 per bin to start
+ * with (cdf[0] = 0). */
+int xyz_importance_query(const xyz_importon_map *map, uint32_t n, const float *basis, float *cdf,
+                         void *stream);
+void xyz_importon_map_destroy(xyz_importon_map *map);
+
+/* Importance-driven path tracing: replaces the pixel loop around
+ * Scene::EvaluteImportanceDrivenPathTracing (reference
+ * xyz/core/scene_importancedrivenpathtracing.cpp:11-179).  xyz_render's
+ * estimator, except at a Lambert vertex reached through a specular bounce
+ * (not the first hit): there the importance histogram of the map's importons
+ * within 0.1 (Lambert::CalculateWeightedDirectLighting and
+ * NextScatteringDirection with importons, xyz/bsdf/lambert.cpp:189-259)
+ * scales the NEE weight's implicit density and chooses the next direction (a
+ * bin from the cdf, then uniform in the bin's cos^2 theta and azimuth), and
+ * the path power is divided by the bin's density.  Same film contract as
+ * xyz_render; map must be built for this scene on the current device. */
+int xyz_render_idpt(xyz_scene *scene, const xyz_importon_map *map, const xyz_camera *camera,
+                    const xyz_render_params *params, float *film, xyz_render_stats *stats,
+                    void *stream);
+
 /* Path-replay adjoint of xyz_render (no counterpart in the reference).  Given
  * d_film = dLoss/d(film SUM) (device float[w*h*3]), adds dLoss/d(texel) to
  * d_texels (device, laid out like the scene's texels) and dLoss/dL_e to

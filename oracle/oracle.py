@@ -75,6 +75,24 @@ def lib():
                                           ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                           ctypes.c_uint32, ctypes.c_float, ctypes.c_uint64, _f32p,
                                           _f64p, _f64p, ctypes.c_int]
+        L.orc_importon_map_create.restype = _vp
+        L.orc_importon_map_create.argtypes = [_vp, ctypes.c_uint32, ctypes.c_uint32,
+                                              ctypes.c_uint64]
+        L.orc_importon_count.restype = ctypes.c_uint64
+        L.orc_importon_count.argtypes = [_vp]
+        L.orc_importon_export.argtypes = [_vp, _f32p]
+        L.orc_importon_map_destroy.argtypes = [_vp]
+        L.orc_guided_sample.argtypes = [_f32p, _f32p, ctypes.c_uint64, _f32p, _f32p]
+        L.orc_importance_density.argtypes = [_f32p, _f32p, ctypes.c_uint64, _f32p, _f32p]
+        L.orc_azimuth_octant.argtypes = [ctypes.c_uint64, _f32p, _f32p, _i32p]
+        L.orc_importon_map_from.restype = _vp
+        L.orc_importon_map_from.argtypes = [_f32p, ctypes.c_uint64]
+        L.orc_importance_query.argtypes = [_vp, _f32p, ctypes.c_uint32, _f32p, _f32p, _f32p]
+        L.orc_render_idpt.restype = ctypes.c_double
+        L.orc_render_idpt.argtypes = [_vp, _vp, _f32p, ctypes.c_uint32, ctypes.c_uint32,
+                                      ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                      ctypes.c_uint32, ctypes.c_float, ctypes.c_uint64,
+                                      ctypes.c_uint32, ctypes.c_uint32, _f32p, _u64p, ctypes.c_int]
         L.orc_max_threads.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -239,6 +257,25 @@ class OracleScene:
                                     tris=int(st[2]))
         return own, splat
 
+    def importon_map(self, n_emit=100000, max_depth=16, seed=0):
+        """Importons of `n_emit` light paths (Scene::CreateLightImportanceMap)."""
+        return OracleImportonMap(self, n_emit, max_depth, seed)
+
+    def render_idpt(self, imap, camera, W, H, spp, max_depth, rr_start=0, rr_threshold=0.2,
+                    seed=0, sample0=0, rows=None, threads=0, film=None, stats=False):
+        """Importance-driven path tracing guided by `imap`: per-pixel sums, (H, W, 3)."""
+        cam = camera12(camera)
+        if film is None:
+            film = np.zeros((H, W, 3), np.float32)
+        y0, y1 = (0, H) if rows is None else rows
+        st = np.zeros(3, np.uint64)
+        secs = lib().orc_render_idpt(self.h, imap.h, _p(cam, _f32p), W, H, spp, sample0,
+                                     max_depth, rr_start, rr_threshold, seed, y0, y1,
+                                     _p(film, _f32p), _p(st, _u64p), threads)
+        if stats:
+            return film, dict(seconds=secs, queries=int(st[0]), nodes=int(st[1]), tris=int(st[2]))
+        return film
+
     def render_backward(self, camera, W, H, spp, max_depth, d_film, rr_start=0,
                         rr_threshold=0.2, seed=0, sample0=0, threads=0):
         cam = camera12(camera)
@@ -249,6 +286,83 @@ class OracleScene:
                                   rr_threshold, seed, _p(d_film, _f32p), _p(d_tex, _f64p),
                                   _p(d_emit, _f64p), threads)
         return d_tex[:self.n_texels], d_emit.reshape(-1, 3)
+
+
+class OracleImportonMap:
+    """Importons (position, incoming direction) in path order, then vertex order."""
+
+    def __init__(self, scene, n_emit, max_depth, seed):
+        self.scene = scene  # keeps the scene alive
+        self.h = lib().orc_importon_map_create(scene.h, n_emit, max_depth, seed)
+
+    @classmethod
+    def from_importons(cls, data):
+        """Map of caller importons, an (n, 6) array of position, incoming direction."""
+        m = cls.__new__(cls)
+        m.scene = None
+        data = _c(data, np.float32).reshape(-1, 6)
+        m.h = lib().orc_importon_map_from(_p(data, _f32p), len(data))
+        return m
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_importon_map_destroy(self.h)
+            self.h = None
+
+    def __len__(self):
+        return int(lib().orc_importon_count(self.h))
+
+    def export(self):
+        out = np.zeros((len(self), 6), np.float32)
+        if len(out):
+            lib().orc_importon_export(self.h, _p(out, _f32p))
+        return out
+
+    def query(self, pos, ns, t, b, dirs=None):
+        """ImportanceQuery at a vertex: (cdf[17], density of each row of dirs)."""
+        basis = np.concatenate([np.asarray(x, np.float32).reshape(3) for x in (pos, ns, t, b)])
+        dirs = np.zeros((0, 3), np.float32) if dirs is None else _c(dirs, np.float32).reshape(-1, 3)
+        cdf = np.zeros(17, np.float32)
+        dens = np.zeros(len(dirs), np.float32)
+        lib().orc_importance_query(self.h, _p(basis, _f32p), len(dirs), _p(dirs, _f32p),
+                                   _p(cdf, _f32p), _p(dens, _f32p))
+        return cdf, dens
+
+
+def _basis12(ns, t, b, pos=(0, 0, 0)):
+    return np.concatenate([np.asarray(x, np.float32).reshape(3) for x in (pos, ns, t, b)])
+
+
+def guided_sample(cdf, ns, t, b, u):
+    """Guided Lambert direction for each row of u (bin, cos^2 theta, azimuth
+    numbers): (dirs (n, 3), bin density (n,), density per solid angle (n,))."""
+    cdf = _c(cdf, np.float32).reshape(17)
+    basis = _basis12(ns, t, b)
+    u = _c(u, np.float32).reshape(-1, 3)
+    out = np.zeros((len(u), 5), np.float32)
+    lib().orc_guided_sample(_p(cdf, _f32p), _p(basis, _f32p), len(u), _p(u, _f32p),
+                            _p(out, _f32p))
+    return out[:, :3], out[:, 3], out[:, 4]
+
+
+def importance_density(cdf, ns, t, b, dirs):
+    """ImportanceQuery::GetDensity of each incoming direction for a given cdf."""
+    cdf = _c(cdf, np.float32).reshape(17)
+    basis = _basis12(ns, t, b)
+    dirs = _c(dirs, np.float32).reshape(-1, 3)
+    out = np.zeros(len(dirs), np.float32)
+    lib().orc_importance_density(_p(cdf, _f32p), _p(basis, _f32p), len(dirs), _p(dirs, _f32p),
+                                 _p(out, _f32p))
+    return out
+
+
+def azimuth_octant(x, y):
+    """Azimuth octant of the importance histogram for arrays x, y."""
+    x = _c(x, np.float32).ravel()
+    y = _c(y, np.float32).ravel()
+    out = np.zeros(len(x), np.int32)
+    lib().orc_azimuth_octant(len(x), _p(x, _f32p), _p(y, _f32p), _p(out, _i32p))
+    return out
 
 
 def intersect_triangle(v0, v1, v2, o, d, t_min=1e-5, t_max=np.inf):

@@ -644,11 +644,250 @@ struct VertexRecord {
 
 struct PathStats { uint32_t queries = 0; Counters c; };
 
+// ------------------------------------------------------------------ IDPT
+// Importance-driven path tracing (Scene::EvaluteImportanceDrivenPathTracing,
+// xyz/core/scene_importancedrivenpathtracing.cpp:11-167): a map of importons
+// (position, incoming direction) left by light paths on Lambert surfaces right
+// after a specular bounce (Scene::CreateLightImportanceMap,
+// xyz/core/scene_photonmap.cpp:272-334, with Bsdf::BoundsImportance) guides
+// the direction sampling and NEE weights of Lambert eye vertices reached
+// through a specular bounce, through a 2 x 8 histogram of the importons within
+// 0.1 around the vertex (ImportanceQuery, xyz/core/photonmap.cpp:321-512).
+struct Importon { V3 pos, dir; };
+constexpr uint32_t kIdptStreamEmit = 4;
+constexpr int kU = 2, kV = 8, kN = kU * kV;  // CONFIG_U_SIZE, CONFIG_V_SIZE
+const real kSearchR2 = real(0.1) * real(0.1);  // square_of(CONFIG_SEARCH_RADIUS)
+
+// Bsdf::BoundsImportance; v.dir holds the incoming direction and becomes the
+// outgoing one.  u: x choice / Russian roulette, y factor test (glass), y z
+// direction (Lambert, Metal).
+bool bounds_importance(Bsdf const& b, bool back, V3 const& ns, V3 const& ng, V3 const& t,
+                       V3 const& bn, Draw const& u, V3* dir, bool* specular) {
+  switch (b.type) {
+    case LAMBERT: {  // lambert.cpp:16-40
+      real const cs = -dot(*dir, ns);
+      if (cs <= 0) return false;
+      real const cg = -dot(*dir, ns);  // the reference takes the shading normal twice
+      real const f = cs / cg;
+      if (u.u[0] >= b.rgb[1] * f) return false;
+      *specular = false;
+      real const phi = real(2 * M_PI) * u.u[2];
+      real const ct = std::sqrt(u.u[1]);
+      real const st = std::sqrt(1 - u.u[1]);
+      *dir = (t * (st * xyz_cos(phi)) + bn * (st * xyz_sin(phi))) + ng * ct;
+      return true;
+    }
+    case MIRROR: {  // mirror.cpp:18-48
+      real const cs = -dot(*dir, ns);
+      if (cs <= 0) return false;
+      real const cg = -dot(*dir, ng);
+      *dir = *dir + (real(2) * cs) * ns;
+      real const co = dot(*dir, ng);
+      if (co <= 0) return false;
+      if (u.u[0] >= co / cg) return false;
+      *specular = true;
+      return true;
+    }
+    case GLASS: {  // glass.cpp:16-91
+      real const cs = -dot(*dir, ns);
+      if (cs <= 0) return false;
+      real const cg = -dot(*dir, ng);
+      real const r = back ? rcp(b.ior) : b.ior;
+      real const ior_sq = r * r;
+      real const nk = cs;
+      real const bb = ior_sq + nk * nk - 1;
+      *specular = true;
+      if (bb > 0) {
+        real const g = std::sqrt(bb);
+        real const gpc = g + nk, gmc = g - nk;
+        real const alpha = gmc / gpc;
+        real const beta = (nk * gpc - 1) / (nk * gmc + 1);
+        real const fr = real(0.5) * alpha * alpha * (beta * beta + 1);
+        if (u.u[0] < fr) {
+          *dir = *dir + (real(2) * nk) * ns;
+          real const co = dot(*dir, ng);
+          if (co <= 0) return false;
+          if (u.u[1] >= co / cg) return false;
+        } else {
+          *dir = (*dir - gmc * ns) / std::sqrt(ior_sq);
+          real const co = -dot(*dir, ng);
+          if (co <= 0) return false;
+          if (u.u[1] >= co / cg) return false;
+        }
+      } else {
+        *dir = *dir + (real(2) * nk) * ns;
+        real const co = dot(*dir, ng);
+        if (co <= 0) return false;
+        if (u.u[1] >= co / cg) return false;
+      }
+      return true;
+    }
+    case METAL: {  // metal.cpp:22-66
+      real const nk1 = -dot(*dir, ns);
+      if (nk1 <= 0) return false;
+      real const cg = -dot(*dir, ng);
+      real const phi = real(2 * M_PI) * u.u[2];
+      real const ch = std::pow(u.u[1], rcp(b.shininess));
+      real const sh = std::sqrt(real(1) - ch * ch);
+      V3 const hv = (t * (sh * xyz_cos(phi)) + bn * (sh * xyz_sin(phi))) + ns * ch;
+      real const hk = -dot(hv, *dir);
+      *dir = *dir + (2 * hk) * hv;
+      real const nk2 = dot(*dir, ns);
+      if (nk2 <= 0) return false;
+      real const co = dot(ng, *dir);
+      if (co <= 0) return false;
+      real const f = fake_fresnel(hk) / hmax(nk1, nk2) * nk2 * (co / cg);
+      if (u.u[0] >= b.rgb[1] * f) return false;
+      *specular = false;
+      return true;
+    }
+    default:
+      return false;  // light.cpp:11-16
+  }
+}
+
+// Scene::CreateLightImportanceMap with n_emit light paths of at most kmax
+// segments; random numbers: Philox stream 4 keyed (path index, seed),
+// vertex 0 the light sample (x, y) and emission direction (z, w).
+void emit_importons(Scene const& sc, uint32_t n_emit, uint32_t kmax, uint64_t seed,
+                    std::vector<Importon>* out) {
+  out->clear();
+  if (sc.lights.empty()) return;
+  for (uint32_t i = 0; i < n_emit; ++i) {
+    Draw const d0 = draw(i, seed, 0, 0, kIdptStreamEmit);
+    Vertex lv;
+    int n = -1;
+    sample_light(sc, d0.u[0], d0.u[1], &lv, &n);
+    V3 t(-lv.ng[2], 0, lv.ng[0]), bn;
+    real const l2 = length_squared(t);
+    if (l2 > 0) {
+      t = t * rsqrt(l2);
+      bn = cross(t, lv.ng);
+    } else {
+      bn = V3(0, 0, (lv.ng[1] > 0) ? 1 : -1);
+      t = cross(lv.ng, bn);
+    }
+    real const phi = real(2 * M_PI) * d0.u[3];
+    real const ct = std::sqrt(d0.u[2]);
+    real const st = std::sqrt(1 - d0.u[2]);
+    V3 dir = (t * (st * xyz_cos(phi)) + bn * (st * xyz_sin(phi))) + lv.ng * ct;
+    V3 pos = lv.pos;
+    bool specular = false;
+    for (uint32_t depth = 0; depth < kmax; ++depth) {
+      Hit h;
+      Counters c;
+      int const tri = scene_find(sc, pos, dir, &h, &c);
+      if (tri < 0) break;
+      Tri const& tr = sc.tris[size_t(tri)];
+      Bsdf const& b = sc.bsdfs[tr.bsdf];
+      Vertex v;
+      if (b.type == LAMBERT) {
+        // Triangle::GeometricBasis
+        V3 ns = normalize((tr.n0 + h.f * tr.s1) + h.g * tr.s2);
+        V3 ng = tr.ng;
+        if (h.back) { ng = -ng; ns = -ns; }
+        V3 tt(-ng[2], 0, ng[0]);
+        real const q2 = length_squared(tt);
+        V3 bb;
+        if (q2 > 0) {
+          tt = tt * rsqrt(q2);
+          bb = cross(tt, ng);
+        } else {
+          bb = V3(0, 0, (ng[1] > 0) ? 1 : -1);
+          tt = cross(ng, bb);
+        }
+        v.ns = ns; v.ng = ng; v.t = tt; v.b = bb;
+        pos = pos + dir * h.t_max;
+        if (depth > 0 && specular) out->push_back(Importon{pos, dir});
+      } else {
+        shading_basis(tr, h, &v);
+        pos = pos + dir * h.t_max;
+      }
+      if (!bounds_importance(b, h.back, v.ns, v.ng, v.t, v.b,
+                             draw(i, seed, 0, depth + 1, kIdptStreamEmit), &dir, &specular))
+        break;
+    }
+  }
+}
+
+// Octant of the azimuth theta = atan2(y, x) in [0, 2 pi), an edge belonging to
+// the octant above it: the reference's v = floor(phi * 8 / pi) with
+// phi = atan2(y, x) / 2 (+ pi when negative) (photonmap.cpp:351-360) in exact
+// arithmetic.  The reference rounds atan2 to float first, which can move a
+// direction within about 1e-7 rad of an octant edge to the neighbouring bin.
+int azimuth_octant(real x, real y) {
+  if (x == 0 && y == 0) return 0;
+  if (y > 0 || (y == 0 && x > 0)) {  // theta in [0, pi)
+    if (x > y) return 0;
+    if (x > 0) return 1;
+    return (y > -x) ? 2 : 3;
+  }
+  if (x < 0) return (y > x) ? 4 : 5;  // theta in [pi, 2 pi)
+  return (y < -x) ? 6 : 7;
+}
+
+// Bin of a direction with cosine ct to the shading normal and components x, y
+// along -tangent, -binormal: u = int(ct^2 * 2) (photonmap.cpp:345-349), v the
+// azimuth octant.
+int importance_bin(real ct, real x, real y) {
+  int u = int((ct * ct) * kU);
+  if (u >= kU) u = kU - 1;
+  return u * kV + azimuth_octant(x, y);
+}
+
+// ImportanceQuery: cdf[0..16] of the importons within reach of vertex v.
+void importance_cdf(std::vector<Importon> const& imp, Vertex const& v, real cdf[kN + 1]) {
+  cdf[0] = 0;
+  for (int i = 1; i <= kN; ++i) cdf[i] = real(kN);
+  for (Importon const& m : imp) {
+    V3 const d = m.pos - v.pos;
+    if (!(length_squared(d) < kSearchR2)) continue;
+    real const ct = -dot(m.dir, v.ns);
+    if (ct <= 0) continue;
+    V3 const pt = d + m.dir * dot(d, v.ns);
+    if (length_squared(pt) >= kSearchR2) continue;
+    cdf[1 + importance_bin(ct, -dot(m.dir, v.t), -dot(m.dir, v.b))] += 1;
+  }
+  for (int i = 2; i <= kN; ++i) cdf[i] += cdf[i - 1];
+}
+
+// ImportanceQuery::GetDensity (photonmap.cpp:430-457)
+real importance_density(real const cdf[kN + 1], Vertex const& v, V3 const& dir) {
+  real const ct = -dot(dir, v.ns);
+  if (ct <= 0) return 0;
+  int const n = importance_bin(ct, -dot(dir, v.t), -dot(dir, v.b));
+  return kN * (cdf[n + 1] - cdf[n]) / cdf[kN];
+}
+
+// Lambert::NextScatteringDirection with importons (lambert.cpp:243-259) and
+// ImportanceQuery::SetDiffusedVector (photonmap.cpp:459-494): a bin from the
+// cdf with u_bin, then cos^2 theta uniform in the bin's half (u_t) and the
+// azimuth uniform in its octant (u_p).  Returns the bin's density; *f_next =
+// the direction's density per solid angle, density cos theta / pi.
+real guided_sample(real const cdf[kN + 1], Vertex const& v, real u_bin, real u_t, real u_p,
+                   V3* dir, real* f_next) {
+  real const value = cdf[kN] * u_bin;
+  int n = int(std::upper_bound(cdf, cdf + kN + 1, value) - cdf) - 1;
+  if (n < 0) n = 0;
+  if (n >= kN) n = kN - 1;
+  int const bu = n / kV, bv = n % kV;
+  real const mu_u = (real(bu) + u_t) / real(kU);
+  real const mu_v = (real(bv) + u_p) / real(kV);
+  real const cos_t = std::sqrt(mu_u);
+  real const sin_t = std::sqrt(1 - mu_u);
+  real const phi = real(2 * M_PI) * mu_v;
+  *dir = (v.t * (sin_t * xyz_cos(phi)) + v.b * (sin_t * xyz_sin(phi))) + v.ns * cos_t;
+  real const density = kN * (cdf[n + 1] - cdf[n]) / cdf[kN];
+  *f_next = real(double(density * cos_t) * M_1_PI);
+  return density;
+}
+
 // xyz/core/scene_pathtracing.cpp:11-144.  Returns the sample's colour.
 // Optional: records Lambert vertices and per-light emission contributions.
 V3 pt_path(Scene const& sc, Camera const& cam, PTParams const& pp, uint32_t W, uint32_t H,
            uint32_t px, uint32_t py, uint32_t sample, PathStats* st,
-           std::vector<VertexRecord>* rec, std::vector<V3>* light_contrib) {
+           std::vector<VertexRecord>* rec, std::vector<V3>* light_contrib,
+           std::vector<Importon> const* guide = nullptr) {
   uint32_t const pixel = py * W + px;
   Draw d0 = draw(pixel, pp.seed, sample, 0);
   // xyz/integrator/integrator.cpp:124-125
@@ -719,6 +958,12 @@ V3 pt_path(Scene const& sc, Camera const& cam, PTParams const& pp, uint32_t W, u
     uint32_t const j = k - 1;  // vertex index, 1-based
     Draw dj = draw(pixel, pp.seed, sample, j);
 
+    // Importance-driven branch (scene_importancedrivenpathtracing.cpp:122-152):
+    // a Lambert vertex reached through a specular bounce, not the first hit.
+    bool const guided = guide && (b.type == LAMBERT) && (k > 2) && ev.specular;
+    real cdf[kN + 1];
+    if (guided) importance_cdf(*guide, ev, cdf);
+
     // ---- NEE: Bsdf::CalculateWeightedDirectLighting
     if (b.type == LAMBERT || b.type == METAL) {
       int texel = -1;
@@ -752,7 +997,10 @@ V3 pt_path(Scene const& sc, Camera const& cam, PTParams const& pp, uint32_t W, u
             if (b.type == LAMBERT) {
               // xyz/bsdf/lambert.cpp:157-172
               real const G = nk2 * nk3 / (sh.t_max * sh.t_max);
-              real const p_imp = real(M_1_PI * double(G));
+              // lambert.cpp:228-229 scales the implicit density by the histogram's
+              real const p_imp = guided ? real((M_1_PI * double(G)) *
+                                               double(importance_density(cdf, ev, w)))
+                                        : real(M_1_PI * double(G));
               real const p_exp = rcp(A);
               real const dens = p_imp / p_exp;
               double const wgt = (double(G) * (M_1_PI * M_1_PI)) * double(rcp(1 + dens * dens));
@@ -799,7 +1047,12 @@ V3 pt_path(Scene const& sc, Camera const& cam, PTParams const& pp, uint32_t W, u
     }
 
     // ---- Bsdf::NextScatteringDirection
-    if (b.type == LAMBERT) {
+    if (guided) {
+      real const density = guided_sample(cdf, ev, dj.u[3], dj.u[1], dj.u[2], &ev.dir, &ev.f_next);
+      ev.power = ev.power * rcp(density);
+      ev.specular = false;
+      if (!(dot(ev.dir, ev.ng) > 0)) break;
+    } else if (b.type == LAMBERT) {
       // xyz/bsdf/lambert.cpp:176-187, xyz/stdafx.hpp:50-63
       real const phi = real(2 * M_PI) * dj.u[2];
       real const cos_t = std::sqrt(dj.u[1]);
@@ -1723,6 +1976,123 @@ double orc_render_backward(void* s, const float* cam, uint32_t W, uint32_t H, ui
     for (size_t i = 0; i < tex_off.back(); ++i) d_tex[i] += gt[size_t(t)][i];
     for (size_t i = 0; i < sc.bsdfs.size() * 3; ++i) d_emit[i] += ge[size_t(t)][i];
   }
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// Importon map (Scene::CreateLightImportanceMap, scene_photonmap.cpp:272-334):
+// n_emit light paths of at most max_depth + 1 segments.  The map keeps the
+// importons in path order, then vertex order.
+void* orc_importon_map_create(void* s, uint32_t n_emit, uint32_t max_depth, uint64_t seed) {
+  auto* m = new std::vector<Importon>();
+  emit_importons(*static_cast<Scene*>(s), n_emit, max_depth + 1, seed, m);
+  return m;
+}
+uint64_t orc_importon_count(void* m) { return static_cast<std::vector<Importon>*>(m)->size(); }
+// out: count * 6 floats (position, incoming direction)
+void orc_importon_export(void* m, float* out) {
+  auto const& v = *static_cast<std::vector<Importon>*>(m);
+  for (size_t i = 0; i < v.size(); ++i)
+    for (int c = 0; c < 3; ++c) {
+      out[6 * i + c] = v[i].pos[c];
+      out[6 * i + 3 + c] = v[i].dir[c];
+    }
+}
+void orc_azimuth_octant(uint64_t n, const float* x, const float* y, int32_t* out) {
+  for (uint64_t i = 0; i < n; ++i) out[i] = azimuth_octant(x[i], y[i]);
+}
+// Importon map from caller importons (n * 6 floats: position, direction).
+void* orc_importon_map_from(const float* data, uint64_t n) {
+  auto* m = new std::vector<Importon>(n);
+  for (uint64_t i = 0; i < n; ++i) {
+    (*m)[i].pos = V3(data[6 * i], data[6 * i + 1], data[6 * i + 2]);
+    (*m)[i].dir = V3(data[6 * i + 3], data[6 * i + 4], data[6 * i + 5]);
+  }
+  return m;
+}
+// guided_sample for n draws (u: 3 per draw: bin, cos^2 theta, azimuth) at the
+// vertex basis12 = pos, ns, t, b with cdf (17 floats).  out: 5 per draw:
+// direction, bin density, density per solid angle.
+void orc_guided_sample(const float* cdf, const float* basis12, uint64_t n, const float* u,
+                       float* out) {
+  Vertex q;
+  q.pos = V3(basis12[0], basis12[1], basis12[2]);
+  q.ns = V3(basis12[3], basis12[4], basis12[5]);
+  q.t = V3(basis12[6], basis12[7], basis12[8]);
+  q.b = V3(basis12[9], basis12[10], basis12[11]);
+  real c[kN + 1];
+  for (int i = 0; i <= kN; ++i) c[i] = cdf[i];
+  for (uint64_t i = 0; i < n; ++i) {
+    V3 d;
+    real fn;
+    real const dens = guided_sample(c, q, u[3 * i], u[3 * i + 1], u[3 * i + 2], &d, &fn);
+    out[5 * i] = d[0];
+    out[5 * i + 1] = d[1];
+    out[5 * i + 2] = d[2];
+    out[5 * i + 3] = dens;
+    out[5 * i + 4] = fn;
+  }
+}
+// importance_density of n directions for a given cdf (the query without a map)
+void orc_importance_density(const float* cdf, const float* basis12, uint64_t n, const float* dirs,
+                            float* out) {
+  Vertex q;
+  q.ns = V3(basis12[3], basis12[4], basis12[5]);
+  q.t = V3(basis12[6], basis12[7], basis12[8]);
+  q.b = V3(basis12[9], basis12[10], basis12[11]);
+  real c[kN + 1];
+  for (int i = 0; i <= kN; ++i) c[i] = cdf[i];
+  for (uint64_t i = 0; i < n; ++i)
+    out[i] = importance_density(c, q, V3(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
+}
+void orc_importon_map_destroy(void* m) { delete static_cast<std::vector<Importon>*>(m); }
+
+// ImportanceQuery at a vertex: cdf (17 floats) for position / shading normal /
+// tangent / binormal in basis12 = pos, ns, t, b; density of each dir in dirs.
+void orc_importance_query(void* m, const float* basis12, uint32_t n_dirs, const float* dirs,
+                          float* cdf, float* density) {
+  auto const& v = *static_cast<std::vector<Importon>*>(m);
+  Vertex q;
+  q.pos = V3(basis12[0], basis12[1], basis12[2]);
+  q.ns = V3(basis12[3], basis12[4], basis12[5]);
+  q.t = V3(basis12[6], basis12[7], basis12[8]);
+  q.b = V3(basis12[9], basis12[10], basis12[11]);
+  importance_cdf(v, q, cdf);
+  for (uint32_t i = 0; i < n_dirs; ++i)
+    density[i] = importance_density(cdf, q, V3(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
+}
+
+// Importance-driven path tracing (Scene::EvaluteImportanceDrivenPathTracing,
+// scene_importancedrivenpathtracing.cpp:11-179): orc_render with the guided
+// Lambert vertices.  film: per-pixel sums in sample order.
+double orc_render_idpt(void* s, void* m, const float* cam, uint32_t W, uint32_t H, uint32_t spp,
+                       uint32_t sample0, uint32_t max_depth, uint32_t rr_start, float rr_threshold,
+                       uint64_t seed, uint32_t y0, uint32_t y1, float* film, uint64_t* stats,
+                       int threads) {
+  Scene const& sc = *static_cast<Scene*>(s);
+  auto const* guide = static_cast<std::vector<Importon>*>(m);
+  Camera const c = make_camera(cam, W, H);
+  PTParams pp;
+  pp.kmax = max_depth + 1;
+  pp.rr_start = rr_start;
+  pp.rr_threshold = rr_threshold;
+  pp.seed = seed;
+  int const nt = n_threads(threads);
+  std::atomic<uint64_t> q(0), nn(0), ntr(0);
+  auto t0 = std::chrono::steady_clock::now();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
+  for (int64_t y = int64_t(y0); y < int64_t(y1); ++y) {
+    PathStats ps;
+    for (uint32_t x = 0; x < W; ++x) {
+      float* px = film + (size_t(y) * W + x) * 3;
+      for (uint32_t smp = sample0; smp < sample0 + spp; ++smp) {
+        V3 col = pt_path(sc, c, pp, W, H, x, uint32_t(y), smp, stats ? &ps : nullptr, nullptr,
+                         nullptr, guide);
+        px[0] += col[0]; px[1] += col[1]; px[2] += col[2];
+      }
+    }
+    if (stats) { q += ps.queries; nn += ps.c.nodes; ntr += ps.c.tris; }
+  }
+  if (stats) { stats[0] = q; stats[1] = nn; stats[2] = ntr; }
   return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 

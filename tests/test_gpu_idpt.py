@@ -1,0 +1,197 @@
+"""Importance-driven path tracing on the GPU against the oracle.
+
+The light paths of the importon map, the histogram query and the guided
+integrator evaluate the same binary32 operations in the same order as the
+oracle with the same counter-based random numbers, so importon arrays (in map
+order), histograms and films are compared bit for bit."""
+import numpy as np
+import pytest
+
+from xyz_amd import scenes as S
+
+pytestmark = pytest.mark.gpu
+
+
+def both(oracle_mod, capi_mod, sc):
+    o = oracle_mod.OracleScene(sc["v0"], sc["v1"], sc["v2"], tri_bsdf=sc.get("tri_bsdf"),
+                               bsdfs=sc.get("bsdfs"), normals=sc.get("normals"))
+    g = capi_mod.Scene.from_dict(sc)
+    return o, g
+
+
+def _same(a, b, what):
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    assert a.shape == b.shape, (what, a.shape, b.shape)
+    diff = a.view(np.uint32) != b.view(np.uint32)
+    assert not diff.any(), f"{what}: {diff.mean():.4%} differ, max abs {np.abs(a - b).max()}"
+
+
+def idpt_film(scene, imap, cam, W, H, spp, D, rr_start=0, seed=0, sample_offset=0):
+    from xyz_amd.hip import DeviceArray
+    film = DeviceArray.zeros((H, W, 3), np.float32)
+    st = scene.render_idpt(imap, cam, film, spp, D, rr_start=rr_start, seed=seed,
+                           sample_offset=sample_offset)
+    return film.numpy(), st
+
+
+@pytest.mark.parametrize("name", ["mirror_ceiling", "glass_slab", "caustic"])
+def test_importon_map_equals_oracle(oracle_mod, capi_mod, device, name):
+    sc = {"mirror_ceiling": S.mirror_ceiling_scene, "glass_slab": S.glass_slab_scene,
+          "caustic": lambda: S.caustic_scene(sphere_level=2)}[name]()
+    o, g = both(oracle_mod, capi_mod, sc)
+    print(f"importon map {name}", flush=True)
+    ref = o.importon_map(20000, max_depth=15, seed=9).export()
+    m = g.importon_map(20000, max_depth=15, seed=9)
+    info = m.info()
+    assert info["n_importons"] == len(ref) > 0
+    assert info["n_emit"] == 20000 and info["cell"] >= info["radius"] > 0
+    _same(m.export(), ref, f"importons ({name})")
+
+
+def test_importance_query_equals_oracle(oracle_mod, capi_mod, device):
+    from xyz_amd.hip import DeviceArray
+    sc = S.caustic_scene(sphere_level=2)
+    o, g = both(oracle_mod, capi_mod, sc)
+    om = o.importon_map(30000, max_depth=15, seed=4)
+    gm = g.importon_map(30000, max_depth=15, seed=4)
+    imp = om.export()
+    rng = np.random.default_rng(0)
+    n = 512
+    # query points on and around importons, random orthonormal frames
+    pos = imp[rng.integers(0, len(imp), n), :3] + rng.normal(0, 0.05, (n, 3)).astype(np.float32)
+    ns = rng.normal(size=(n, 3))
+    ns[: n // 2] = [0, 1, 0]
+    ns /= np.linalg.norm(ns, axis=1, keepdims=True)
+    a = np.where(np.abs(ns[:, :1]) < 0.9, [[1.0, 0, 0]], [[0, 1.0, 0]])
+    t = np.cross(a, ns)
+    t /= np.linalg.norm(t, axis=1, keepdims=True)
+    b = np.cross(ns, t)
+    basis = np.concatenate([pos, ns, t, b], 1).astype(np.float32)
+    ref = np.stack([om.query(q[:3], q[3:6], q[6:9], q[9:12])[0] for q in basis])
+    db = DeviceArray.from_numpy(basis)
+    dc = DeviceArray.zeros((n, 17), np.float32)
+    gm.query(db, dc)
+    got = dc.numpy()
+    _same(got, ref, "cdf")
+    assert (ref[:, 16] > 256).mean() > 0.3  # most queries see importons
+
+
+def test_importance_query_caller_importons(oracle_mod, capi_mod, device):
+    """A map from caller importons spread over a large box (the grid's cells
+    grow beyond the radius to stay within the cell limit)."""
+    from xyz_amd.hip import DeviceArray
+    sc = S.cornell_box()
+    g = capi_mod.Scene.from_dict(sc)
+    rng = np.random.default_rng(1)
+    n_imp = 40000
+    pos = np.concatenate([rng.uniform(-50, 50, (n_imp // 2, 3)),
+                          rng.normal(0, 0.05, (n_imp // 2, 3))]).astype(np.float32)
+    d = rng.normal(size=(n_imp, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    data = np.concatenate([pos, d], 1).astype(np.float32)
+    gm = capi_mod.ImportonMap(g, importons=data)
+    om = oracle_mod.OracleImportonMap.from_importons(data)
+    assert len(gm) == n_imp and gm.info()["n_emit"] == 0
+    _same(gm.export(), data, "caller importons")
+    q = 256
+    qp = np.concatenate([rng.normal(0, 0.05, (q // 2, 3)),
+                         pos[rng.integers(0, n_imp // 2, q // 2)]]).astype(np.float32)
+    basis = np.tile(np.array([0, 1, 0, 1, 0, 0, 0, 0, -1], np.float32), (q, 1))
+    basis = np.concatenate([qp, basis], 1)
+    ref = np.stack([om.query(r[:3], r[3:6], r[6:9], r[9:12])[0] for r in basis])
+    db = DeviceArray.from_numpy(basis)
+    dc = DeviceArray.zeros((q, 17), np.float32)
+    gm.query(db, dc)
+    _same(dc.numpy(), ref, "cdf (caller importons)")
+
+
+@pytest.mark.parametrize("smooth", [False, True])
+def test_idpt_film_bit_exact(oracle_mod, capi_mod, device, smooth):
+    sc = S.caustic_scene(sphere_level=2)
+    if not smooth:
+        sc.pop("normals")
+    o, g = both(oracle_mod, capi_mod, sc)
+    om = o.importon_map(30000, max_depth=12, seed=2)
+    gm = g.importon_map(30000, max_depth=12, seed=2)
+    print(f"idpt film smooth={smooth} importons={len(om)}", flush=True)
+    cam = sc["camera"]
+    ref = o.render_idpt(om, cam, 48, 40, 8, 12, rr_start=2, seed=6)
+    got, st = idpt_film(g, gm, cam, 48, 40, 8, 12, rr_start=2, seed=6)
+    _same(got, ref, "idpt film")
+    assert st["paths"] == 48 * 40 * 8
+    pt = o.render(cam, 48, 40, 8, 12, rr_start=2, seed=6)
+    assert not np.array_equal(pt, ref)  # guidance is active
+
+
+def test_idpt_reference_depth_and_rr(oracle_mod, capi_mod, device):
+    sc = S.caustic_scene(sphere_level=2)
+    o, g = both(oracle_mod, capi_mod, sc)
+    om = o.importon_map(20000, max_depth=15, seed=8)
+    gm = g.importon_map(20000, max_depth=15, seed=8)
+    ref = o.render_idpt(om, sc["camera"], 40, 32, 8, 15, rr_start=0, seed=12)
+    got, _ = idpt_film(g, gm, sc["camera"], 40, 32, 8, 15, rr_start=0, seed=12)
+    _same(got, ref, "idpt film (depth 15, rr everywhere)")
+
+
+def test_idpt_without_specular_is_pt(capi_mod, device):
+    from xyz_amd.hip import DeviceArray
+    sc = S.cornell_box()
+    g = capi_mod.Scene.from_dict(sc)
+    m = g.importon_map(20000, max_depth=8, seed=1)
+    assert len(m) == 0
+    a = DeviceArray.zeros((32, 32, 3), np.float32)
+    g.render(sc["camera"], a, 8, 8, rr_start=3, seed=4)
+    b, _ = idpt_film(g, m, sc["camera"], 32, 32, 8, 8, rr_start=3, seed=4)
+    _same(b, a.numpy(), "idpt == pt without specular surfaces")
+
+
+def test_idpt_progressive_passes(capi_mod, device):
+    from xyz_amd.hip import DeviceArray
+    sc = S.caustic_scene(sphere_level=2)
+    g = capi_mod.Scene.from_dict(sc)
+    m = g.importon_map(20000, max_depth=10, seed=3)
+    a = DeviceArray.zeros((24, 24, 3), np.float32)
+    g.render_idpt(m, sc["camera"], a, 9, 10, seed=1)
+    b = DeviceArray.zeros((24, 24, 3), np.float32)
+    g.render_idpt(m, sc["camera"], b, 4, 10, seed=1)
+    g.render_idpt(m, sc["camera"], b, 5, 10, seed=1, sample_offset=4)
+    _same(b.numpy(), a.numpy(), "progressive idpt")
+
+
+def test_idpt_mean_equals_pt_mean(capi_mod, device):
+    from xyz_amd.hip import DeviceArray
+    sc = S.caustic_scene(sphere_level=3)
+    g = capi_mod.Scene.from_dict(sc)
+    m = g.importon_map(100000, max_depth=15, seed=11)
+    W, H, B, spp = 96, 96, 16, 64
+    pt, idpt = [], []
+    for k in range(B):
+        a = DeviceArray.zeros((H, W, 3), np.float32)
+        g.render(sc["camera"], a, spp, 15, seed=5, sample_offset=k * spp)
+        pt.append(a.numpy().mean((0, 1)))
+        b, _ = idpt_film(g, m, sc["camera"], W, H, spp, 15, seed=5, sample_offset=k * spp)
+        idpt.append(b.mean((0, 1)))
+    pt, idpt = np.array(pt), np.array(idpt)
+    se = np.sqrt(pt.var(0, ddof=1) / B + idpt.var(0, ddof=1) / B)
+    z = (idpt.mean(0) - pt.mean(0)) / se
+    assert np.all(np.abs(z) < 4.5), z
+
+
+def test_idpt_errors(capi_mod, device):
+    from xyz_amd.hip import DeviceArray
+    sc = S.caustic_scene(sphere_level=1)
+    g1 = capi_mod.Scene.from_dict(sc)
+    g2 = capi_mod.Scene.from_dict(sc)
+    m = g1.importon_map(1000, max_depth=8, seed=0)
+    film = DeviceArray.zeros((8, 8, 3), np.float32)
+    with pytest.raises(capi_mod.XyzError, match="another scene"):
+        g2.render_idpt(m, sc["camera"], film, 1, 8)
+    with pytest.raises(capi_mod.XyzError, match="max_depth"):
+        g1.importon_map(10, max_depth=0)
+    bad = np.zeros((1, 6), np.float32)
+    bad[0, 0] = np.nan
+    with pytest.raises(capi_mod.XyzError, match="non-finite"):
+        capi_mod.ImportonMap(g1, importons=bad)
+    empty = g1.importon_map(0, max_depth=8)
+    assert len(empty) == 0

@@ -7,7 +7,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libxyz_hip.so")
 
-SOURCES = ["xyz_hip.hip", "render.hip", "kdtree_gpu.hip", "kdtree_build.cpp", "mqo_loader.cpp"]
+SOURCES = ["xyz_hip.hip", "render.hip", "kdtree_gpu.hip", "kdtree_build.cpp", "mqo_loader.cpp", "importon_map.hip"]
 
 # IEEE binary32 as the reference's source semantics: no FMA contraction,
 # correctly rounded division/sqrt (hipcc's default), denormals kept.  Packed

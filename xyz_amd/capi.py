@@ -92,6 +92,17 @@ class MqoInfo(ctypes.Structure):
         return d
 
 
+class ImportonInfo(ctypes.Structure):
+    _fields_ = [("n_importons", ctypes.c_uint64), ("n_emit", ctypes.c_uint32),
+                ("grid", ctypes.c_uint32 * 3), ("cell", ctypes.c_float),
+                ("radius", ctypes.c_float), ("seconds", ctypes.c_double)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["grid"] = list(d["grid"])
+        return d
+
+
 class BatchTuning(ctypes.Structure):
     _fields_ = [("tile_pixels", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32),
                 ("block_threads", ctypes.c_uint32), ("use_graph", ctypes.c_uint32),
@@ -103,7 +114,10 @@ EXPORTS = ["xyz_kdtree_build", "xyz_kdtree_build_device", "xyz_kdtree_info", "xy
            "xyz_mqo_load", "xyz_mqo_load_file", "xyz_mqo_describe", "xyz_mqo_destroy",
            "xyz_scene_create", "xyz_scene_get_info", "xyz_scene_destroy", "xyz_closest_hit",
            "xyz_closest_hit_work",
-           "xyz_render", "xyz_render_bdpt", "xyz_render_backward", "xyz_render_batch", "xyz_batch_autotune",
+           "xyz_importon_map_create", "xyz_importon_map_from_host", "xyz_importon_map_info",
+           "xyz_importon_map_export", "xyz_importance_query", "xyz_importon_map_destroy",
+           "xyz_render", "xyz_render_idpt", "xyz_render_bdpt", "xyz_render_backward", "xyz_render_batch",
+           "xyz_batch_autotune",
            "xyz_last_error", "xyz_version"]
 
 _lib = None
@@ -144,6 +158,16 @@ def lib():
         L.xyz_closest_hit_work.argtypes = [_vp, ctypes.c_uint32, _vp, _vp, _vp, _vp, _vp, _vp]
         L.xyz_render.argtypes = [_vp, ctypes.POINTER(Camera), ctypes.POINTER(RenderParams), _vp,
                                  ctypes.POINTER(RenderStats), _vp]
+        L.xyz_importon_map_create.argtypes = [_vp, ctypes.c_uint32, ctypes.c_uint32,
+                                              ctypes.c_uint64, ctypes.POINTER(_vp)]
+        L.xyz_importon_map_from_host.argtypes = [_vp, ctypes.c_uint64, _f32p, ctypes.POINTER(_vp)]
+        L.xyz_importon_map_info.argtypes = [_vp, ctypes.POINTER(ImportonInfo)]
+        L.xyz_importon_map_export.argtypes = [_vp, _f32p]
+        L.xyz_importance_query.argtypes = [_vp, ctypes.c_uint32, _vp, _vp, _vp]
+        L.xyz_importon_map_destroy.argtypes = [_vp]
+        L.xyz_importon_map_destroy.restype = None
+        L.xyz_render_idpt.argtypes = [_vp, _vp, ctypes.POINTER(Camera), ctypes.POINTER(RenderParams),
+                                      _vp, ctypes.POINTER(RenderStats), _vp]
         L.xyz_render_bdpt.argtypes = [_vp, ctypes.POINTER(Camera), ctypes.POINTER(RenderParams),
                                       _vp, _vp, ctypes.POINTER(RenderStats), _vp]
         L.xyz_render_backward.argtypes = [_vp, ctypes.POINTER(Camera),
@@ -333,6 +357,21 @@ class Scene:
                                ctypes.byref(st), _stream(stream)))
         return dict(paths=st.paths, queries=st.queries, seconds=st.seconds)
 
+    def importon_map(self, n_emit=100000, max_depth=16, seed=0):
+        """Importon map of `n_emit` light paths traced on the device."""
+        return ImportonMap(self, n_emit=n_emit, max_depth=max_depth, seed=seed)
+
+    def render_idpt(self, imap, camera, film, spp, max_depth, rr_start=0, rr_threshold=0.2,
+                    seed=0, sample_offset=0, stream=None):
+        """Importance-driven path tracing guided by `imap` (an ImportonMap of this
+        scene); film contract of render()."""
+        cam = make_camera(camera, film.shape[1], film.shape[0])
+        p = make_params(spp, max_depth, rr_start, rr_threshold, seed, sample_offset)
+        st = RenderStats()
+        check(lib().xyz_render_idpt(self.h, imap.h, ctypes.byref(cam), ctypes.byref(p),
+                                    _dptr(film), ctypes.byref(st), _stream(stream)))
+        return dict(paths=st.paths, queries=st.queries, seconds=st.seconds)
+
     def render_bdpt(self, camera, film, spp, max_depth, seed=0, sample_offset=0, splat=None,
                     stream=None):
         """Bidirectional path tracing: adds the per-pixel SUM of `spp` samples' own
@@ -352,6 +391,46 @@ class Scene:
         check(lib().xyz_render_backward(self.h, ctypes.byref(cam), ctypes.byref(p),
                                         _dptr(d_film), _dptr(d_texels), _dptr(d_emission),
                                         _stream(stream)))
+
+
+class ImportonMap:
+    """Importons (position, incoming direction) of a scene, on the device with
+    a grid for the 0.1-radius query (reference Scene::CreateLightImportanceMap)."""
+
+    def __init__(self, scene, n_emit=100000, max_depth=16, seed=0, importons=None):
+        self.scene = scene  # the map refers to the scene
+        h = _vp()
+        if importons is None:
+            check(lib().xyz_importon_map_create(scene.h, n_emit, max_depth, seed, ctypes.byref(h)))
+        else:
+            data = _np(importons, np.float32).reshape(-1, 6)
+            check(lib().xyz_importon_map_from_host(scene.h, len(data), _ptr(data, _f32p),
+                                                   ctypes.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().xyz_importon_map_destroy(self.h)
+            self.h = None
+
+    def info(self):
+        i = ImportonInfo()
+        check(lib().xyz_importon_map_info(self.h, ctypes.byref(i)))
+        return i.as_dict()
+
+    def __len__(self):
+        return int(self.info()["n_importons"])
+
+    def export(self):
+        out = np.zeros((len(self), 6), np.float32)
+        check(lib().xyz_importon_map_export(self.h, _ptr(out, _f32p)))
+        return out
+
+    def query(self, basis, cdf, stream=None):
+        """cdf (device float32 (n, 17)) of the importance histogram at each row of
+        basis (device float32 (n, 12): position, shading normal, tangent, binormal)."""
+        check(lib().xyz_importance_query(self.h, basis.shape[0], _dptr(basis), _dptr(cdf),
+                                         _stream(stream)))
 
 
 def make_camera(camera, W, H):

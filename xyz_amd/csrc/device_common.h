@@ -74,7 +74,7 @@ struct DevScene {
   const int32_t* ref_prim;
   const uint32_t* rec_rank;
   const uint32_t* rank_pos;
-  uint32_t stack_cap;  // kd_max_depth + 1 entries of 8 bytes per lane
+  uint32_t stack_cap;  // kd_max_depth + 1 stack entries (one 32-bit word each) per lane
   // shading
   const float4* prim_ng;    // (ng.xyz, bsdf index as bits)
   const float4* prim_n0;    // NULL for flat shading, else n0.xyz

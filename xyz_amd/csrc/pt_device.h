@@ -321,6 +321,7 @@ struct ShadowReq {
   float nk2, nk3;
   float3 pe;         // light power * path power
   float fsn, fres, mx;  // METAL only
+  float gd;          // LAMBERT: the importance histogram's density of w (1 when unguided)
 };
 
 // Contribution of a visible NEE sample (xyz/bsdf/lambert.cpp:157-172,
@@ -328,7 +329,8 @@ struct ShadowReq {
 __device__ __forceinline__ float3 nee_contribution(const ShadowReq& r, float t, float A) {
   if (r.type == XYZ_LAMBERT) {
     float const G = r.nk2 * r.nk3 / (t * t);
-    float const p_imp = float(kInvPi * double(G));
+    // lambert.cpp:161 and, guided, :228-229 (M_1_PI * G * density)
+    float const p_imp = float((kInvPi * double(G)) * double(r.gd));
     float const p_exp = rcpf(A);
     float const dens = p_imp / p_exp;
     double const wgt = (double(G) * (kInvPi * kInvPi)) * double(rcpf(1.0f + dens * dens));
@@ -348,6 +350,149 @@ __device__ __forceinline__ float3 nee_contribution(const ShadowReq& r, float t, 
             float(((((double(r.pe.z) * s1) * fs) / mx) * fr) * rw));
 }
 
+// ------------------------------------------ importance-driven path tracing
+// Importon map on the device (Scene::CreateLightImportanceMap,
+// xyz/core/scene_photonmap.cpp:272-334): positions and incoming directions
+// sorted by the cell of a uniform grid whose cells are at least the search
+// radius wide; cell_start[c] .. cell_start[c + 1] are cell c's importons, with
+// cells numbered x fastest.
+struct DevImportons {
+  const float4* pos;
+  const float4* dir;
+  const uint32_t* cell_start;
+  float lo[3];
+  float inv_cell;
+  float reach;  // half-width of a query's cell range (radius plus a margin)
+  float r2;     // square of the search radius, float(0.1f * 0.1f)
+  uint32_t dim[3];
+  uint32_t n;
+};
+
+constexpr uint32_t kImpU = 2, kImpV = 8, kImpN = 16;  // CONFIG_U_SIZE, CONFIG_V_SIZE
+
+// Octant of the azimuth atan2(y, x) in [0, 2 pi), an edge belonging to the
+// octant above it: ImportanceQuery's v = floor(phi * 8 / pi) with
+// phi = atan2(y, x) / 2 (+ pi when negative) (photonmap.cpp:351-360) in exact
+// arithmetic, found with comparisons instead of atan2.
+__host__ __device__ __forceinline__ uint32_t azimuth_octant(float x, float y) {
+  if (x == 0.0f && y == 0.0f) return 0u;
+  if (y > 0.0f || (y == 0.0f && x > 0.0f)) {
+    if (x > y) return 0u;
+    if (x > 0.0f) return 1u;
+    return (y > -x) ? 2u : 3u;
+  }
+  if (x < 0.0f) return (y > x) ? 4u : 5u;
+  return (y < -x) ? 6u : 7u;
+}
+
+// Histogram bin of a direction with cosine ct to the shading normal and
+// components x = -dot(dir, t), y = -dot(dir, b) (photonmap.cpp:345-360).
+__device__ __forceinline__ uint32_t importance_bin(float ct, float x, float y) {
+  int u = int((ct * ct) * float(kImpU));
+  if (u >= int(kImpU)) u = int(kImpU) - 1;
+  return uint32_t(u) * kImpV + azimuth_octant(x, y);
+}
+
+// Grid cell of coordinate v (host and device: the map is sorted on the host).
+__host__ __device__ __forceinline__ int imp_cell(float v, float lo, float inv, uint32_t dim) {
+  float const f = fminf(fmaxf(floorf((v - lo) * inv), 0.0f), float(dim - 1u));
+  return int(f);
+}
+
+// ImportanceQuery over the importons within the search radius of p
+// (hi::array_kdtree::kNN_query, libhi/hi/sds/array_kdtree.inl:211-243, and
+// ImportanceQuery::push_back, photonmap.cpp:331-363): per-bin counts c.
+// The bins' cdf is cdf[i] = 16 i + c[0] + ... + c[i - 1] (begin_query's 16 per
+// bin, end_query's prefix sums).
+__device__ __forceinline__ void importance_counts(const DevImportons& m, float3 p, const Basis& B,
+                                                  uint32_t c[kImpN]) {
+#pragma unroll
+  for (uint32_t q = 0; q < kImpN; ++q) c[q] = 0u;
+  if (m.n == 0u) return;
+  // the margin covers the rounding of p +- reach for coordinates of any size
+  float const rx = m.reach + fabsf(p.x) * (1.0f / 4194304.0f);
+  float const ry = m.reach + fabsf(p.y) * (1.0f / 4194304.0f);
+  float const rz = m.reach + fabsf(p.z) * (1.0f / 4194304.0f);
+  int const x0 = imp_cell(p.x - rx, m.lo[0], m.inv_cell, m.dim[0]);
+  int const x1 = imp_cell(p.x + rx, m.lo[0], m.inv_cell, m.dim[0]);
+  int const y0 = imp_cell(p.y - ry, m.lo[1], m.inv_cell, m.dim[1]);
+  int const y1 = imp_cell(p.y + ry, m.lo[1], m.inv_cell, m.dim[1]);
+  int const z0 = imp_cell(p.z - rz, m.lo[2], m.inv_cell, m.dim[2]);
+  int const z1 = imp_cell(p.z + rz, m.lo[2], m.inv_cell, m.dim[2]);
+  for (int z = z0; z <= z1; ++z) {
+    for (int y = y0; y <= y1; ++y) {
+      uint32_t const row = (uint32_t(z) * m.dim[1] + uint32_t(y)) * m.dim[0];
+      uint32_t const b0 = m.cell_start[row + uint32_t(x0)];
+      uint32_t const b1 = m.cell_start[row + uint32_t(x1) + 1u];
+      for (uint32_t i = b0; i < b1; ++i) {
+        float4 const P = m.pos[i];
+        float3 const d = f3(P.x - p.x, P.y - p.y, P.z - p.z);
+        if (!(dot3(d, d) < m.r2)) continue;
+        float3 const w = ld3(m.dir[i]);
+        float const ct = -dot3(w, B.ns);
+        if (ct <= 0.0f) continue;
+        float3 const vp = add3(d, scl3(w, dot3(d, B.ns)));
+        if (dot3(vp, vp) >= m.r2) continue;
+        uint32_t const bin = importance_bin(ct, -dot3(w, B.t), -dot3(w, B.b));
+#pragma unroll
+        for (uint32_t q = 0; q < kImpN; ++q) c[q] += (bin == q) ? 1u : 0u;
+      }
+    }
+  }
+}
+
+// ImportanceQuery::GetDensity (photonmap.cpp:430-457): 16 (cdf[n+1] - cdf[n]) / cdf[16].
+__device__ __forceinline__ float importance_density(const uint32_t c[kImpN], float3 dir,
+                                                    const Basis& B) {
+  float const ct = -dot3(dir, B.ns);
+  if (ct <= 0.0f) return 0.0f;
+  uint32_t const n = importance_bin(ct, -dot3(dir, B.t), -dot3(dir, B.b));
+  uint32_t cn = 0u, tot = 0u;
+#pragma unroll
+  for (uint32_t q = 0; q < kImpN; ++q) {
+    cn = (q == n) ? c[q] : cn;
+    tot += c[q];
+  }
+  return float(kImpN) * float(kImpN + cn) / float(kImpN * kImpN + tot);
+}
+
+// Lambert::NextScatteringDirection with importons (lambert.cpp:243-259) and
+// ImportanceQuery::SetDiffusedVector (photonmap.cpp:459-494): a bin from the
+// cdf with u.w, a direction uniform in the bin's cos^2 theta (u.y) and azimuth
+// (u.z) ranges; density cos / pi times the bin's density; the path power
+// divided by the bin's density.
+__device__ __forceinline__ bool guided_scatter(const uint32_t c[kImpN], float4 u, const Basis& B,
+                                               PathState& ps) {
+  uint32_t tot = 0u;
+#pragma unroll
+  for (uint32_t q = 0; q < kImpN; ++q) tot += c[q];
+  float const value = float(kImpN * kImpN + tot) * u.w;
+  // std::upper_bound over cdf[0..16]: n = #{i in 1..16 : cdf[i] <= value}
+  uint32_t n = 0u, pre = 0u, cn = 0u;
+#pragma unroll
+  for (uint32_t i = 1; i <= kImpN; ++i) {
+    pre += c[i - 1];
+    n += (float(kImpN * i + pre) <= value) ? 1u : 0u;
+  }
+  if (n >= kImpN) n = kImpN - 1u;
+#pragma unroll
+  for (uint32_t q = 0; q < kImpN; ++q) cn = (q == n) ? c[q] : cn;
+  uint32_t const bu = n / kImpV, bv = n % kImpV;
+  float const mu_u = (float(bu) + u.y) / float(kImpU);
+  float const mu_v = (float(bv) + u.z) / float(kImpV);
+  float const cos_t = sqrtf(mu_u);
+  float const sin_t = sqrtf(1.0f - mu_u);
+  float const phi = 6.28318530717958647692f * mu_v;
+  float cp, sp;
+  cos_sin(phi, &cp, &sp);
+  ps.dir = add3(add3(scl3(B.t, sin_t * cp), scl3(B.b, sin_t * sp)), scl3(B.ns, cos_t));
+  float const density = float(kImpN) * float(kImpN + cn) / float(kImpN * kImpN + tot);
+  ps.f_next = float(double(density * cos_t) * kInvPi);
+  ps.power = scl3(ps.power, rcpf(density));
+  ps.specular = false;
+  return dot3(ps.dir, B.ng) > 0.0f;
+}
+
 // Gradient hooks of the path-replay adjoint; the primal uses NoGrad.
 struct NoGrad {
   __device__ __forceinline__ void albedo(int32_t, int32_t, float3, float3) {}
@@ -357,11 +502,16 @@ struct NoGrad {
 // One vertex of Scene::EvalutePathTracing (xyz/core/scene_pathtracing.cpp:52-142)
 // after the extension ray found `h`.  Updates the path; emits at most one
 // shadow request.  Returns false when the path ends.
-template <class Grad>
+// kGuide: importance-driven path tracing (Scene::EvaluteImportanceDrivenPathTracing,
+// xyz/core/scene_importancedrivenpathtracing.cpp:11-179) with the importon map
+// imp: a Lambert vertex reached through a specular bounce, other than the first
+// hit, takes its NEE weight and its next direction from the importance histogram
+// around it (:122-152).
+template <class Grad, bool kGuide = false>
 __device__ __forceinline__ bool shade_vertex(const DevScene& s, const PTConst& c, PathState& ps,
                                              const HitRec& h, uint32_t pixel, uint32_t sample,
                                              const LightVertex& lv, ShadowReq* sr, bool* has_sr,
-                                             Grad& grad) {
+                                             Grad& grad, const DevImportons* imp = nullptr) {
   *has_sr = false;
   if (h.prim < 0) return false;  // through the air
   uint32_t bsdf_id;
@@ -394,6 +544,12 @@ __device__ __forceinline__ bool shade_vertex(const DevScene& s, const PTConst& c
   if (++ps.k >= c.kmax) return false;
   uint32_t const j = ps.k - 1;
   float4 const u = draw4(c, pixel, sample, j);
+  bool guided = false;
+  uint32_t hc[kImpN];
+  if constexpr (kGuide) {
+    guided = (b.type == XYZ_LAMBERT) && (ps.k > 2u) && ps.specular;
+    if (guided) importance_counts(*imp, ps.pos, B, hc);
+  }
 
   // NEE: Bsdf::CalculateWeightedDirectLighting
   if (b.type == XYZ_LAMBERT || b.type == XYZ_METAL) {
@@ -420,6 +576,10 @@ __device__ __forceinline__ bool shade_vertex(const DevScene& s, const PTConst& c
         sr->nk3 = nk3;
         sr->pe = mul3(lv.power, ps.power);
         sr->fsn = sr->fres = sr->mx = 0.0f;
+        sr->gd = 1.0f;
+        if constexpr (kGuide) {
+          if (guided) sr->gd = importance_density(hc, w, B);
+        }
         if (b.type == XYZ_METAL) {
           float3 const hv = neg3(normalize3(add3(w, ps.dir)));
           float const hk = -dot3(hv, w);
@@ -447,6 +607,9 @@ __device__ __forceinline__ bool shade_vertex(const DevScene& s, const PTConst& c
   }
 
   // Bsdf::NextScatteringDirection
+  if constexpr (kGuide) {
+    if (guided) return guided_scatter(hc, u, B, ps);
+  }
   if (b.type == XYZ_LAMBERT) {
     // xyz/bsdf/lambert.cpp:176-187, xyz/stdafx.hpp:50-63
     float const phi = 6.28318530717958647692f * u.z;

@@ -54,6 +54,7 @@ struct RJob {
   DevScene s;
   DevCamera cam;
   PTConst c;
+  DevImportons imp;  // importance-driven path tracing only
   float* film;
   unsigned long long base;
   uint32_t wh, sample0;
@@ -95,7 +96,7 @@ struct WaveBufs {
   float4* sw;     // shadow: w.xyz, nk3
   float4* spe;    // shadow: pe.xyz, bits(slot)
   uint4* smeta;   // shadow: expected prim, type, light bsdf, 0
-  float4* smet;   // shadow: fsn, fres, mx, 0
+  float4* smet;   // shadow: fsn, fres, mx, histogram density
   uint32_t* qa;   // 8 * cap
   uint32_t* qb;   // 8 * cap
   WaveCounters* ctr;
@@ -490,7 +491,7 @@ __device__ __forceinline__ uint32_t lds_wave_append(bool pred, uint32_t key, uin
 // positions at a time, which lie in one queue segment (one job), so the
 // workgroup's appends go to that job's queues: counted in LDS, then one global
 // atomic per queue.
-template <bool kReplay>
+template <bool kReplay, bool kGuide>
 __global__ void __launch_bounds__(kBlock)
 k_shade(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBufs wb,
         const uint32_t* __restrict__ qcur, uint32_t* qnext, GradArgs ga) {
@@ -540,7 +541,8 @@ k_shade(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBuf
         cont = shade_vertex(J.s, J.c, ps, h, pixel, sample, lv, &sr, &has_sr, gr);
       } else {
         NoGrad ng;
-        cont = shade_vertex(J.s, J.c, ps, h, pixel, sample, lv, &sr, &has_sr, ng);
+        cont = shade_vertex<NoGrad, kGuide>(J.s, J.c, ps, h, pixel, sample, lv, &sr, &has_sr, ng,
+                                            &J.imp);
       }
       store_state(wb, slot, ps);
       oct = (ps.dir.x < 0.0f ? 1u : 0u) | (ps.dir.y < 0.0f ? 2u : 0u) | (ps.dir.z < 0.0f ? 4u : 0u);
@@ -563,7 +565,7 @@ k_shade(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBuf
       wb.sw[rq] = make_float4(sr.w.x, sr.w.y, sr.w.z, sr.nk3);
       wb.spe[rq] = make_float4(sr.pe.x, sr.pe.y, sr.pe.z, __uint_as_float(slot));
       wb.smeta[rq] = make_uint4(uint32_t(sr.expected), sr.type, sr.light_bsdf, 0u);
-      wb.smet[rq] = make_float4(sr.fsn, sr.fres, sr.mx, 0.0f);
+      wb.smet[rq] = make_float4(sr.fsn, sr.fres, sr.mx, sr.gd);
     }
   }
   if constexpr (kReplay) grad_lds_flush(*glds, ga.d_tex, ga.d_emit, 3 * jobs[wj.jlo].s.n_bsdfs);
@@ -637,6 +639,7 @@ k_shadow(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBu
     sr.fsn = met.x;
     sr.fres = met.y;
     sr.mx = met.z;
+    sr.gd = met.w;
     float3 const add = nee_contribution(sr, h.t, s.light_area);
     uint32_t const slot = __float_as_uint(pe.w);
     if constexpr (kReplay) {
@@ -918,7 +921,7 @@ struct EventSet {
 // Runs the bounce loop of one wave.  kReplay: adjoint replay pass.
 template <bool kReplay>
 int run_bounces(const RJob* jobs, const WaveJobs* wj, WaveBufs wb, uint32_t n, uint32_t depth,
-                uint32_t stack_cap, GradArgs ga, hipStream_t st) {
+                uint32_t stack_cap, GradArgs ga, hipStream_t st, bool guided = false) {
   int const ncu = num_cu();
   size_t const lds = size_t(stack_cap) * kBlock * sizeof(uint32_t);
   uint32_t const trace_grid = std::max(1u, std::min((n + kBlock - 1) / kBlock, uint32_t(ncu) * 8));
@@ -927,8 +930,17 @@ int run_bounces(const RJob* jobs, const WaveJobs* wj, WaveBufs wb, uint32_t n, u
   uint32_t* qn = wb.qb;
   for (uint32_t b = 0; b < depth; ++b) {
     hipLaunchKernelGGL(k_extend, dim3(trace_grid), dim3(kBlock), lds, st, jobs, wj, wb, qc);
-    hipLaunchKernelGGL(k_shade<kReplay>, dim3(shade_grid), dim3(kBlock), 0, st, jobs, wj, wb, qc, qn,
-                       ga);
+    if constexpr (!kReplay) {
+      if (guided)
+        hipLaunchKernelGGL((k_shade<false, true>), dim3(shade_grid), dim3(kBlock), 0, st, jobs, wj,
+                           wb, qc, qn, ga);
+      else
+        hipLaunchKernelGGL((k_shade<false, false>), dim3(shade_grid), dim3(kBlock), 0, st, jobs,
+                           wj, wb, qc, qn, ga);
+    } else {
+      hipLaunchKernelGGL((k_shade<true, false>), dim3(shade_grid), dim3(kBlock), 0, st, jobs, wj,
+                         wb, qc, qn, ga);
+    }
     hipLaunchKernelGGL(k_shadow<kReplay>, dim3(trace_grid), dim3(kBlock), lds, st, jobs, wj, wb, ga);
     hipLaunchKernelGGL(k_prep, dim3(1), dim3(512), 0, st, wj, wb.ctr);
     std::swap(qc, qn);
@@ -953,7 +965,7 @@ struct JobSpec {
 // backward: the adjoint of the single job (ga).  queries: total closest-hit
 // queries (may be NULL; then nothing waits on the device).
 int render_jobs(std::vector<JobSpec> js, hipStream_t st, bool backward, GradArgs ga,
-                unsigned long long* queries) {
+                unsigned long long* queries, bool guided = false) {
   std::vector<JobSpec> jobs;
   for (JobSpec& x : js)
     if (x.items > 0) jobs.push_back(x);
@@ -1040,7 +1052,7 @@ int render_jobs(std::vector<JobSpec> js, hipStream_t st, bool backward, GradArgs
     uint32_t const grid = (n + kBlock - 1) / kBlock;
     hipLaunchKernelGGL(k_init_wave, dim3(1), dim3(512), 0, s, wj, wb[i].ctr);
     hipLaunchKernelGGL(k_camera, dim3(grid), dim3(kBlock), 0, s, jobs_dev, wj, wb[i], n);
-    rc = run_bounces<false>(jobs_dev, wj, wb[i], n, wdepth[k], wstack[k], none, s);
+    rc = run_bounces<false>(jobs_dev, wj, wb[i], n, wdepth[k], wstack[k], none, s, guided);
     if (rc != XYZ_OK) return rc;
     if (!backward) {
       // wave k is added to the films after wave k - 1 (sample order per pixel)
@@ -1085,9 +1097,10 @@ int render_jobs(std::vector<JobSpec> js, hipStream_t st, bool backward, GradArgs
   return XYZ_OK;
 }
 
+// imp: the importon map of importance-driven path tracing, or NULL.
 int render_impl(xyz_scene* sc, const xyz_camera* cam, const xyz_render_params* prm, float* film,
                 xyz_render_stats* stats, hipStream_t st, bool backward, const float* d_film,
-                float* d_tex, float* d_emit) {
+                float* d_tex, float* d_emit, const xyz_importon_map* imp = nullptr) {
   if (!sc) {
     set_error("render: NULL scene");
     return XYZ_E_INVALID;
@@ -1121,6 +1134,19 @@ int render_impl(xyz_scene* sc, const xyz_camera* cam, const xyz_render_params* p
   js.j.sample0 = prm->sample_offset;
   js.items = wh * prm->spp;
   js.max_depth = prm->max_depth;
+  if (imp) {
+    if (imp->scene != sc) {
+      set_error("render_idpt: the importon map belongs to another scene");
+      return XYZ_E_INVALID;
+    }
+    int dev = -1;
+    (void)hipGetDevice(&dev);
+    if (dev != imp->device) {
+      set_error("render_idpt: the importon map lives on another device");
+      return XYZ_E_INVALID;
+    }
+    js.j.imp = imp->dev;
+  }
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (stats) {
     (void)hipEventCreate(&e0);
@@ -1128,7 +1154,8 @@ int render_impl(xyz_scene* sc, const xyz_camera* cam, const xyz_render_params* p
     (void)hipEventRecord(e0, st);
   }
   unsigned long long q = 0;
-  rc = render_jobs({js}, st, backward, GradArgs{d_film, d_tex, d_emit}, stats ? &q : nullptr);
+  rc = render_jobs({js}, st, backward, GradArgs{d_film, d_tex, d_emit}, stats ? &q : nullptr,
+                   imp != nullptr);
   if (rc != XYZ_OK) return rc;
   if (stats) {
     (void)hipEventRecord(e1, st);
@@ -1783,6 +1810,17 @@ int xyz_render(xyz_scene* scene, const xyz_camera* camera, const xyz_render_para
                float* film, xyz_render_stats* stats, void* stream) {
   return render_impl(scene, camera, params, film, stats, static_cast<hipStream_t>(stream), false,
                      nullptr, nullptr, nullptr);
+}
+
+int xyz_render_idpt(xyz_scene* scene, const xyz_importon_map* map, const xyz_camera* camera,
+                    const xyz_render_params* params, float* film, xyz_render_stats* stats,
+                    void* stream) {
+  if (!map) {
+    set_error("render_idpt: NULL importon map");
+    return XYZ_E_INVALID;
+  }
+  return render_impl(scene, camera, params, film, stats, static_cast<hipStream_t>(stream), false,
+                     nullptr, nullptr, nullptr, map);
 }
 
 int xyz_render_bdpt(xyz_scene* scene, const xyz_camera* camera, const xyz_render_params* params,

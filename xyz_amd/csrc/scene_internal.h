@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "device_common.h"
+#include "pt_device.h"
 #include "xyz_internal.h"
 
 struct xyz_scene {
@@ -16,6 +17,18 @@ struct xyz_scene {
   std::vector<void*> allocs;
   uint64_t n_tris = 0;
   uint64_t n_texels = 0;
+};
+
+// Importon map of importance-driven path tracing: device arrays sorted by grid
+// cell, plus the host copy in map order (path order, then vertex order).
+struct xyz_importon_map {
+  xyzhip::DevImportons dev;
+  const xyz_scene* scene = nullptr;
+  int device = -1;
+  std::vector<float> host;  // 6 floats per importon: position, incoming direction
+  std::vector<void*> allocs;
+  uint32_t n_emit = 0;
+  double seconds = 0;
 };
 
 namespace xyzhip {

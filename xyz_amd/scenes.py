@@ -449,3 +449,71 @@ def batch_scene(seed):
     sc["width"] = sc["height"] = res
     sc["spp"] = spp
     return sc
+
+
+# ------------------------------------------- importance-driven path tracing
+def mirror_ceiling_scene(albedo=(0.2, 0.5, 0.8), light_size=0.02, extent=1000.0):
+    """Mirror floor (y = 0, facing up) under a Lambert ceiling (y = 1, facing
+    down) of `albedo`, both 2 * extent wide, and a small square light at
+    y = 0.5 facing down.  Light paths bounce between mirror and ceiling, so the
+    ceiling holds an importon for every visit."""
+    e = extent
+    parts = [quad((-e, 0, -e), (2 * e, 0, 0), (0, 0, 2 * e), (0, 1, 0)),
+             quad((-e, 1, -e), (2 * e, 0, 0), (0, 0, 2 * e), (0, -1, 0)),
+             quad((-light_size / 2, 0.5, -light_size / 2), (light_size, 0, 0),
+                  (0, 0, light_size), (0, -1, 0))]
+    v0, v1, v2 = _pack(parts)
+    bsdfs = [dict(type=MIRROR), dict(type=LAMBERT, rgb=albedo), dict(type=LIGHT, rgb=(5, 5, 5))]
+    return dict(v0=v0, v1=v1, v2=v2, tri_bsdf=np.array([0, 0, 1, 1, 2, 2], np.uint32),
+                bsdfs=bsdfs, light=(-light_size / 2, light_size / 2, 0.5),
+                camera=np.array([0, 0.5, -3, 0, 0.5, 0, 0, 1, 0, 40.0], np.float32))
+
+
+def glass_slab_scene(ior=1.5, extent=100.0):
+    """Square light (0.5 wide, y = 2, facing down) above a glass slab between
+    y = 0.9 and 1.0 (2 * extent wide) over a black Lambert floor at y = 0."""
+    e = extent
+    light = quad((-0.25, 2.0, -0.25), (0.5, 0, 0), (0, 0, 0.5), (0, -1, 0))
+    slab = box((0, 0.95, 0), (2 * e, 0.1, 2 * e))
+    floor = quad((-10 * e, 0, -10 * e), (20 * e, 0, 0), (0, 0, 20 * e), (0, 1, 0))
+    v0, v1, v2 = _pack([light, slab, floor])
+    tb = np.concatenate([np.full(2, 2), np.full(len(slab[0]), 1), np.full(2, 0)]).astype(np.uint32)
+    bsdfs = [dict(type=LAMBERT, rgb=(0.0, 0.0, 0.0)), dict(type=GLASS, ior=ior),
+             dict(type=LIGHT, rgb=(5, 5, 5))]
+    return dict(v0=v0, v1=v1, v2=v2, tri_bsdf=tb, bsdfs=bsdfs,
+                camera=np.array([0, 1.5, -4, 0, 0.5, 0, 0, 1, 0, 40.0], np.float32))
+
+
+def caustic_scene(sphere_level=3):
+    """Caustics: a smooth glass sphere (radius 0.5) and a mirror box on a
+    Lambert floor with a Lambert back wall, under a small bright light and a
+    larger dim one.  The camera sees the floor directly, through the sphere and
+    in the mirror."""
+    v, f = icosphere(sphere_level)
+    sph = displaced_sphere(v, f, 0.0, 0, center=(0, 0.5, 0), radius=0.5)
+    objects = [
+        (quad((-3, 0, -3), (6, 0, 0), (0, 0, 6), (0, 1, 0)), 0),
+        (quad((-3, 0, 2), (6, 0, 0), (0, 3, 0), (0, 0, -1)), 0),
+        (sph, 1),
+        (box((1.1, 0.5, 0.6), (0.1, 1.0, 1.2), rot_y(-30)), 2),
+        (quad((-0.15, 2.2, -0.15), (0.3, 0, 0), (0, 0, 0.3), (0, -1, 0)), 3),
+        (quad((-2.5, 2.8, -1.5), (1.0, 0, 0), (0, 0, 1.0), (0, -1, 0)), 4),
+    ]
+    bsdfs = [dict(type=LAMBERT, rgb=(0.7, 0.65, 0.6)), dict(type=GLASS, ior=1.5),
+             dict(type=MIRROR), dict(type=LIGHT, rgb=(60, 55, 50)),
+             dict(type=LIGHT, rgb=(1.5, 1.5, 1.5))]
+    parts = [o[0] for o in objects]
+    v0, v1, v2 = _pack(parts)
+    tb = np.concatenate([np.full(len(p[0]), m) for p, m in objects]).astype(np.uint32)
+    # smooth normals on the sphere: the radial direction
+    ng = np.cross(v2 - v0, v1 - v0).astype(np.float64)
+    ng /= np.linalg.norm(ng, axis=1, keepdims=True)
+    n0, n1, n2 = ng.copy(), ng.copy(), ng.copy()
+    a = len(objects[0][0][0]) + len(objects[1][0][0])
+    b = a + len(sph[0])
+    for vv, nn in ((v0, n0), (v1, n1), (v2, n2)):
+        r = vv[a:b].astype(np.float64) - np.array([0, 0.5, 0])
+        nn[a:b] = r / np.linalg.norm(r, axis=1, keepdims=True)
+    return dict(v0=v0, v1=v1, v2=v2, tri_bsdf=tb, bsdfs=bsdfs,
+                normals=tuple(x.astype(np.float32) for x in (n0, n1, n2)),
+                camera=np.array([0.0, 1.6, -2.6, 0.0, 0.3, 0.2, 0, 1, 0, 45.0], np.float32))
