@@ -41,6 +41,8 @@ def parse():
                     help="skip the Metasequoia scene load timing")
     ap.add_argument("--skip-bdpt", action="store_true",
                     help="skip the bidirectional path-tracing leg")
+    ap.add_argument("--skip-idpt", action="store_true",
+                    help="skip the importance-driven path-tracing leg")
     ap.add_argument("--pt-spp", type=int, default=16,
                     help="samples per pixel timed in the 1080p leg (the workload renders 256)")
     ap.add_argument("--pt-cpu-rows", type=int, default=128,
@@ -201,6 +203,7 @@ def main():
     kdb = None if args.skip_kd_build else bench_kd_build(capi, mesh, info)
     mqo_leg = None if args.skip_mqo else bench_mqo(capi, scenes)
     bdpt = None if args.skip_bdpt else bench_bdpt(args, hip, capi, scenes)
+    idpt = None if args.skip_idpt else bench_idpt(args, hip, capi, scenes)
     pt = None if args.skip_pt else bench_pt(args, hip, capi, scenes)
     adj = None if args.skip_adjoint else bench_adjoint(args, hip, capi, scenes)
     bat = None if args.skip_batch else bench_batch(args, hip, capi, scenes)
@@ -235,6 +238,7 @@ def main():
         "kd_build": kdb,
         "mqo_load": mqo_leg,
         "bdpt_cornell": bdpt,
+        "idpt_caustic": idpt,
     }
     print(json.dumps(out), flush=True)
 
@@ -343,6 +347,70 @@ def bench_bdpt(args, hip, capi, scenes):
                                                       cores=1, sample="2 spp"))
         out["speedup_vs_cpu"] = round(gpu / cpu, 1)
     log(f"[bench] bdpt: {out}")
+    return out
+
+
+def bench_idpt(args, hip, capi, scenes):
+    """Importance-driven path tracing of the caustic scene at 1920x1080 with the
+    reference's settings (depth 16, Russian roulette at every vertex): the
+    importon map of 100,000 light paths (device vs the oracle on one thread),
+    then 16 spp of xyz_render_idpt against xyz_render on the same frame, and the
+    oracle on a band of rows at 1 spp (checked bit-identical)."""
+    sc = scenes.caustic_scene(sphere_level=4)
+    g = capi.Scene.from_dict(sc)
+    W, H, D, spp, n_emit = 1920, 1080, 15, 16, 100000
+    g.importon_map(1000, max_depth=D, seed=1)  # warm-up
+    builds = [g.importon_map(n_emit, max_depth=D, seed=1) for _ in range(3)]
+    m = builds[-1]
+    info = m.info()
+    build_s = min(b.info()["seconds"] for b in builds)
+    film = hip.DeviceArray.zeros((H, W, 3), np.float32)
+    g.render_idpt(m, sc["camera"], film, 1, D, seed=2, sample_offset=10**6)  # warm-up
+    film.zero()
+    st = g.render_idpt(m, sc["camera"], film, spp, D, seed=2)
+    pfilm = hip.DeviceArray.zeros((H, W, 3), np.float32)
+    pst = g.render(sc["camera"], pfilm, spp, D, seed=2)
+    samples = W * H * spp
+    rate = samples / st["seconds"] / 1e6
+    prate = samples / pst["seconds"] / 1e6
+    out = dict(msamples_per_s=round(rate, 2), ms=round(st["seconds"] * 1e3, 2), spp=spp,
+               width=W, height=H, max_depth=D, n_tris=int(g.info()["n_tris"]),
+               queries_per_sample=round(st["queries"] / samples, 3),
+               pt_same_frame_msamples_per_s=round(prate, 2),
+               guided_cost=round(prate / rate, 3),
+               importons=int(info["n_importons"]), n_emit=n_emit,
+               map_build_ms=round(build_s * 1e3, 2), grid=info["grid"])
+    if not args.no_cpu_baseline:
+        from oracle import oracle as orc
+        o = orc.OracleScene(sc["v0"], sc["v1"], sc["v2"], tri_bsdf=sc["tri_bsdf"], bsdfs=sc["bsdfs"],
+                            normals=sc["normals"])
+        tb = time.time()
+        om = o.importon_map(n_emit, max_depth=D, seed=1)
+        cpu_build = time.time() - tb
+        same_map = bool(np.array_equal(om.export().view(np.uint32), m.export().view(np.uint32)))
+        threads = args.cpu_threads or orc.max_threads()
+        y0 = H // 2 - 64
+        rows = (y0, y0 + 128)
+        ofilm, ost = o.render_idpt(om, sc["camera"], W, H, 2, D, seed=2, rows=rows,
+                                   threads=threads, stats=True)
+        gfilm = hip.DeviceArray.zeros((H, W, 3), np.float32)
+        g.render_idpt(m, sc["camera"], gfilm, 2, D, seed=2)
+        same = bool(np.array_equal(gfilm.numpy()[rows[0]:rows[1]].view(np.uint32),
+                                   ofilm[rows[0]:rows[1]].view(np.uint32)))
+        rows1 = (y0, y0 + 8)
+        _, ost1 = o.render_idpt(om, sc["camera"], W, H, 1, D, seed=2, rows=rows1, threads=1,
+                                stats=True)
+        n = W * (rows[1] - rows[0]) * 2
+        cpu = n / ost["seconds"] / 1e6
+        out["cpu_baseline"] = dict(
+            value=round(cpu, 4), unit="Msamples/s", cores=threads, kind="port",
+            sample=f"rows {rows[0]}-{rows[1] - 1} of the same frame at 2 spp ({n} samples); "
+                   f"GPU film bit-identical on those rows: {same}",
+            single_thread=dict(value=round(W * 8 / ost1["seconds"] / 1e6, 4), cores=1,
+                               sample=f"rows {rows1[0]}-{rows1[1] - 1} at 1 spp"),
+            map_build_s=round(cpu_build, 3), map_build_cores=1, map_identical=same_map)
+        out["vs_cpu"] = round(rate / cpu, 1)
+    log(f"[bench] idpt: {out}")
     return out
 
 

@@ -54,3 +54,16 @@ def test_ren_script_renders_snapshots(capi_mod, device, tmp_path):
     assert spps == sorted(spps) and spps[-1] > spps[0]
     img = film.load_pfm(sh.outputs[-1])
     assert img.shape == (48, 64, 3) and np.isfinite(img).all() and img.mean() > 0
+
+
+@pytest.mark.parametrize("rep,sub", [(1, "IDPT"), (8, "BPT")])
+def test_ren_script_other_integrators(capi_mod, device, tmp_path, rep, sub):
+    from xyz_amd import film, shell
+    sh = shell.Shell(mqo_dir=GOLDEN, out_dir=str(tmp_path), ren_dir=str(tmp_path),
+                     log=lambda *a: None)
+    sh.run(["load cornell_box", "set dim 32 24", "set rt 1 s", "set it 1 s", f"set rep {rep}",
+            "ren"])
+    assert sh.outputs
+    assert all(os.path.basename(os.path.dirname(p)) == sub for p in sh.outputs)
+    img = film.load_pfm(sh.outputs[-1])
+    assert img.shape == (24, 32, 3) and np.isfinite(img).all() and img.mean() > 0

@@ -4,8 +4,12 @@ xyz_render as the renderer.
 
 Commands (one per line; blank lines, "rem ..." lines and =begin/=end blocks
 are skipped):
-    set rep N        integrator (0 = path tracing; the reference's other
-                     integrators are not part of this library)
+    set rep N        integrator: 0 path tracing (PT/), 1 importance-driven path
+                     tracing (IDPT/; the importon map of 100,000 light paths is
+                     built once before the first pass, as ImportanceDrivenPathTracer
+                     does), 8 bidirectional path tracing (BPT/; own and light-
+                     tracing contributions in one film); the reference's other
+                     integrators are not part of this library
     set rt <time>    rendering time, e.g. "1 h 30 min", "10 s" (units d, h, m, s
                      and their long forms, as stream2time)
     set it <time>    interval between film snapshots
@@ -15,7 +19,7 @@ are skipped):
     load NAME        loads <mqo_dir>/NAME.mqo and builds its kd-tree
     save NAME        writes <ren_dir>/NAME.ren reproducing the current settings
     ren              renders until the rendering time is spent, writing
-                     "<out_dir>/<scene> WxH <date>/PT/<hh>h<mm>m<ss>s <spp>spp
+                     "<out_dir>/<scene> WxH <date>/<PT|IDPT|BPT>/<hh>h<mm>m<ss>s <spp>spp
                      <queries>.pfm" at every interval and at the end
     quit / exit / abort
 The reference renders one spp pass over the image at a time and checks the
@@ -35,6 +39,9 @@ from . import mqo as mqo_mod
 
 MAX_DEPTH = 15
 RR_THRESHOLD = 0.2
+N_EMIT = 100000  # CONFIG_EMITED_SIZE (xyz/core/config.hpp:57)
+# set rep N -> output folder (xyz/core/shell.cpp:391-437)
+INTEGRATORS = {0: "PT", 1: "IDPT", 8: "BPT"}
 
 _UNITS = {}
 for _names, _secs in ((("d", "day", "days"), 86400), (("h", "hr", "hour", "hours"), 3600),
@@ -157,13 +164,15 @@ class Shell:
     def render(self):
         if self.scene is None:
             raise RuntimeError("ren: no scene loaded")
-        if self.algorithm != 0:
+        if self.algorithm not in INTEGRATORS:
             raise NotImplementedError(
-                f"ren: integrator {self.algorithm} is not provided (0 = path tracing)")
+                f"ren: integrator {self.algorithm} is not provided (0 = path tracing, "
+                "1 = importance-driven path tracing, 8 = bidirectional path tracing)")
         from . import hip
+        sub = INTEGRATORS[self.algorithm]
         stamp = datetime.datetime.now().strftime("%Y-%m-%dT%H.%M.%S")
         outdir = os.path.join(self.out_dir, f"{self.scene_name} {self.width:04d}x{self.height:04d} "
-                                            f"{stamp} ", "PT")
+                                            f"{stamp} ", sub)
         os.makedirs(outdir, exist_ok=True)
         W, H = self.width, self.height
         cam = [*self.eye, *self.at, 0.0, 1.0, 0.0, 0.0, mqo_mod.FOCAL_LENGTH, mqo_mod.F_NUMBER]
@@ -172,11 +181,23 @@ class Shell:
         batch = 1
         interval = max(1, self.interval_time or self.rendering_time)
         next_save = interval
+        imap = None
+        if sub == "IDPT":
+            imap = self.scene.importon_map(N_EMIT, max_depth=MAX_DEPTH, seed=self.seed)
         t0 = self.clock()
         while True:
             t1 = self.clock()
-            st = self.scene.render(cam, film, batch, MAX_DEPTH, rr_start=0,
-                                   rr_threshold=RR_THRESHOLD, seed=self.seed, sample_offset=spp)
+            if sub == "PT":
+                st = self.scene.render(cam, film, batch, MAX_DEPTH, rr_start=0,
+                                       rr_threshold=RR_THRESHOLD, seed=self.seed,
+                                       sample_offset=spp)
+            elif sub == "IDPT":
+                st = self.scene.render_idpt(imap, cam, film, batch, MAX_DEPTH, rr_start=0,
+                                            rr_threshold=RR_THRESHOLD, seed=self.seed,
+                                            sample_offset=spp)
+            else:
+                st = self.scene.render_bdpt(cam, film, batch, MAX_DEPTH, seed=self.seed,
+                                            sample_offset=spp)
             hip.synchronize()
             spp += batch
             queries += int(st["queries"])
