@@ -106,6 +106,7 @@ class Scene {
  public:
   explicit Scene(const xyz_scene_desc& d) { check(xyz_scene_create(&d, &s_)); }
   ~Scene() {
+    xyz_importon_map_destroy(imap_);
     xyz_scene_destroy(s_);
     if (own_stream_) (void)hipStreamDestroy(stream_);
   }
@@ -165,6 +166,46 @@ class Scene {
     hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
   }
 
+  // Scene::CreateLightImportanceMap (reference CONFIG_EMITED_SIZE = 100000
+  // light paths of at most max_depth + 1 segments); replaces an earlier map.
+  size_t CreateLightImportanceMap(uint32_t n_emit = 100000, uint32_t max_depth = 15,
+                                  uint64_t seed = 0) {
+    xyz_importon_map* m = nullptr;
+    check(xyz_importon_map_create(s_, n_emit, max_depth, seed, &m));
+    xyz_importon_map_destroy(imap_);
+    imap_ = m;
+    xyz_importon_info info;
+    check(xyz_importon_map_info(imap_, &info));
+    return size_t(info.n_importons);
+  }
+
+  // One progressive pass of ImportanceDrivenPathTracer::run (after
+  // CreateLightImportanceMap); film contract of Render.
+  void RenderImportanceDriven(const xyz_camera& cam, const xyz_render_params& prm,
+                              std::vector<float>* film_sum) {
+    if (!imap_) throw std::runtime_error("RenderImportanceDriven: no importon map");
+    size_t const n = size_t(cam.width) * cam.height * 3;
+    film_sum->resize(n, 0.0f);
+    hipStream_t s = stream();
+    d_film_.upload(film_sum->data(), n, s);
+    check(xyz_render_idpt(s_, imap_, &cam, &prm, d_film_.get(), nullptr, s));
+    d_film_.download(film_sum->data(), n, s);
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  }
+
+  // One progressive pass of BidirectionalPathTracer::run: own colours and
+  // light-tracing contributions added to the same film.
+  void RenderBidirectional(const xyz_camera& cam, const xyz_render_params& prm,
+                           std::vector<float>* film_sum) {
+    size_t const n = size_t(cam.width) * cam.height * 3;
+    film_sum->resize(n, 0.0f);
+    hipStream_t s = stream();
+    d_film_.upload(film_sum->data(), n, s);
+    check(xyz_render_bdpt(s_, &cam, &prm, d_film_.get(), nullptr, nullptr, s));
+    d_film_.download(film_sum->data(), n, s);
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  }
+
  private:
   hipStream_t stream() {
     if (!stream_) {
@@ -174,6 +215,7 @@ class Scene {
     return stream_;
   }
   xyz_scene* s_ = nullptr;
+  xyz_importon_map* imap_ = nullptr;
   hipStream_t stream_ = nullptr;
   bool own_stream_ = false;
   DeviceBuffer<float> d_org_, d_dir_, d_t_, d_f_, d_g_, d_film_;

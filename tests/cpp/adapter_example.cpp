@@ -49,5 +49,20 @@ int main(int argc, char** argv) {
   double sum = 0;
   for (float v : film) sum += v;
   std::printf("film sum %g\n", sum);
-  return (sum > 0 && xyz_hip::SaveAsPfm("/tmp/xyz_adapter_example.pfm", 16, 16, film, 4.0f)) ? 0 : 3;
+  if (!(sum > 0 && xyz_hip::SaveAsPfm("/tmp/xyz_adapter_example.pfm", 16, 16, film, 4.0f)))
+    return 3;
+  // no specular surface: no importons, and the importance-driven pass is the
+  // path-tracing pass bit for bit
+  size_t const n_imp = scene.CreateLightImportanceMap(1000, 4, 0);
+  std::vector<float> idpt;
+  scene.RenderImportanceDriven(cam, prm, &idpt);
+  bool const same = std::memcmp(idpt.data(), film.data(), film.size() * sizeof(float)) == 0;
+  std::printf("importons %zu, idpt film equal to pt: %d\n", n_imp, int(same));
+  if (n_imp != 0 || !same) return 4;
+  std::vector<float> bdpt;
+  scene.RenderBidirectional(cam, prm, &bdpt);
+  double bsum = 0;
+  for (float v : bdpt) bsum += v;
+  std::printf("bdpt film sum %g\n", bsum);
+  return bsum > 0 ? 0 : 5;
 }

@@ -181,6 +181,51 @@ def test_emission_glass_slab_fresnel(oracle_mod):
     assert abs(np.mean(imp[:, 4] ** 2) - want) < 0.004
 
 
+def test_emission_metal_bounce_closed_form(oracle_mod):
+    """Light (y = 0.5, facing down) over a square of Metal (shininess 1e6, a
+    near-delta lobe) inside a black Lambert floor, under a mirror ceiling
+    (y = 1).  A light path lands on the metal, survives with probability
+    albedo_G F(cos theta) (the reference's fake Fresnel), bounces off the
+    mirror and lands again further out along the same azimuth; it leaves an
+    importon where it first lands outside the square.  The expected count per
+    path is simulated in numpy over the unfolded straight-line geometry."""
+    a, h_l, h_m, ls, e = 1.0, 0.5, 1.0, 0.02, 50.0
+    parts = [S.quad((-a, 0, -a), (2 * a, 0, 0), (0, 0, 2 * a), (0, 1, 0)),
+             S.quad((-e, -0.001, -e), (2 * e, 0, 0), (0, 0, 2 * e), (0, 1, 0)),
+             S.quad((-e, h_m, -e), (2 * e, 0, 0), (0, 0, 2 * e), (0, -1, 0)),
+             S.quad((-ls / 2, h_l, -ls / 2), (ls, 0, 0), (0, 0, ls), (0, -1, 0))]
+    v0, v1, v2 = S._pack(parts)
+    alb = (0.3, 0.7, 0.5)
+    sc = dict(v0=v0, v1=v1, v2=v2, tri_bsdf=np.array([0, 0, 1, 1, 2, 2, 3, 3], np.uint32),
+              bsdfs=[dict(type=S.METAL, rgb=alb, shininess=1e6),
+                     dict(type=S.LAMBERT, rgb=(0, 0, 0)), dict(type=S.MIRROR),
+                     dict(type=S.LIGHT, rgb=(5, 5, 5))])
+    o = _oscene(oracle_mod, sc)
+    N = 100000
+    imp = o.importon_map(N, max_depth=15, seed=7).export()
+    assert np.all(np.abs(imp[:, 1] + 0.001) < 1e-6) and np.all(imp[:, 4] < 0)
+    rng = np.random.default_rng(0)
+    M = 2000000
+    p = rng.uniform(-ls / 2, ls / 2, (M, 2))
+    c2 = rng.uniform(size=M)
+    c = np.sqrt(c2)
+    phi = rng.uniform(0, 2 * np.pi, M)
+    d = np.stack([np.cos(phi), np.sin(phi)], 1) * (np.sqrt(1 - c2) / c)[:, None]
+    survive = alb[1] * (0.9 + 0.1 * (1 - c) ** 5)
+
+    def inside(x):
+        return (np.abs(x[:, 0]) < a) & (np.abs(x[:, 1]) < a)
+
+    w = inside(p + h_l * d).astype(np.float64)
+    count = np.zeros(M)
+    for k in range(1, 8):  # the importon of landing k is at depth 2k < 16
+        w = w * survive
+        out = ~inside(p + (h_l + 2 * k * h_m) * d)
+        count += np.where(out, w, 0.0)
+        w = np.where(out, 0.0, w)
+    assert abs(len(imp) / N - count.mean()) < 0.0075, (len(imp) / N, count.mean())
+
+
 def test_no_specular_scene_has_no_importons_and_idpt_is_pt(oracle_mod):
     sc = S.cornell_box()
     o = _oscene(oracle_mod, sc)

@@ -43,6 +43,29 @@ def test_mirror_reflects_light_exactly(oracle_mod):
     assert np.allclose(px, [15, 7, 3], rtol=1e-5), px[:3]
 
 
+def test_metal_high_shininess_is_a_fresnel_mirror(oracle_mod):
+    # the 45-degree mirror scene with Metal of shininess 1e6: the lobe is a
+    # near-delta, NEE's MIS weight vanishes and the implicit hit's is ~1, so a
+    # sample carries L_e * albedo * F(cos 45), F the reference's
+    # FakeFresnelTerm 0.9 + 0.1 (1 - c)^5 (xyz/stdafx.hpp:44-47); the lobe's
+    # width leaves about 1e-3 (cos of the outgoing side below the incoming)
+    u = np.array([1, 0, 1]) / math.sqrt(2)
+    metal = S.quad(-u - np.array([0, 1, 0]), 2 * u, (0, 2, 0), (-1, 0, 1))
+    light = S.quad((3, -1, -1), (0, 0, 2), (0, 2, 0), (-1, 0, 0))
+    v0, v1, v2 = _parts(metal, light)
+    bs = np.array([0, 0, 1, 1], np.uint32)
+    alb = np.array([0.9, 0.8, 0.7])
+    bsdfs = [dict(type=METAL, rgb=tuple(alb), shininess=1e6), dict(type=LIGHT, rgb=(15, 7, 3))]
+    sc = oracle_mod.OracleScene(v0, v1, v2, tri_bsdf=bs, bsdfs=bsdfs)
+    cam = [0, 0, -3, 0, 0, 0, 0, 1, 0, 2.0]
+    film = sc.render(cam, 8, 8, 64, max_depth=8, seed=1)
+    px = film.reshape(-1, 3) / 64
+    c = math.cos(math.pi / 4)
+    want = np.array([15, 7, 3]) * alb * (0.9 + 0.1 * (1 - c) ** 5)
+    assert np.all(np.abs(px.mean(0) / want - 1) < 3e-3), (px.mean(0), want)
+    assert np.all(np.abs(px / want - 1) < 0.02)
+
+
 def test_glass_slab_normal_incidence(oracle_mod):
     # slab 0.2 thick between camera and light; F(0) = ((1.5-1)/(1.5+1))^2 = 0.04;
     # infinite internal bounces: T = (1-F)^2 / (1-F^2) = (1-F)/(1+F)
