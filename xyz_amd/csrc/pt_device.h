@@ -441,6 +441,115 @@ __device__ __forceinline__ void importance_counts(const DevImportons& m, float3 
   }
 }
 
+// importance_counts for the lanes of a wave that want it, all lanes working on
+// one query at a time: the query's point and frame are broadcast, the 64 lanes
+// test 64 candidate importons at once and each bin's count grows by the
+// population of a ballot.  A lane near a caustic may own thousands of
+// candidates while its neighbours own none; shared, such a query takes
+// candidates / 64 steps.  The counts are those of importance_counts.  Every
+// lane of the wave must call it (with want false for no query).
+// Bin of candidate importon (P, W) for the query at qp with frame (ns, tt, bb),
+// or kImpN when it is not counted (radius, front and tangent-disc tests).
+__device__ __forceinline__ uint32_t importance_candidate_bin(float4 P, float4 W, float3 qp,
+                                                            float3 ns, float3 tt, float3 bb,
+                                                            float r2) {
+  float3 const d = f3(P.x - qp.x, P.y - qp.y, P.z - qp.z);
+  if (!(dot3(d, d) < r2)) return kImpN;
+  float3 const w = ld3(W);
+  float const ct = -dot3(w, ns);
+  if (ct <= 0.0f) return kImpN;
+  float3 const vp = add3(d, scl3(w, dot3(d, ns)));
+  if (dot3(vp, vp) >= r2) return kImpN;
+  return importance_bin(ct, -dot3(w, tt), -dot3(w, bb));
+}
+
+constexpr uint32_t kImpRows = 16;  // grid rows of a query read at once (4 x 4)
+
+__device__ __forceinline__ float3 shfl3(float3 v, uint32_t lane) {
+  return f3(__shfl(v.x, int(lane)), __shfl(v.y, int(lane)), __shfl(v.z, int(lane)));
+}
+
+__device__ __forceinline__ void wave_importance_counts(const DevImportons& m, bool want, float3 p,
+                                                       const Basis& B, uint32_t c[kImpN]) {
+#pragma unroll
+  for (uint32_t q = 0; q < kImpN; ++q) c[q] = 0u;
+  unsigned long long pending = __ballot(want);
+  if (m.n == 0u) return;
+  uint32_t const lane = __lane_id();
+  while (pending != 0ull) {
+    uint32_t const owner = uint32_t(__builtin_ctzll(pending));
+    pending &= pending - 1ull;
+    float3 const qp = shfl3(p, owner);
+    float3 const ns = shfl3(B.ns, owner), tt = shfl3(B.t, owner), bb = shfl3(B.b, owner);
+    float const rx = m.reach + fabsf(qp.x) * (1.0f / 4194304.0f);
+    float const ry = m.reach + fabsf(qp.y) * (1.0f / 4194304.0f);
+    float const rz = m.reach + fabsf(qp.z) * (1.0f / 4194304.0f);
+    int const x0 = imp_cell(qp.x - rx, m.lo[0], m.inv_cell, m.dim[0]);
+    int const x1 = imp_cell(qp.x + rx, m.lo[0], m.inv_cell, m.dim[0]);
+    int const y0 = imp_cell(qp.y - ry, m.lo[1], m.inv_cell, m.dim[1]);
+    int const y1 = imp_cell(qp.y + ry, m.lo[1], m.inv_cell, m.dim[1]);
+    int const z0 = imp_cell(qp.z - rz, m.lo[2], m.inv_cell, m.dim[2]);
+    int const z1 = imp_cell(qp.z + rz, m.lo[2], m.inv_cell, m.dim[2]);
+    uint32_t tot[kImpN];
+#pragma unroll
+    for (uint32_t q = 0; q < kImpN; ++q) tot[q] = 0u;
+    uint32_t const ny = uint32_t(y1 - y0 + 1), nrows = ny * uint32_t(z1 - z0 + 1);
+    if (nrows <= kImpRows) {
+      // lane r < nrows reads row r's importon range; the rows' candidates are
+      // then taken as one list, 64 at a time, position and direction together
+      uint32_t rb0 = 0u, rlen = 0u;
+      if (lane < nrows) {
+        uint32_t const row = ((uint32_t(z0) + lane / ny) * m.dim[1] + uint32_t(y0) + lane % ny) *
+                             m.dim[0];
+        rb0 = m.cell_start[row + uint32_t(x0)];
+        rlen = m.cell_start[row + uint32_t(x1) + 1u] - rb0;
+      }
+      uint32_t b0s[kImpRows], pre[kImpRows + 1];
+      pre[0] = 0u;
+#pragma unroll
+      for (uint32_t r = 0; r < kImpRows; ++r) {
+        b0s[r] = uint32_t(__builtin_amdgcn_readlane(int(rb0), int(r)));
+        pre[r + 1] = pre[r] + uint32_t(__builtin_amdgcn_readlane(int(rlen), int(r)));
+      }
+      uint32_t const total = pre[kImpRows];
+      for (uint32_t base = 0; base < total; base += 64u) {
+        uint32_t const k = base + lane;
+        uint32_t bin = kImpN;  // no bin
+        if (k < total) {
+          uint32_t i = 0u;
+#pragma unroll
+          for (uint32_t r = 0; r < kImpRows; ++r)
+            i = (k >= pre[r] && k < pre[r + 1]) ? b0s[r] + (k - pre[r]) : i;
+          float4 const P = m.pos[i];
+          float4 const W = m.dir[i];
+          bin = importance_candidate_bin(P, W, qp, ns, tt, bb, m.r2);
+        }
+#pragma unroll
+        for (uint32_t q = 0; q < kImpN; ++q) tot[q] += uint32_t(__popcll(__ballot(bin == q)));
+      }
+    } else {  // a range of many cells (coordinates far beyond the grid's scale)
+      for (int z = z0; z <= z1; ++z) {
+        for (int y = y0; y <= y1; ++y) {
+          uint32_t const row = (uint32_t(z) * m.dim[1] + uint32_t(y)) * m.dim[0];
+          uint32_t const b0 = m.cell_start[row + uint32_t(x0)];
+          uint32_t const b1 = m.cell_start[row + uint32_t(x1) + 1u];
+          for (uint32_t base = b0; base < b1; base += 64u) {
+            uint32_t const i = base + lane;
+            uint32_t bin = kImpN;
+            if (i < b1) bin = importance_candidate_bin(m.pos[i], m.dir[i], qp, ns, tt, bb, m.r2);
+#pragma unroll
+            for (uint32_t q = 0; q < kImpN; ++q) tot[q] += uint32_t(__popcll(__ballot(bin == q)));
+          }
+        }
+      }
+    }
+    if (lane == owner) {
+#pragma unroll
+      for (uint32_t q = 0; q < kImpN; ++q) c[q] = tot[q];
+    }
+  }
+}
+
 // ImportanceQuery::GetDensity (photonmap.cpp:430-457): 16 (cdf[n+1] - cdf[n]) / cdf[16].
 __device__ __forceinline__ float importance_density(const uint32_t c[kImpN], float3 dir,
                                                     const Basis& B) {
@@ -507,15 +616,26 @@ struct NoGrad {
 // imp: a Lambert vertex reached through a specular bounce, other than the first
 // hit, takes its NEE weight and its next direction from the importance histogram
 // around it (:122-152).
-template <class Grad, bool kGuide = false>
-__device__ __forceinline__ bool shade_vertex(const DevScene& s, const PTConst& c, PathState& ps,
-                                             const HitRec& h, uint32_t pixel, uint32_t sample,
-                                             const LightVertex& lv, ShadowReq* sr, bool* has_sr,
-                                             Grad& grad, const DevImportons* imp = nullptr) {
-  *has_sr = false;
-  if (h.prim < 0) return false;  // through the air
+// The vertex is shaded in two steps so that a kernel can run the importance
+// query of its guided lanes between them: shade_begin (hit, basis, emission,
+// depth cap) and shade_end (NEE, Russian roulette, scattering).
+struct VertexCtx {
+  Basis B;
   uint32_t bsdf_id;
-  Basis const B = shading_basis(s, h.prim, h.f, h.g, h.back, &bsdf_id);
+  float4 u;      // the vertex's numbers (stream 0)
+  bool guided;   // importance-driven vertex (kGuide only)
+};
+
+// Returns false when the path ends at this hit.
+template <class Grad, bool kGuide = false>
+__device__ __forceinline__ bool shade_begin(const DevScene& s, const PTConst& c, PathState& ps,
+                                            const HitRec& h, uint32_t pixel, uint32_t sample,
+                                            VertexCtx& x, Grad& grad) {
+  x.guided = false;
+  if (h.prim < 0) return false;  // through the air
+  x.B = shading_basis(s, h.prim, h.f, h.g, h.back, &x.bsdf_id);
+  Basis const& B = x.B;
+  uint32_t const bsdf_id = x.bsdf_id;
   if (-dot3(ps.dir, B.ns) <= 0.0f) return false;
   ps.pos = add3(ps.pos, scl3(ps.dir, h.t));
   DevBsdf const& b = s.bsdfs[bsdf_id];
@@ -542,14 +662,24 @@ __device__ __forceinline__ bool shade_vertex(const DevScene& s, const PTConst& c
     return false;
   }
   if (++ps.k >= c.kmax) return false;
+  x.u = draw4(c, pixel, sample, ps.k - 1);
+  if constexpr (kGuide) x.guided = (b.type == XYZ_LAMBERT) && (ps.k > 2u) && ps.specular;
+  return true;
+}
+
+// hc: the importance histogram's counts when x.guided.  Returns false when the
+// path ends.
+template <class Grad, bool kGuide = false>
+__device__ __forceinline__ bool shade_end(const DevScene& s, const PTConst& c, PathState& ps,
+                                          const HitRec& h, const VertexCtx& x,
+                                          const LightVertex& lv, ShadowReq* sr, bool* has_sr,
+                                          Grad& grad, const uint32_t* hc) {
+  *has_sr = false;
+  Basis const& B = x.B;
+  DevBsdf const& b = s.bsdfs[x.bsdf_id];
   uint32_t const j = ps.k - 1;
-  float4 const u = draw4(c, pixel, sample, j);
-  bool guided = false;
-  uint32_t hc[kImpN];
-  if constexpr (kGuide) {
-    guided = (b.type == XYZ_LAMBERT) && (ps.k > 2u) && ps.specular;
-    if (guided) importance_counts(*imp, ps.pos, B, hc);
-  }
+  float4 const u = x.u;
+  bool const guided = kGuide && x.guided;
 
   // NEE: Bsdf::CalculateWeightedDirectLighting
   if (b.type == XYZ_LAMBERT || b.type == XYZ_METAL) {
@@ -684,6 +814,21 @@ __device__ __forceinline__ bool shade_vertex(const DevScene& s, const PTConst& c
     return true;
   }
   return false;  // LIGHT seen from behind: xyz/bsdf/light.cpp:27-31
+}
+
+template <class Grad, bool kGuide = false>
+__device__ __forceinline__ bool shade_vertex(const DevScene& s, const PTConst& c, PathState& ps,
+                                             const HitRec& h, uint32_t pixel, uint32_t sample,
+                                             const LightVertex& lv, ShadowReq* sr, bool* has_sr,
+                                             Grad& grad, const DevImportons* imp = nullptr) {
+  *has_sr = false;
+  VertexCtx x;
+  if (!shade_begin<Grad, kGuide>(s, c, ps, h, pixel, sample, x, grad)) return false;
+  uint32_t hc[kImpN];
+  if constexpr (kGuide) {
+    if (x.guided) importance_counts(*imp, ps.pos, x.B, hc);
+  }
+  return shade_end<Grad, kGuide>(s, c, ps, h, x, lv, sr, has_sr, grad, hc);
 }
 
 }  // namespace xyzhip

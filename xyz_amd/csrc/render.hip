@@ -518,7 +518,36 @@ k_shade(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBuf
     bool cont = false, has_sr = false;
     uint32_t slot = 0, oct = 0;
     ShadowReq sr;
-    if (active) {
+    if constexpr (kGuide) {
+      // importance-driven: the wave queries the importon grid for its guided
+      // lanes together, between the two shading steps
+      PathState ps{};
+      HitRec h{};
+      LightVertex lv{};
+      VertexCtx x{};
+      uint32_t pixel = 0, sample = 0;
+      bool go = false;
+      NoGrad ng;
+      if (active) {
+        slot = qcur[ext_entry(wj, g, r)];
+        unsigned long long const item = wj.q0 + slot - J.base;
+        pixel = uint32_t(item % J.wh);
+        sample = J.sample0 + uint32_t(item / J.wh);
+        load_state(wb, slot, ps);
+        h = unpack_hit(wb.hit[slot]);
+        float4 const u0 = draw4(J.c, pixel, sample, 0);
+        lv = sample_light(J.s, u0.z, u0.w);
+        go = shade_begin<NoGrad, true>(J.s, J.c, ps, h, pixel, sample, x, ng);
+      }
+      uint32_t hc[kImpN];
+      wave_importance_counts(J.imp, go && x.guided, ps.pos, x.B, hc);
+      if (active) {
+        if (go) cont = shade_end<NoGrad, true>(J.s, J.c, ps, h, x, lv, &sr, &has_sr, ng, hc);
+        store_state(wb, slot, ps);
+        oct = (ps.dir.x < 0.0f ? 1u : 0u) | (ps.dir.y < 0.0f ? 2u : 0u) |
+              (ps.dir.z < 0.0f ? 4u : 0u);
+      }
+    } else if (active) {
       slot = qcur[ext_entry(wj, g, r)];
       unsigned long long const item = wj.q0 + slot - J.base;
       uint32_t const pixel = uint32_t(item % J.wh);
@@ -541,8 +570,7 @@ k_shade(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBuf
         cont = shade_vertex(J.s, J.c, ps, h, pixel, sample, lv, &sr, &has_sr, gr);
       } else {
         NoGrad ng;
-        cont = shade_vertex<NoGrad, kGuide>(J.s, J.c, ps, h, pixel, sample, lv, &sr, &has_sr, ng,
-                                            &J.imp);
+        cont = shade_vertex(J.s, J.c, ps, h, pixel, sample, lv, &sr, &has_sr, ng);
       }
       store_state(wb, slot, ps);
       oct = (ps.dir.x < 0.0f ? 1u : 0u) | (ps.dir.y < 0.0f ? 2u : 0u) | (ps.dir.z < 0.0f ? 4u : 0u);
