@@ -276,7 +276,9 @@ typedef struct {
  * specular bounce after the first segment.  Light paths are traced on the
  * device; random numbers come from Philox stream 4 keyed by (path index,
  * seed).  The importons keep path order, then vertex order, and are indexed
- * by a uniform grid for the fixed-radius query.  Synchronous. */
+ * by a uniform grid for the fixed-radius query.  Synchronous.  n_emit <=
+ * 2^28; max_depth in [1, 255].  A map refers to its scene: destroy it before
+ * the scene. */
 int xyz_importon_map_create(xyz_scene *scene, uint32_t n_emit, uint32_t max_depth, uint64_t seed,
                             xyz_importon_map **out);
 /* A map of caller importons: host float[6*n], position then incoming

@@ -195,3 +195,44 @@ def test_idpt_errors(capi_mod, device):
         capi_mod.ImportonMap(g1, importons=bad)
     empty = g1.importon_map(0, max_depth=8)
     assert len(empty) == 0
+
+
+def _textured_cornell_with_glass():
+    sc = S.cornell_box(textured=True, tex_res=16)
+    block = S.box((0.5, 0.45, 0.35), (0.22, 0.22, 0.22), S.rot_y(30))
+    nb = len(sc["bsdfs"])
+    sc["bsdfs"] = list(sc["bsdfs"]) + [dict(type=S.GLASS, ior=1.5)]
+    sc["v0"] = np.concatenate([sc["v0"], block[0].astype(np.float32)])
+    sc["v1"] = np.concatenate([sc["v1"], block[1].astype(np.float32)])
+    sc["v2"] = np.concatenate([sc["v2"], block[2].astype(np.float32)])
+    sc["tri_bsdf"] = np.concatenate([sc["tri_bsdf"], np.full(len(block[0]), nb, np.uint32)])
+    sc["uv"] = np.concatenate([sc["uv"], np.zeros((len(block[0]), 6), np.float32)])
+    return sc
+
+
+def test_idpt_textured_thin_lens_bit_exact(oracle_mod, capi_mod, device):
+    """Guided vertices on textured walls, through the reference's thin lens."""
+    sc = _textured_cornell_with_glass()
+    o = oracle_mod.OracleScene(sc["v0"], sc["v1"], sc["v2"], tri_bsdf=sc["tri_bsdf"],
+                               bsdfs=sc["bsdfs"], uv=sc["uv"], textures=sc["textures"])
+    g = capi_mod.Scene.from_dict(sc)
+    om = o.importon_map(30000, max_depth=10, seed=5)
+    gm = g.importon_map(30000, max_depth=10, seed=5)
+    assert len(om) > 100
+    _same(gm.export(), om.export(), "importons (textured box)")
+    cam = np.array([0.5, 0.5, -1.4, 0.5, 0.5, 0.5, 0, 1, 0, 40.0, 0.035, 1.4], np.float32)
+    ref = o.render_idpt(om, cam, 40, 40, 8, 10, rr_start=2, seed=3)
+    got, _ = idpt_film(g, gm, cam, 40, 40, 8, 10, rr_start=2, seed=3)
+    _same(got, ref, "idpt film (textured, thin lens)")
+    pt = o.render(cam, 40, 40, 8, 10, rr_start=2, seed=3)
+    assert not np.array_equal(pt, ref)
+
+
+def test_importon_map_limits(capi_mod, device):
+    sc = S.caustic_scene(sphere_level=1)
+    g = capi_mod.Scene.from_dict(sc)
+    with pytest.raises(capi_mod.XyzError, match="n_emit"):
+        g.importon_map((1 << 28) + 1, max_depth=8)
+    far = np.array([[-3e38, 0, 0, 0, -1, 0], [3e38, 0, 0, 0, -1, 0]], np.float32)
+    with pytest.raises(capi_mod.XyzError, match="float range"):
+        capi_mod.ImportonMap(g, importons=far)

@@ -22,6 +22,7 @@ constexpr uint32_t kEmitBlock = 256;
 constexpr uint32_t kImportonStream = 4u;  // Philox stream of the light paths
 constexpr float kSearchRadius = 0.1f;      // CONFIG_SEARCH_RADIUS (xyz/core/config.hpp:56)
 constexpr uint64_t kMaxCells = 1ull << 22;
+constexpr uint32_t kMaxEmit = 1u << 28;  // light paths per map
 
 // Bsdf::BoundsImportance: *dir holds the incoming direction and becomes the
 // outgoing one.  u.x: Russian roulette / Fresnel choice, u.y: glass's cosine
@@ -216,6 +217,11 @@ int build_grid(xyz_importon_map* m) {
       if (i == 0 || v < lo[a]) lo[a] = v;
       if (i == 0 || v > hi[a]) hi[a] = v;
     }
+  for (int a = 0; a < 3; ++a)
+    if (!std::isfinite(hi[a] - lo[a])) {
+      set_error("importon map: importon coordinates span more than the float range");
+      return XYZ_E_INVALID;
+    }
   float cell = kSearchRadius;
   uint64_t dims[3];
   for (;;) {
@@ -381,6 +387,10 @@ int xyz_importon_map_create(xyz_scene* scene, uint32_t n_emit, uint32_t max_dept
   if (max_depth < 1 || max_depth > 255) {
     set_error("xyz_importon_map_create: max_depth must be in [1, 255]");
     return XYZ_E_INVALID;
+  }
+  if (n_emit > kMaxEmit) {
+    set_error("xyz_importon_map_create: n_emit above 2^28");
+    return XYZ_E_UNSUPPORTED;
   }
   auto const t0 = std::chrono::steady_clock::now();
   auto* m = new xyz_importon_map();
