@@ -111,3 +111,18 @@ def test_device_build_workload_mesh_and_scene(capi_mod, device):
     assert (res[0][0] >= 0).mean() > 0.2
     with pytest.raises(capi_mod.XyzError):
         capi_mod.Scene(m["v0"][:10], m["v1"][:10], m["v2"][:10], kd_builder=7)
+
+
+def test_device_build_root_box_signed_zero_ties(capi_mod, device):
+    """200,000 triangles (the root box is accumulated in parts on host threads)
+    whose y coordinates are all +0 or -0: the box's y bounds are ties between
+    the two zeros, which the host builder breaks by triangle order."""
+    rng = np.random.default_rng(7)
+    n = 200000
+    centre = rng.uniform(-1, 1, (n, 1, 3))
+    tri = (centre + rng.uniform(-0.005, 0.005, (n, 3, 3))).astype(np.float32)
+    tri[..., 1] = np.where(rng.uniform(size=(n, 3)) < 0.5, np.float32(0.0), np.float32(-0.0))
+    hi, di = _same_tree(capi_mod, tri[:, 0], tri[:, 1], tri[:, 2])
+    for k in ("bmin", "bmax"):
+        assert np.array_equal(np.array(hi[k], np.float32).view(np.uint32),
+                              np.array(di[k], np.float32).view(np.uint32)), k

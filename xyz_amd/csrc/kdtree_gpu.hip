@@ -26,8 +26,10 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <future>
 #include <limits>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "scene_internal.h"
@@ -419,6 +421,65 @@ __global__ void k_part_events(size_t m, const uint32_t* __restrict__ tt, const f
 // ------------------------------------------------------------------ host
 // Stream-ordered device memory of one build, from the library pool; buffers
 // grow on demand (references multiply where triangles straddle splits).
+// Root box of triangles [i0, i1), accumulated in triangle order as the host
+// builder does: hi::min keeps the later of two equal values and hi::max the
+// earlier, which matters when -0 meets +0.  bad: the first triangle with a
+// non-finite coordinate, or -1.
+struct BoxPart {
+  float mn[3], mx[3];
+  int64_t bad;
+};
+
+BoxPart box_part(const float* v0, const float* v1, const float* v2, uint32_t i0, uint32_t i1) {
+  BoxPart r;
+  for (int k = 0; k < 3; ++k) {
+    r.mn[k] = std::numeric_limits<float>::infinity();
+    r.mx[k] = -std::numeric_limits<float>::infinity();
+  }
+  r.bad = -1;
+  auto fmn = [](float x, float y) { return (x < y) ? x : y; };
+  auto fmx = [](float x, float y) { return (x < y) ? y : x; };
+  for (uint32_t i = i0; i < i1; ++i) {
+    for (int k = 0; k < 3; ++k) {
+      float const a = v0[3 * size_t(i) + k], b = v1[3 * size_t(i) + k], c = v2[3 * size_t(i) + k];
+      if (!std::isfinite(a) || !std::isfinite(b) || !std::isfinite(c)) {
+        r.bad = int64_t(i);
+        return r;
+      }
+      float const p1 = a + (b - a), p2 = a + (c - a);
+      r.mn[k] = fmn(fmn(r.mn[k], a), fmn(p1, p2));
+      r.mx[k] = fmx(fmx(r.mx[k], a), fmx(p1, p2));
+    }
+  }
+  return r;
+}
+
+// Combines the parts in triangle order with the same min / max, which gives
+// the sequential accumulation's bits (later equal values win the minimum,
+// earlier ones the maximum, in both).
+std::string combine_box_parts(std::vector<std::future<BoxPart>>& parts, float bmin[3],
+                              float bmax[3]) {
+  for (int k = 0; k < 3; ++k) {
+    bmin[k] = std::numeric_limits<float>::infinity();
+    bmax[k] = -std::numeric_limits<float>::infinity();
+  }
+  std::string err;
+  for (auto& f : parts) {
+    BoxPart const r = f.get();
+    if (!err.empty()) continue;
+    if (r.bad >= 0) {
+      err = "non-finite vertex coordinate at triangle " + std::to_string(r.bad);
+      continue;
+    }
+    for (int k = 0; k < 3; ++k) {
+      bmin[k] = (bmin[k] < r.mn[k]) ? bmin[k] : r.mn[k];
+      bmax[k] = (bmax[k] < r.mx[k]) ? r.mx[k] : bmax[k];
+    }
+  }
+  parts.clear();
+  return err;
+}
+
 struct Mem {
   hipStream_t st = nullptr;
   hipError_t err = hipSuccess;
@@ -528,35 +589,32 @@ std::string build_flat_tree_device(uint32_t n, const float* v0, const float* v1,
   if (max_depth > 60) return "kd_max_depth must be <= 60";
   if (n >= (1u << 29)) return "too many triangles (limit 2^29)";
   if (n > 0 && (!v0 || !v1 || !v2)) return "NULL vertex array";
-  // validation and the root box, accumulated in triangle order as the host
-  // builder does (hi::min/max pick a side when -0 meets +0)
+  // validation and the root box on host threads, overlapping the uploads and
+  // the root event sorts below
+  uint32_t const parts = (n >= 65536) ? std::max(1u, std::min(16u, std::thread::hardware_concurrency())) : 1u;
+  std::vector<std::future<BoxPart>> box_parts;
+  for (uint32_t q = 0; q < parts; ++q) {
+    uint32_t const i0 = uint32_t(uint64_t(n) * q / parts), i1 = uint32_t(uint64_t(n) * (q + 1) / parts);
+    box_parts.push_back(std::async(std::launch::async, box_part, v0, v1, v2, i0, i1));
+  }
   float bmin[3], bmax[3];
-  for (int k = 0; k < 3; ++k) {
-    bmin[k] = std::numeric_limits<float>::infinity();
-    bmax[k] = -std::numeric_limits<float>::infinity();
-  }
-  auto fmn = [](float x, float y) { return (x < y) ? x : y; };
-  auto fmx = [](float x, float y) { return (x < y) ? y : x; };
-  for (uint32_t i = 0; i < n; ++i) {
+  auto root_box = [&]() -> std::string {
+    std::string const err = combine_box_parts(box_parts, bmin, bmax);
+    if (!err.empty()) return err;
+    for (int k = 0; k < 3; ++k) bmax[k] = bmax[k] + (bmax[k] - bmin[k]) * 1e-5f;
     for (int k = 0; k < 3; ++k) {
-      float const a = v0[3 * size_t(i) + k], b = v1[3 * size_t(i) + k], c = v2[3 * size_t(i) + k];
-      if (!std::isfinite(a) || !std::isfinite(b) || !std::isfinite(c))
-        return "non-finite vertex coordinate at triangle " + std::to_string(i);
-      float const p1 = a + (b - a), p2 = a + (c - a);
-      bmin[k] = fmn(fmn(bmin[k], a), fmn(p1, p2));
-      bmax[k] = fmx(fmx(bmax[k], a), fmx(p1, p2));
+      out->bmin[k] = bmin[k];
+      out->bmax[k] = bmax[k];
     }
-  }
-  for (int k = 0; k < 3; ++k) bmax[k] = bmax[k] + (bmax[k] - bmin[k]) * 1e-5f;
+    return std::string();
+  };
   out->nodes.clear();
   out->leaf_prims.clear();
   out->max_depth_param = max_depth;
-  for (int k = 0; k < 3; ++k) {
-    out->bmin[k] = bmin[k];
-    out->bmax[k] = bmax[k];
-  }
   out->depth = 0;
   if (n <= max_bucket) {  // a single leaf
+    std::string const err = root_box();
+    if (!err.empty()) return err;
     out->nodes = {(n << 2) | 3u, 0u};
     out->leaf_prims.resize(n);
     for (uint32_t i = 0; i < n; ++i) out->leaf_prims[i] = i;
@@ -616,6 +674,11 @@ std::string build_flat_tree_device(uint32_t n, const float* v0, const float* v1,
   LevelBufs lb[2];
   if (!lb[0].reserve_all(mem, 1)) return fail("allocation", mem.err);
   {
+    std::string const err = root_box();
+    if (!err.empty()) {
+      (void)hipStreamSynchronize(st);  // the uploads above read the caller's arrays
+      return err;
+    }
     uint32_t const root[5] = {n, 0u, 0u, 0u, 0u};  // n, eoff, ioff, depth, bfs
     float const box[6] = {bmin[0], bmin[1], bmin[2], bmax[0], bmax[1], bmax[2]};
     e = hipMemcpyAsync(lb[0].n.p, &root[0], 4, hipMemcpyHostToDevice, st);
