@@ -165,6 +165,13 @@ def main():
                                   bytes_per_launch=int(exec_bytes),
                                   tris_tested_per_ray=round(w_tris / n, 2),
                                   boxes_tested_per_ray=round(w_boxes / n, 2)))
+    # compulsory-only floor: the rays in and the results out (32 B per ray) plus
+    # one pass over the scene's device data
+    floor_bytes = 32 * n + int(info["device_bytes"])
+    roofline["floor"] = dict(bytes_per_launch=floor_bytes,
+                             ms_at_peak=round(floor_bytes / (HBM_PEAK_GBS * 1e9) * 1e3, 4),
+                             frac_of_launch=round(floor_bytes / (HBM_PEAK_GBS * 1e9) /
+                                                  (ms_avg * 1e-3), 4))
     pmc = os.path.join(ROOT, "profiles", "closest_hit_pmc.json")
     if os.path.exists(pmc):
         with open(pmc) as fh:
