@@ -38,11 +38,21 @@ struct BdptConst {
   float inv_area;
 };
 
+// A path vertex as the strategies read it once both paths are built (kept in
+// the per-lane vertex arrays, 68 bytes).
 struct BVert {
   int32_t tri;
-  bool back, specular;
-  float3 pos, win, wout, ng, ns, t, b, power;
-  float bsdf_x_ipdf, s_prev, s_next, cos_in_sh, cos_out_geo, g;
+  bool specular;
+  float3 pos, ng, ns, power;
+  float s_prev, s_next, g;
+};
+
+// The rest of the reference's PathVertex, needed only while the path is
+// extended from this vertex: one instance per path, for its last vertex.
+struct BWork {
+  bool back;
+  float3 win, wout, t, b;
+  float bsdf_x_ipdf, cos_in_sh, cos_out_geo;
 };
 
 // The fields a sample can read before writing them (the flags of the dummy
@@ -50,7 +60,7 @@ struct BVert {
 // returned early); every other field is written before it is read.
 __device__ __forceinline__ void bvert_clear(BVert& v) {
   v.tri = -1;
-  v.back = v.specular = false;
+  v.specular = false;
   v.s_prev = v.s_next = 0.0f;
 }
 
@@ -94,12 +104,12 @@ __device__ __forceinline__ void basis_about(float3 n, float3* t, float3* b) {
 
 // Triangle::GeometricBasis (xyz/geom/triangle.cpp:167-192)
 __device__ __forceinline__ void geometric_basis(const DevScene& s, int32_t prim, float f, float g,
-                                                bool back, BVert& v) {
+                                                bool back, BVert& v, BWork& w) {
   uint32_t bsdf;
   Basis const B = shading_basis(s, prim, f, g, back, &bsdf);  // ns, ng as ShadingBasis
   v.ns = B.ns;
   v.ng = B.ng;
-  basis_about(B.ng, &v.t, &v.b);
+  basis_about(B.ng, &w.t, &w.b);
 }
 
 __device__ __forceinline__ float min1(float x) { return (x < 1.0f) ? x : 1.0f; }
@@ -116,41 +126,42 @@ __device__ __forceinline__ float glass_ior_sq(const DevBsdf& b, bool back) {
 
 // Bsdf::LightsToCameraScatteringDirection
 __device__ __forceinline__ bool lights_to_camera(const DevScene& s, const DevBsdf& bs, int32_t prim,
-                                                 float hf, float hg, BVert& v, float4 u) {
+                                                 float hf, float hg, BVert& v, BWork& w,
+                                                 float4 u) {
   switch (bs.type) {
     case XYZ_LAMBERT: {  // lambert.cpp:314-365
       int32_t texel;
       float3 const alb = albedo_at(s, bs, prim, hf, hg, &texel);
-      float const cig = dot3(v.win, v.ng);
-      float const alpha = v.cos_in_sh / cig;
+      float const cig = dot3(w.win, v.ng);
+      float const alpha = w.cos_in_sh / cig;
       float const pp = min1(alb.y), pn = min1(alb.y * alpha);
       v.specular = false;
       v.s_prev = float(kInvPi * double(pp));
       v.s_next = float(kInvPi * double(pn));
       v.power = mul3(v.power, alb);
       v.power = scl3(v.power, alpha);
-      v.bsdf_x_ipdf = rcpf(pn);
+      w.bsdf_x_ipdf = rcpf(pn);
       if (u.z >= pn) return false;
-      v.cos_out_geo = bd_diffused(u.x, u.y, v.t, v.ng, v.b, &v.wout);
-      return dot3(v.wout, v.ns) > 0.0f;
+      w.cos_out_geo = bd_diffused(u.x, u.y, w.t, v.ng, w.b, &w.wout);
+      return dot3(w.wout, v.ns) > 0.0f;
     }
     case XYZ_MIRROR: {  // mirror.cpp:172-203
       v.specular = true;
-      v.wout = neg3(v.win);
-      v.wout = add3(v.wout, scl3l(2.0f * v.cos_in_sh, v.ns));
-      v.cos_out_geo = dot3(v.wout, v.ng);
-      if (v.cos_out_geo <= 0.0f) return false;
-      v.s_prev = rcpf(v.cos_in_sh);
-      v.s_next = rcpf(v.cos_out_geo);
-      float const cig = dot3(v.win, v.ng);
-      v.bsdf_x_ipdf = v.cos_out_geo / cig;
+      w.wout = neg3(w.win);
+      w.wout = add3(w.wout, scl3l(2.0f * w.cos_in_sh, v.ns));
+      w.cos_out_geo = dot3(w.wout, v.ng);
+      if (w.cos_out_geo <= 0.0f) return false;
+      v.s_prev = rcpf(w.cos_in_sh);
+      v.s_next = rcpf(w.cos_out_geo);
+      float const cig = dot3(w.win, v.ng);
+      w.bsdf_x_ipdf = w.cos_out_geo / cig;
       return true;
     }
     case XYZ_GLASS: {  // glass.cpp:239-323
-      float const ior_sq = glass_ior_sq(bs, v.back);
-      float const nk = dot3(v.win, v.ns);
+      float const ior_sq = glass_ior_sq(bs, w.back);
+      float const nk = dot3(w.win, v.ns);
       float const bb = ior_sq + nk * nk - 1.0f;
-      v.wout = neg3(v.win);
+      w.wout = neg3(w.win);
       v.specular = true;
       if (bb > 0.0f) {
         float const g = sqrtf(bb);
@@ -161,42 +172,42 @@ __device__ __forceinline__ bool lights_to_camera(const DevScene& s, const DevBsd
         if (u.z >= fr) {
           v.s_prev = v.s_next = 1.0f - fr;
           float const q = sqrtf(ior_sq);
-          float3 const w = sub3(v.wout, scl3l(gmc, v.ns));
-          v.wout = f3(w.x / q, w.y / q, w.z / q);
-          v.cos_out_geo = -dot3(v.wout, v.ng);
-          if (v.cos_out_geo <= 0.0f) return false;
-          float const cig = dot3(v.win, v.ng);
-          v.bsdf_x_ipdf = v.cos_out_geo / cig;
+          float3 const r = sub3(w.wout, scl3l(gmc, v.ns));
+          w.wout = f3(r.x / q, r.y / q, r.z / q);
+          w.cos_out_geo = -dot3(w.wout, v.ng);
+          if (w.cos_out_geo <= 0.0f) return false;
+          float const cig = dot3(w.win, v.ng);
+          w.bsdf_x_ipdf = w.cos_out_geo / cig;
           return true;
         }
         v.s_prev = v.s_next = fr;
       } else {
         v.s_prev = v.s_next = 1.0f;
       }
-      v.wout = add3(v.wout, scl3l(2.0f * nk, v.ns));
-      v.cos_out_geo = dot3(v.wout, v.ng);
-      if (v.cos_out_geo <= 0.0f) return false;
-      float const cig = dot3(v.win, v.ng);
-      v.bsdf_x_ipdf = v.cos_out_geo / cig;
+      w.wout = add3(w.wout, scl3l(2.0f * nk, v.ns));
+      w.cos_out_geo = dot3(w.wout, v.ng);
+      if (w.cos_out_geo <= 0.0f) return false;
+      float const cig = dot3(w.win, v.ng);
+      w.bsdf_x_ipdf = w.cos_out_geo / cig;
       return true;
     }
     case XYZ_METAL: {  // metal.cpp:227-278
       v.specular = false;
       float3 hv;
-      float const nh = bd_glossy_half(u.x, u.y, bs.shininess, v.t, v.ns, v.b, &hv);
-      float const hk = dot3(hv, v.win);
-      v.wout = sub3(scl3l(2.0f * hk, hv), v.win);
-      v.cos_out_geo = dot3(v.ng, v.wout);
-      if (v.cos_out_geo <= 0.0f) return false;
-      float const cos_sh = dot3(v.ns, v.wout);
+      float const nh = bd_glossy_half(u.x, u.y, bs.shininess, w.t, v.ns, w.b, &hv);
+      float const hk = dot3(hv, w.win);
+      w.wout = sub3(scl3l(2.0f * hk, hv), w.win);
+      w.cos_out_geo = dot3(v.ng, w.wout);
+      if (w.cos_out_geo <= 0.0f) return false;
+      float const cos_sh = dot3(v.ns, w.wout);
       if (cos_sh <= 0.0f) return false;
-      float const cig = dot3(v.ng, v.win);
-      v.bsdf_x_ipdf = fake_fresnel(hk) * v.cos_in_sh / hmaxf(cos_sh, v.cos_in_sh) *
-                      v.cos_out_geo / cig;
+      float const cig = dot3(v.ng, w.win);
+      w.bsdf_x_ipdf = fake_fresnel(hk) * w.cos_in_sh / hmaxf(cos_sh, w.cos_in_sh) *
+                      w.cos_out_geo / cig;
       v.s_prev = v.s_next = float(double(bs.shininess + 1.0f) * (kInvPi / 8) *
                                   double(powf(nh, bs.shininess)) / double(hk));
-      v.s_prev *= rcpf(v.cos_in_sh);
-      v.s_next *= rcpf(v.cos_out_geo);
+      v.s_prev *= rcpf(w.cos_in_sh);
+      v.s_next *= rcpf(w.cos_out_geo);
       return true;
     }
     default:
@@ -206,39 +217,40 @@ __device__ __forceinline__ bool lights_to_camera(const DevScene& s, const DevBsd
 
 // Bsdf::CameraToLightsScatteringDirection
 __device__ __forceinline__ bool camera_to_lights(const DevScene& s, const DevBsdf& bs, int32_t prim,
-                                                 float hf, float hg, BVert& v, float4 u) {
+                                                 float hf, float hg, BVert& v, BWork& w,
+                                                 float4 u) {
   switch (bs.type) {
     case XYZ_LAMBERT: {  // lambert.cpp:380-424
       v.specular = false;
-      v.cos_in_sh = bd_diffused(u.x, u.y, v.t, v.ns, v.b, &v.win);
-      if (dot3(v.win, v.ng) <= 0.0f) return false;
-      float const cog = dot3(v.wout, v.ng);
+      w.cos_in_sh = bd_diffused(u.x, u.y, w.t, v.ns, w.b, &w.win);
+      if (dot3(w.win, v.ng) <= 0.0f) return false;
+      float const cog = dot3(w.wout, v.ng);
       int32_t texel;
       float3 const alb = albedo_at(s, bs, prim, hf, hg, &texel);
-      float const alpha = v.cos_in_sh / cog;
+      float const alpha = w.cos_in_sh / cog;
       float const pp = min1(alb.y), pn = min1(alb.y * alpha);
       v.s_prev = float(kInvPi * double(pp));
       v.s_next = float(kInvPi * double(pn));
       v.power = mul3(v.power, alb);
-      v.bsdf_x_ipdf = rcpf(pp);
+      w.bsdf_x_ipdf = rcpf(pp);
       return !(u.z >= pp);
     }
     case XYZ_MIRROR: {  // mirror.cpp:217-248
       v.specular = true;
-      v.cos_in_sh = dot3(v.wout, v.ns);
-      v.win = neg3(v.wout);
-      v.win = add3(v.win, scl3l(2.0f * v.cos_in_sh, v.ns));
-      if (dot3(v.win, v.ng) > 0.0f) return false;
-      v.s_prev = rcpf(v.cos_in_sh);
-      v.s_next = rcpf(v.cos_out_geo);
-      v.bsdf_x_ipdf = 1.0f;
+      w.cos_in_sh = dot3(w.wout, v.ns);
+      w.win = neg3(w.wout);
+      w.win = add3(w.win, scl3l(2.0f * w.cos_in_sh, v.ns));
+      if (dot3(w.win, v.ng) > 0.0f) return false;
+      v.s_prev = rcpf(w.cos_in_sh);
+      v.s_next = rcpf(w.cos_out_geo);
+      w.bsdf_x_ipdf = 1.0f;
       return true;
     }
     case XYZ_GLASS: {  // glass.cpp:339-405
-      float const ior_sq = glass_ior_sq(bs, v.back);
-      float const nk = dot3(v.wout, v.ns);
+      float const ior_sq = glass_ior_sq(bs, w.back);
+      float const nk = dot3(w.wout, v.ns);
       float const bb = ior_sq + nk * nk - 1.0f;
-      v.win = neg3(v.wout);
+      w.win = neg3(w.wout);
       v.specular = true;
       if (bb > 0.0f) {
         float const g = sqrtf(bb);
@@ -249,38 +261,38 @@ __device__ __forceinline__ bool camera_to_lights(const DevScene& s, const DevBsd
         if (u.z >= fr) {
           v.s_prev = v.s_next = 1.0f - fr;
           float const q = sqrtf(ior_sq);
-          float3 const w = sub3(v.win, scl3l(gmc, v.ns));
-          v.win = f3(w.x / q, w.y / q, w.z / q);
-          v.cos_in_sh = -dot3(v.win, v.ng);
-          if (v.cos_in_sh <= 0.0f) return false;
-          v.bsdf_x_ipdf = 1.0f;
+          float3 const r = sub3(w.win, scl3l(gmc, v.ns));
+          w.win = f3(r.x / q, r.y / q, r.z / q);
+          w.cos_in_sh = -dot3(w.win, v.ng);
+          if (w.cos_in_sh <= 0.0f) return false;
+          w.bsdf_x_ipdf = 1.0f;
           return true;
         }
         v.s_prev = v.s_next = fr;
       } else {
         v.s_prev = v.s_next = 1.0f;
       }
-      v.win = add3(v.win, scl3l(2.0f * nk, v.ns));
-      v.cos_in_sh = dot3(v.win, v.ng);
-      if (v.cos_in_sh <= 0.0f) return false;
-      v.bsdf_x_ipdf = 1.0f;
+      w.win = add3(w.win, scl3l(2.0f * nk, v.ns));
+      w.cos_in_sh = dot3(w.win, v.ng);
+      if (w.cos_in_sh <= 0.0f) return false;
+      w.bsdf_x_ipdf = 1.0f;
       return true;
     }
     case XYZ_METAL: {  // metal.cpp:292-340
       v.specular = false;
       float3 hv;
-      float const nh = bd_glossy_half(u.x, u.y, bs.shininess, v.t, v.ns, v.b, &hv);
-      float const hk = dot3(hv, v.wout);
-      v.win = sub3(scl3l(2.0f * hk, hv), v.wout);
-      if (dot3(v.ng, v.win) <= 0.0f) return false;
-      v.cos_in_sh = dot3(v.ns, v.win);
-      if (v.cos_in_sh <= 0.0f) return false;
-      float const cos_sh = dot3(v.ns, v.wout);
-      v.bsdf_x_ipdf = fake_fresnel(hk) * v.cos_in_sh / hmaxf(cos_sh, v.cos_in_sh);
+      float const nh = bd_glossy_half(u.x, u.y, bs.shininess, w.t, v.ns, w.b, &hv);
+      float const hk = dot3(hv, w.wout);
+      w.win = sub3(scl3l(2.0f * hk, hv), w.wout);
+      if (dot3(v.ng, w.win) <= 0.0f) return false;
+      w.cos_in_sh = dot3(v.ns, w.win);
+      if (w.cos_in_sh <= 0.0f) return false;
+      float const cos_sh = dot3(v.ns, w.wout);
+      w.bsdf_x_ipdf = fake_fresnel(hk) * w.cos_in_sh / hmaxf(cos_sh, w.cos_in_sh);
       v.s_prev = v.s_next = float(double(bs.shininess + 1.0f) * (kInvPi / 8) *
                                   double(powf(nh, bs.shininess)) / double(hk));
-      v.s_prev *= rcpf(v.cos_in_sh);
-      v.s_next *= rcpf(v.cos_out_geo);
+      v.s_prev *= rcpf(w.cos_in_sh);
+      v.s_next *= rcpf(w.cos_out_geo);
       return true;
     }
     default:
@@ -346,7 +358,9 @@ __device__ __forceinline__ float3 bdpt_sample(const DevScene& s, const DevCamera
   };
   float4 const de = draw4(bc.c, pixel, smp, 0u, kBdptStreamEye);
   float4 const dl = draw4(bc.c, pixel, smp, 0u, kBdptStreamLight);
-  // SampleLightPosition and Triangle::Sample's basis
+  // SampleLightPosition and Triangle::Sample's basis.  wl: the working part of
+  // the light path's last vertex.
+  BWork wl;
   {
     LightVertex const lv = sample_light(s, dl.x, dl.y);
     L[1].tri = lv.prim;
@@ -354,7 +368,7 @@ __device__ __forceinline__ float3 bdpt_sample(const DevScene& s, const DevCamera
       L[1].pos = lv.pos;
       L[1].ns = lv.ns;
       L[1].ng = lv.ng;
-      basis_about(lv.ng, &L[1].t, &L[1].b);
+      basis_about(lv.ng, &wl.t, &wl.b);
       L[1].s_next = bc.inv_area;
       L[1].specular = false;
       L[1].power = lv.power;
@@ -365,23 +379,22 @@ __device__ __forceinline__ float3 bdpt_sample(const DevScene& s, const DevCamera
   float3 const eye = f3(cam.eye[0], cam.eye[1], cam.eye[2]);
   float3 const fwd = neg3(f3(cam.back[0], cam.back[1], cam.back[2]));
   E[1].pos = eye;
-  E[1].t = f3(cam.right[0], cam.right[1], cam.right[2]);
-  E[1].b = f3(cam.up[0], cam.up[1], cam.up[2]);
   E[1].ng = E[1].ns = fwd;
   E[1].power = f3(1.f, 1.f, 1.f);
 
   // BuildLightsPath
   if (L[1].tri >= 0) {
     L[0].s_next = bc.inv_area;
-    L[1].cos_in_sh = 1.0f;
-    L[1].cos_out_geo = bd_diffused(dl.z, dl.w, L[1].t, L[1].ng, L[1].b, &L[1].wout);
+    wl.cos_in_sh = 1.0f;
+    wl.cos_out_geo = bd_diffused(dl.z, dl.w, wl.t, L[1].ng, wl.b, &wl.wout);
     L[1].s_prev = 1.0f;
     L[1].s_next = float(kInvPi);
     L[1].g = 1.0f;
-    L[1].bsdf_x_ipdf = 1.0f;
+    wl.bsdf_x_ipdf = 1.0f;
     for (uint32_t k = 1;;) {
       BVert& nw = L[k + 1];
-      HitRec const h = find(L[k].pos, L[k].wout);
+      BWork wn;
+      HitRec const h = find(L[k].pos, wl.wout);
       nw.tri = h.prim;
       if (h.prim < 0) break;
       uint32_t bsdf;
@@ -394,22 +407,24 @@ __device__ __forceinline__ float3 bdpt_sample(const DevScene& s, const DevCamera
         nw.tri = -1;
         break;
       }
-      geometric_basis(s, h.prim, h.f, h.g, h.back, nw);
-      nw.back = h.back;
-      nw.win = neg3(L[k].wout);
-      nw.cos_in_sh = dot3(nw.win, nw.ns);
-      if (nw.cos_in_sh <= 0.0f) {
+      geometric_basis(s, h.prim, h.f, h.g, h.back, nw, wn);
+      wn.back = h.back;
+      wn.win = neg3(wl.wout);
+      wn.cos_in_sh = dot3(wn.win, nw.ns);
+      if (wn.cos_in_sh <= 0.0f) {
         nw.tri = -1;
         break;
       }
-      nw.power = scl3(L[k].power, L[k].bsdf_x_ipdf);
-      nw.pos = add3(scl3(L[k].wout, h.t), L[k].pos);
-      nw.g = L[k].cos_out_geo * nw.cos_in_sh / (h.t * h.t);
+      nw.power = scl3(L[k].power, wl.bsdf_x_ipdf);
+      nw.pos = add3(scl3(wl.wout, h.t), L[k].pos);
+      nw.g = wl.cos_out_geo * wn.cos_in_sh / (h.t * h.t);
+      wl = wn;
       if (++k >= kmax) {
         L[k + 1].tri = -1;
         break;
       }
-      if (!lights_to_camera(s, bs, h.prim, h.f, h.g, L[k], draw4(bc.c, pixel, smp, k, kBdptStreamLight))) {
+      if (!lights_to_camera(s, bs, h.prim, h.f, h.g, L[k], wl,
+                            draw4(bc.c, pixel, smp, k, kBdptStreamLight))) {
         L[k + 1].tri = -1;
         break;
       }
@@ -448,35 +463,38 @@ __device__ __forceinline__ float3 bdpt_sample(const DevScene& s, const DevCamera
     atomicAdd(dst + 2, pw.z * w);
   }
 
-  // BuildTheEyePath
-  E[1].win = primary_dir(cam, px, py, de.x, de.y);
-  E[1].cos_in_sh = dot3(E[1].win, E[1].ns);
-  E[1].cos_out_geo = 1.0f;
-  E[1].s_prev = bc.C / fourth(E[1].cos_in_sh);
+  // BuildTheEyePath.  we: the working part of the eye path's last vertex.
+  BWork we;
+  we.win = primary_dir(cam, px, py, de.x, de.y);
+  we.cos_in_sh = dot3(we.win, E[1].ns);
+  we.cos_out_geo = 1.0f;
+  E[1].s_prev = bc.C / fourth(we.cos_in_sh);
   E[1].s_next = 1.0f;
   E[1].g = 1.0f;
-  E[1].bsdf_x_ipdf = 1.0f;
+  we.bsdf_x_ipdf = 1.0f;
   for (uint32_t k = 1;;) {
     BVert& nw = E[k + 1];
-    HitRec const h = find(E[k].pos, E[k].win);
+    BWork wn;
+    HitRec const h = find(E[k].pos, we.win);
     nw.tri = h.prim;
     if (h.prim < 0) break;
     uint32_t bsdf;
     Basis const B = shading_basis(s, h.prim, h.f, h.g, h.back, &bsdf);
     nw.ns = B.ns;
     nw.ng = B.ng;
-    nw.t = B.t;
-    nw.b = B.b;
-    nw.back = h.back;
-    nw.wout = neg3(E[k].win);
-    if (dot3(nw.wout, nw.ns) <= 0.0f) {
+    wn.t = B.t;
+    wn.b = B.b;
+    wn.back = h.back;
+    wn.wout = neg3(we.win);
+    if (dot3(wn.wout, nw.ns) <= 0.0f) {
       nw.tri = -1;
       break;
     }
-    nw.power = scl3(E[k].power, E[k].bsdf_x_ipdf);
-    nw.pos = add3(scl3(E[k].win, h.t), E[k].pos);
-    nw.cos_out_geo = dot3(nw.wout, nw.ng);
-    nw.g = nw.cos_out_geo * E[k].cos_in_sh / (h.t * h.t);
+    nw.power = scl3(E[k].power, we.bsdf_x_ipdf);
+    nw.pos = add3(scl3(we.win, h.t), E[k].pos);
+    wn.cos_out_geo = dot3(wn.wout, nw.ng);
+    nw.g = wn.cos_out_geo * we.cos_in_sh / (h.t * h.t);
+    we = wn;
     DevBsdf const bs = s.bsdfs[bsdf];
     if (++k >= kmax) {
       E[k + 1].tri = -1;
@@ -486,7 +504,8 @@ __device__ __forceinline__ float3 bdpt_sample(const DevScene& s, const DevCamera
       E[k + 1].tri = -1;
       break;
     }
-    if (!camera_to_lights(s, bs, h.prim, h.f, h.g, E[k], draw4(bc.c, pixel, smp, k, kBdptStreamEye))) {
+    if (!camera_to_lights(s, bs, h.prim, h.f, h.g, E[k], we,
+                          draw4(bc.c, pixel, smp, k, kBdptStreamEye))) {
       E[k + 1].tri = -1;
       break;
     }
