@@ -50,6 +50,12 @@ constexpr uint32_t kSegPad = 256;      // queue segments start at multiples of t
 // One render job -- scene, camera, integrator constants, film -- on the device.
 // Its work items are the global items [base, base + wh * spp); item base + q
 // is sample sample0 + q / wh of pixel q % wh.
+// Workgroups per CU the guided shading kernel is compiled for (4: 4 waves per
+// SIMD, at most 128 VGPRs).
+#ifndef XYZ_GUIDED_SHADE_BLOCKS
+#define XYZ_GUIDED_SHADE_BLOCKS 4
+#endif
+
 struct RJob {
   DevScene s;
   DevCamera cam;
@@ -492,7 +498,7 @@ __device__ __forceinline__ uint32_t lds_wave_append(bool pred, uint32_t key, uin
 // workgroup's appends go to that job's queues: counted in LDS, then one global
 // atomic per queue.
 template <bool kReplay, bool kGuide>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock, kGuide ? XYZ_GUIDED_SHADE_BLOCKS : 1)
 k_shade(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBufs wb,
         const uint32_t* __restrict__ qcur, uint32_t* qnext, GradArgs ga) {
   __shared__ uint32_t vst[kMaxWaveJobs * 8 + 1];
