@@ -1,7 +1,7 @@
 """Pins the oracle's geometry (triangle test, kd build, kd traversal) without the
-reference binary, which cannot be built here: closed forms,
-hand-derived corner cases of the reference semantics, brute force, and an
-independent counting formulation of the SAH split choice."""
+reference binary (its sources compile only with Windows headers that this image
+lacks): closed forms, hand-derived corner cases of the reference semantics,
+brute force, and an independent counting formulation of the SAH split choice."""
 import numpy as np
 import pytest
 
