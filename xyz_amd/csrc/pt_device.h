@@ -442,12 +442,14 @@ __device__ __forceinline__ void importance_counts(const DevImportons& m, float3 
 }
 
 // importance_counts for the lanes of a wave that want it, all lanes working on
-// one query at a time: the query's point and frame are broadcast, the 64 lanes
-// test 64 candidate importons at once and each bin's count grows by the
-// population of a ballot.  A lane near a caustic may own thousands of
-// candidates while its neighbours own none; shared, such a query takes
-// candidates / 64 steps.  The counts are those of importance_counts.  Every
-// lane of the wave must call it (with want false for no query).
+// one group of queries at a time: the first pending query and every pending
+// query whose grid cells are the same.  The group's candidate importons are
+// loaded 64 at a time, once, and tested against each member query (its point
+// and frame broadcast); a bin's count grows by the population of a ballot.  A
+// lane near a caustic may own thousands of candidates while its neighbours
+// own none; shared, such a query takes candidates / 64 steps, and neighbouring
+// queries share their loads.  The counts are those of importance_counts.
+// Every lane of the wave must call it (with want false for no query).
 // Bin of candidate importon (P, W) for the query at qp with frame (ns, tt, bb),
 // or kImpN when it is not counted (radius, front and tangent-disc tests).
 __device__ __forceinline__ uint32_t importance_candidate_bin(float4 P, float4 W, float3 qp,
@@ -469,6 +471,37 @@ __device__ __forceinline__ float3 shfl3(float3 v, uint32_t lane) {
   return f3(__shfl(v.x, int(lane)), __shfl(v.y, int(lane)), __shfl(v.z, int(lane)));
 }
 
+__device__ __forceinline__ void imp_range(const DevImportons& m, float3 p, int r[6]) {
+  // the margin covers the rounding of p +- reach for coordinates of any size
+  float const rx = m.reach + fabsf(p.x) * (1.0f / 4194304.0f);
+  float const ry = m.reach + fabsf(p.y) * (1.0f / 4194304.0f);
+  float const rz = m.reach + fabsf(p.z) * (1.0f / 4194304.0f);
+  r[0] = imp_cell(p.x - rx, m.lo[0], m.inv_cell, m.dim[0]);
+  r[1] = imp_cell(p.x + rx, m.lo[0], m.inv_cell, m.dim[0]);
+  r[2] = imp_cell(p.y - ry, m.lo[1], m.inv_cell, m.dim[1]);
+  r[3] = imp_cell(p.y + ry, m.lo[1], m.inv_cell, m.dim[1]);
+  r[4] = imp_cell(p.z - rz, m.lo[2], m.inv_cell, m.dim[2]);
+  r[5] = imp_cell(p.z + rz, m.lo[2], m.inv_cell, m.dim[2]);
+}
+
+// Adds, for every query lane of `members`, the bins of the candidates this lane
+// holds (bin kImpN: none) to that lane's counts.
+__device__ __forceinline__ void add_member_bins(unsigned long long members, float4 P, float4 W,
+                                                bool have, float3 p, const Basis& B, float r2,
+                                                uint32_t lane, uint32_t c[kImpN]) {
+  for (unsigned long long mm = members; mm != 0ull; mm &= mm - 1ull) {
+    uint32_t const q = uint32_t(__builtin_ctzll(mm));
+    float3 const qp = shfl3(p, q);
+    float3 const ns = shfl3(B.ns, q), tt = shfl3(B.t, q), bb = shfl3(B.b, q);
+    uint32_t const bin = have ? importance_candidate_bin(P, W, qp, ns, tt, bb, r2) : kImpN;
+#pragma unroll
+    for (uint32_t k = 0; k < kImpN; ++k) {
+      uint32_t const n = uint32_t(__popcll(__ballot(bin == k)));
+      c[k] += (lane == q) ? n : 0u;
+    }
+  }
+}
+
 __device__ __forceinline__ void wave_importance_counts(const DevImportons& m, bool want, float3 p,
                                                        const Basis& B, uint32_t c[kImpN]) {
 #pragma unroll
@@ -476,27 +509,25 @@ __device__ __forceinline__ void wave_importance_counts(const DevImportons& m, bo
   unsigned long long pending = __ballot(want);
   if (m.n == 0u) return;
   uint32_t const lane = __lane_id();
+  int rng[6];
+  imp_range(m, p, rng);
   while (pending != 0ull) {
+    // the first pending query and every pending query with the same cells
     uint32_t const owner = uint32_t(__builtin_ctzll(pending));
-    pending &= pending - 1ull;
-    float3 const qp = shfl3(p, owner);
-    float3 const ns = shfl3(B.ns, owner), tt = shfl3(B.t, owner), bb = shfl3(B.b, owner);
-    float const rx = m.reach + fabsf(qp.x) * (1.0f / 4194304.0f);
-    float const ry = m.reach + fabsf(qp.y) * (1.0f / 4194304.0f);
-    float const rz = m.reach + fabsf(qp.z) * (1.0f / 4194304.0f);
-    int const x0 = imp_cell(qp.x - rx, m.lo[0], m.inv_cell, m.dim[0]);
-    int const x1 = imp_cell(qp.x + rx, m.lo[0], m.inv_cell, m.dim[0]);
-    int const y0 = imp_cell(qp.y - ry, m.lo[1], m.inv_cell, m.dim[1]);
-    int const y1 = imp_cell(qp.y + ry, m.lo[1], m.inv_cell, m.dim[1]);
-    int const z0 = imp_cell(qp.z - rz, m.lo[2], m.inv_cell, m.dim[2]);
-    int const z1 = imp_cell(qp.z + rz, m.lo[2], m.inv_cell, m.dim[2]);
-    uint32_t tot[kImpN];
+    int o[6];
 #pragma unroll
-    for (uint32_t q = 0; q < kImpN; ++q) tot[q] = 0u;
+    for (int a = 0; a < 6; ++a) o[a] = __builtin_amdgcn_readlane(rng[a], int(owner));
+    bool same = ((pending >> lane) & 1ull) != 0ull;
+#pragma unroll
+    for (int a = 0; a < 6; ++a) same = same && (rng[a] == o[a]);
+    unsigned long long const members = __ballot(same) | (1ull << owner);
+    pending &= ~members;
+    int const x0 = o[0], x1 = o[1], y0 = o[2], y1 = o[3], z0 = o[4], z1 = o[5];
     uint32_t const ny = uint32_t(y1 - y0 + 1), nrows = ny * uint32_t(z1 - z0 + 1);
     if (nrows <= kImpRows) {
       // lane r < nrows reads row r's importon range; the rows' candidates are
-      // then taken as one list, 64 at a time, position and direction together
+      // then taken as one list, 64 at a time, position and direction together,
+      // and tested against every member query
       uint32_t rb0 = 0u, rlen = 0u;
       if (lane < nrows) {
         uint32_t const row = ((uint32_t(z0) + lane / ny) * m.dim[1] + uint32_t(y0) + lane % ny) *
@@ -514,18 +545,17 @@ __device__ __forceinline__ void wave_importance_counts(const DevImportons& m, bo
       uint32_t const total = pre[kImpRows];
       for (uint32_t base = 0; base < total; base += 64u) {
         uint32_t const k = base + lane;
-        uint32_t bin = kImpN;  // no bin
-        if (k < total) {
+        bool const have = k < total;
+        float4 P = make_float4(0.f, 0.f, 0.f, 0.f), W = P;
+        if (have) {
           uint32_t i = 0u;
 #pragma unroll
           for (uint32_t r = 0; r < kImpRows; ++r)
             i = (k >= pre[r] && k < pre[r + 1]) ? b0s[r] + (k - pre[r]) : i;
-          float4 const P = m.pos[i];
-          float4 const W = m.dir[i];
-          bin = importance_candidate_bin(P, W, qp, ns, tt, bb, m.r2);
+          P = m.pos[i];
+          W = m.dir[i];
         }
-#pragma unroll
-        for (uint32_t q = 0; q < kImpN; ++q) tot[q] += uint32_t(__popcll(__ballot(bin == q)));
+        add_member_bins(members, P, W, have, p, B, m.r2, lane, c);
       }
     } else {  // a range of many cells (coordinates far beyond the grid's scale)
       for (int z = z0; z <= z1; ++z) {
@@ -535,17 +565,16 @@ __device__ __forceinline__ void wave_importance_counts(const DevImportons& m, bo
           uint32_t const b1 = m.cell_start[row + uint32_t(x1) + 1u];
           for (uint32_t base = b0; base < b1; base += 64u) {
             uint32_t const i = base + lane;
-            uint32_t bin = kImpN;
-            if (i < b1) bin = importance_candidate_bin(m.pos[i], m.dir[i], qp, ns, tt, bb, m.r2);
-#pragma unroll
-            for (uint32_t q = 0; q < kImpN; ++q) tot[q] += uint32_t(__popcll(__ballot(bin == q)));
+            bool const have = i < b1;
+            float4 P = make_float4(0.f, 0.f, 0.f, 0.f), W = P;
+            if (have) {
+              P = m.pos[i];
+              W = m.dir[i];
+            }
+            add_member_bins(members, P, W, have, p, B, m.r2, lane, c);
           }
         }
       }
-    }
-    if (lane == owner) {
-#pragma unroll
-      for (uint32_t q = 0; q < kImpN; ++q) c[q] = tot[q];
     }
   }
 }
