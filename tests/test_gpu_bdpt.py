@@ -60,6 +60,19 @@ def test_glass_mirror_two_lights_matches_oracle(oracle_mod, capi_mod, device, sm
     assert_films(own, spl, ref_own, ref_spl)
 
 
+@pytest.mark.parametrize("camera", ["camera", "mirror_camera"])
+def test_mirror_wall_matches_oracle(oracle_mod, capi_mod, device, camera):
+    """Caustics through a mirror by light tracing, and eye subpaths that end
+    at the mirror as the reference's do."""
+    sc = S.mirror_wall_scene("mirror")
+    o, g = scenes_of(oracle_mod, capi_mod, sc)
+    print(f"[gpu bdpt] mirror wall {camera} 48x36 16 spp D 8", flush=True)
+    ref_own, ref_spl = o.render_bdpt(sc[camera], 48, 36, 16, 8, seed=5)
+    own, spl, _ = gpu_bdpt(g, sc[camera], 48, 36, 16, 8, seed=5)
+    assert_films(own, spl, ref_own, ref_spl)
+    assert spl.sum() > 0.1 * own.sum()
+
+
 def test_mqo_scene_with_metal_statistical(oracle_mod, capi_mod, device):
     from xyz_amd import mqo
     m = mqo.load(os.path.join(ROOT, "tests", "golden", "cornell_box.mqo"))

@@ -9,9 +9,13 @@ densities, so for a scene that has only such paths (a floor under an area
 light) the bidirectional image must equal the path tracer's, which is pinned
 by the analytic direct-lighting checks of test_oracle_pt.py.  This fixes the
 camera's importance constant, the film mapping of light tracing, the flux of
-the light sample and the three weights.  Longer paths use the reference's
-weights as it computes them (a connection's density stays in the path vertex
-for the strategies after it), which do not sum to one: there the image is the
+the light sample and the three weights.  Paths whose only Lambert vertex
+sits between the camera and a mirror in front of the light (three segments)
+are made by light tracing alone, and must again equal the path tracer's
+image: this pins light tracing through a specular vertex and its weight.
+Paths with two or more Lambert vertices use the reference's weights as it
+computes them (a connection's density stays in the path vertex for the
+strategies after it), which do not sum to one: there the image is the
 reference's estimator, not the path tracer's, and its parity is unpinned."""
 import numpy as np
 import pytest
@@ -77,3 +81,66 @@ def test_repeatable_and_sample_ranges(oracle_mod):
     assert not np.array_equal(d[0], e[0])
     with pytest.raises(ValueError):
         o.render_bdpt(sc["camera"], 4, 4, 1, 17, seed=1)
+
+
+def _oracle_scene(oracle_mod, sc):
+    return oracle_mod.OracleScene(sc["v0"], sc["v1"], sc["v2"], tri_bsdf=sc["tri_bsdf"],
+                                  bsdfs=sc["bsdfs"])
+
+
+def test_three_segment_caustic_equals_path_tracing(oracle_mod):
+    """A floor lit by a light and by its image in a mirror wall: camera ->
+    floor -> mirror -> light paths (three segments) reach the film only by
+    light tracing (t = 1), whose weight counts the eye path that would hit the
+    light through the mirror.  No path has a second Lambert vertex, so over the
+    pixels that do not see the wall the image must equal the path tracer's,
+    caustic included; a caustic weighted wrongly, or lost, shows as a bias."""
+    from xyz_amd import scenes as S
+    W, H, spp, chunks = 24, 18, 64, 24
+    sc = S.mirror_wall_scene("mirror")
+    cam = sc["camera"]
+    o = _oracle_scene(oracle_mod, sc)
+    # pixels whose primary rays may see the wall are left out
+    wall = _oracle_scene(oracle_mod, S.mirror_wall_scene("light", (0, 0, 0)))
+    seen = wall.render(cam, W, H, 256, 8, rr_start=100, seed=1).sum(-1) > 0
+    keep = (~seen).astype(np.float64)[None, :, :, None]
+    assert 0 < seen.sum() < 0.1 * W * H
+
+    def blocks(x):  # 3 x 4 blocks of the kept pixels, per channel
+        return (x * keep).reshape(x.shape[0], 3, H // 3, 4, W // 4, 3).sum((2, 4))
+    bd = np.stack([sum(o.render_bdpt(cam, W, H, spp, 8, seed=3, sample0=k * spp)) / spp
+                   for k in range(chunks)]).astype(np.float64)
+    pt = np.stack([o.render(cam, W, H, spp, 8, rr_start=100, seed=4, sample0=k * spp) / spp
+                   for k in range(chunks)]).astype(np.float64)
+    a, b = blocks(bd), blocks(pt)
+    z = (a.mean(0) - b.mean(0)) / np.sqrt((a.var(0, ddof=1) + b.var(0, ddof=1)) / chunks)
+    ta, tb = a.sum((1, 2)), b.sum((1, 2))
+    print(f"max |z| over blocks {np.abs(z).max():.2f}; totals {ta.mean(0)} vs {tb.mean(0)}")
+    assert np.abs(z).max() < 4.5
+    assert np.all(np.abs(ta.mean(0) / tb.mean(0) - 1) < 0.02)
+    # the caustic carries a quarter of those pixels' light
+    black = _oracle_scene(oracle_mod, S.mirror_wall_scene("black"))
+    no_caustic = (black.render(cam, W, H, 1024, 8, rr_start=100, seed=4) * keep[0]).sum()
+    full = (o.render(cam, W, H, 1024, 8, rr_start=100, seed=4) * keep[0]).sum()
+    assert 1 - no_caustic / full > 0.2
+
+
+def test_eye_paths_end_at_mirrors(oracle_mod):
+    """The reference's Mirror::CameraToLightsScatteringDirection rejects the
+    reflected direction when it lies on the front of the geometric normal
+    (xyz/bsdf/mirror.cpp:234-239), which every reflection off the front does:
+    its eye subpaths end at mirrors, and with a black floor nothing seen in the
+    mirror reaches the film, while the path tracer shows the light there."""
+    from xyz_amd import scenes as S
+    W, H = 24, 18
+    sc = S.mirror_wall_scene("mirror", (0, 0, 0))
+    cam = sc["mirror_camera"]
+    o = _oracle_scene(oracle_mod, sc)
+    pt = o.render(cam, W, H, 64, 8, rr_start=100, seed=1).sum(-1)
+    direct = _oracle_scene(oracle_mod, S.mirror_wall_scene("black", (0, 0, 0)))
+    pt_direct = direct.render(cam, W, H, 64, 8, rr_start=100, seed=1).sum(-1)
+    reflected = (pt > 0) & (pt_direct == 0)
+    assert reflected.sum() >= 5
+    own, spl = o.render_bdpt(cam, W, H, 64, 8, seed=1)
+    assert not (own + spl).sum(-1)[reflected].any()
+    assert np.array_equal(own.sum(-1) > 0, pt_direct > 0)

@@ -469,6 +469,31 @@ def mirror_ceiling_scene(albedo=(0.2, 0.5, 0.8), light_size=0.02, extent=1000.0)
                 camera=np.array([0, 0.5, -3, 0, 0.5, 0, 0, 1, 0, 40.0], np.float32))
 
 
+def mirror_wall_scene(wall="mirror", floor_albedo=(0.5, 0.5, 0.5)):
+    """Lambert floor (y = 0, 2 x 2, facing up), a 0.4-wide square light at
+    y = 1 facing down between x = 0.45 and 0.85, and a wall at x = 1 (1.5 high,
+    facing -x) that is a mirror (`wall="mirror"`), black Lambert ("black") or
+    a light ("light", for finding the pixels that see it).
+
+    No path returns to the floor after leaving it upwards (the wall keeps a
+    ray's y direction), so every path from the camera has at most one Lambert
+    vertex.  `camera` sees the floor and a few pixels of the wall;
+    `mirror_camera` sees the light only in the mirror."""
+    parts = [quad((-1, 0, -1), (2, 0, 0), (0, 0, 2), (0, 1, 0)),
+             quad((0.45, 1, -0.2), (0.4, 0, 0), (0, 0, 0.4), (0, -1, 0)),
+             quad((1, 0, -1), (0, 1.5, 0), (0, 0, 2), (-1, 0, 0))]
+    v0, v1, v2 = _pack(parts)
+    light = dict(type=LIGHT, rgb=(5.0, 4.0, 3.0))
+    walls = dict(mirror=dict(type=MIRROR), black=dict(type=LAMBERT, rgb=(0.0, 0.0, 0.0)),
+                 light=light)
+    bsdfs = [dict(type=LAMBERT, rgb=floor_albedo), light, walls[wall]]
+    return dict(v0=v0, v1=v1, v2=v2, tri_bsdf=np.array([0, 0, 1, 1, 2, 2], np.uint32),
+                bsdfs=bsdfs,
+                camera=np.array([0, 1.3, -1.2, 0, 0, 0.3, 0, 1, 0, 40.0], np.float32),
+                mirror_camera=np.array([-0.6, 0.7, -2.2, 0.5, 0.25, 0, 0, 1, 0, 55.0],
+                                       np.float32))
+
+
 def glass_slab_scene(ior=1.5, extent=100.0):
     """Square light (0.5 wide, y = 2, facing down) above a glass slab between
     y = 0.9 and 1.0 (2 * extent wide) over a black Lambert floor at y = 0."""
