@@ -388,17 +388,29 @@ __device__ __forceinline__ void wave_prefix(uint32_t c, uint32_t* excl, uint32_t
   *total = __shfl(incl, 63);
 }
 
-// The lane that owns item k of a wave_prefix distribution: the last lane whose
-// prefix start is <= k.
-__device__ __forceinline__ uint32_t wave_owner(uint32_t excl, uint32_t k) {
-  uint32_t lo = 0, hi = 64;
-#pragma unroll
-  for (int it = 0; it < 6; ++it) {
-    uint32_t const mid = (lo + hi) >> 1;
-    uint32_t const pm = __shfl(excl, mid);
-    if (pm <= k) lo = mid; else hi = mid;
-  }
-  return lo;
+// The lane that owns item base + lane of a wave_prefix distribution (excl, c),
+// for a batch with base < total, found without a search.  `own` is this wave's
+// LDS table of 64 one-byte entries kStride bytes apart (a stride of 8 lets a
+// kernel address it from its 64-bit key array's lane address with a constant
+// offset, which saves a register); it holds 0xFF in every entry on entry and
+// again on return.  Each lane whose items meet the batch marks the batch position of
+// its first item there with its lane id; position 0 is always marked (by the
+// lane holding item `base`), and the owner of position p is the lane that
+// marked the last marked position <= p.  All 64 lanes must call it together.
+template <uint32_t kStride>
+__device__ __forceinline__ uint32_t batch_owner(uint8_t* own, uint32_t excl, uint32_t c,
+                                                uint32_t base) {
+  uint32_t const lane = __lane_id();
+  if (c > 0u && excl < base + 64u && excl + c > base)
+    own[kStride * (excl > base ? excl - base : 0u)] = uint8_t(lane);
+  wave_sync();
+  uint32_t const v = own[kStride * lane];
+  // the marks at positions <= lane, shifted to the top: the leading zeros
+  // count the positions from the last of them up to this lane
+  unsigned long long const marks = __ballot(v != 0xFFu) << (63u - lane);
+  uint32_t const p = lane - uint32_t(__builtin_clzll(marks));
+  own[kStride * lane] = 0xFFu;  // after this lane's read of the same entry
+  return uint32_t(__shfl(int(v), int(p)));
 }
 
 // The current leaf in the reference's order, with the sequential semantics.
@@ -429,11 +441,12 @@ __device__ __forceinline__ float interval_t_max(const DevScene& s, const TraceSt
 }
 
 // Returns true when this lane's (active) query finished in this step.
-// `keys` is this wave's 64-entry LDS array (index = lane).
-template <bool kCount>
+// `keys` is this wave's 64-entry LDS array (index = lane), `own` its table for
+// batch_owner (64 entries kOwnStride bytes apart).
+template <bool kCount, uint32_t kOwnStride = 8>
 __device__ __forceinline__ bool coop_step(const DevScene& s, TraceState& st, bool active,
                                           uint32_t* stk, uint32_t stride,
-                                          unsigned long long* keys) {
+                                          unsigned long long* keys, uint8_t* own) {
   uint32_t const lane = __lane_id();
   if (active && !st.in_leaf) {
     uint2 w = s.nodes[st.node];
@@ -485,10 +498,11 @@ __device__ __forceinline__ bool coop_step(const DevScene& s, TraceState& st, boo
     wave_prefix(ncl, &bexcl, &btotal);
     uint32_t* const masks = reinterpret_cast<uint32_t*>(keys);
     masks[2 * lane] = 0u;
+    own[kOwnStride * lane] = 0xFFu;
     wave_sync();
     for (uint32_t base = 0; base < btotal; base += 64) {
       uint32_t const k = base + lane;
-      uint32_t const owner = wave_owner(bexcl, k);
+      uint32_t const owner = batch_owner<kOwnStride>(own, bexcl, ncl, base);
       float3 const o = make_float3(__shfl(st.o.x, owner), __shfl(st.o.y, owner), __shfl(st.o.z, owner));
       float3 const d = make_float3(__shfl(st.d.x, owner), __shfl(st.d.y, owner), __shfl(st.d.z, owner));
       float const tmin = __shfl(st.t_min, owner);
@@ -535,10 +549,11 @@ __device__ __forceinline__ bool coop_step(const DevScene& s, TraceState& st, boo
   uint32_t excl, total;
   wave_prefix(c, &excl, &total);
   keys[lane] = coop ? st.key : ~0ull;
+  own[kOwnStride * lane] = 0xFFu;
   wave_sync();
   for (uint32_t base = 0; base < total; base += 64) {
     uint32_t const k = base + lane;
-    uint32_t const owner = wave_owner(excl, k);
+    uint32_t const owner = batch_owner<kOwnStride>(own, excl, c, base);
     float3 const o = make_float3(__shfl(st.o.x, owner), __shfl(st.o.y, owner), __shfl(st.o.z, owner));
     float3 const d = make_float3(__shfl(st.d.x, owner), __shfl(st.d.y, owner), __shfl(st.d.z, owner));
     float const tmin = __shfl(st.t_min, owner);

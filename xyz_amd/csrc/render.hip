@@ -417,7 +417,7 @@ k_extend(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBu
   extern __shared__ uint32_t lds_stack[];
   __shared__ uint32_t vst[kMaxWaveJobs * 8 + 1];
   __shared__ uint32_t cnt[kMaxWaveJobs * 8];
-  __shared__ unsigned long long keys[kBlock];
+  __shared__ unsigned long long keys[2 * kBlock];  // keys, then batch_owner's tables
   WaveJobs const& wj = *wjp;
   uint32_t const nseg = wj.nj * 8u;
   for (uint32_t k = threadIdx.x; k <= nseg; k += kBlock) {
@@ -447,7 +447,8 @@ k_extend(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBu
     }
     if (__ballot(slot != 0xFFFFFFFFu) == 0) break;
     bool const act = slot != 0xFFFFFFFFu;
-    bool const fin = coop_step<false>(s, st, act, stk, kBlock, keys + (threadIdx.x & ~63u));
+    bool const fin = coop_step<false>(s, st, act, stk, kBlock, keys + (threadIdx.x & ~63u),
+                                            reinterpret_cast<uint8_t*>(keys + kBlock + (threadIdx.x & ~63u)));
     if (act && fin) {
       HitRec const h = trace_result(s, st);
       wb.hit[slot] = make_float4(__int_as_float(h.prim < 0 ? -1 : (h.prim | (h.back ? 0x80000000 : 0))),
@@ -617,7 +618,7 @@ k_shadow(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBu
     glds = &glds_mem;
     grad_lds_init(glds_mem);
   }
-  __shared__ unsigned long long keys[kBlock];
+  __shared__ unsigned long long keys[2 * kBlock];  // keys, then batch_owner's tables
   WaveJobs const& wj = *wjp;
   if (threadIdx.x < 64) {  // padded prefix of the jobs' request counts
     uint32_t const k = threadIdx.x;
@@ -656,7 +657,8 @@ k_shadow(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBu
     }
     if (__ballot(rq != 0xFFFFFFFFu) == 0) break;
     bool const act = rq != 0xFFFFFFFFu;
-    bool const fin = coop_step<false>(s, st, act, stk, kBlock, keys + (threadIdx.x & ~63u));
+    bool const fin = coop_step<false>(s, st, act, stk, kBlock, keys + (threadIdx.x & ~63u),
+                                            reinterpret_cast<uint8_t*>(keys + kBlock + (threadIdx.x & ~63u)));
     if (!act || !fin) continue;
     HitRec const h = trace_result(s, st);
     uint32_t const rr = rq;
@@ -1228,7 +1230,9 @@ __global__ void k_batch(const BatchJob* __restrict__ jobs, uint32_t n_jobs, uint
                         uint32_t tile, uint32_t* work) {
   extern __shared__ uint32_t lds_stack[];
   __shared__ unsigned long long keys_all[1024];
+  __shared__ uint8_t owners_all[1024];
   unsigned long long* const keys = keys_all + (threadIdx.x & ~63u);
+  uint8_t* const owners = owners_all + (threadIdx.x & ~63u);
   uint32_t* const stk = lds_stack + threadIdx.x;
   uint32_t const lane = __lane_id();
   unsigned long long const below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -1295,7 +1299,7 @@ __global__ void k_batch(const BatchJob* __restrict__ jobs, uint32_t n_jobs, uint
       tile_next += want < avail ? want : avail;
     }
     bool const act = phase != kIdle;
-    bool const fin = coop_step<false>(jb.s, st, act, stk, blockDim.x, keys);
+    bool const fin = coop_step<false, 1>(jb.s, st, act, stk, blockDim.x, keys, owners);
     if (!(act && fin)) continue;
     HitRec const h = trace_result(jb.s, st);
     bool path_done = false;

@@ -87,7 +87,9 @@ closest_hit_kernel(DevScene s, uint32_t n, const float* __restrict__ org,
                    uint8_t* __restrict__ bout, uint32_t* __restrict__ counters,
                    unsigned long long* __restrict__ totals, uint32_t* __restrict__ work) {
   extern __shared__ uint32_t lds_stack[];
-  __shared__ unsigned long long keys[kHitBlock];
+  // per wave: 64 keys, and batch_owner's table in the low bytes of the
+  // block's second half
+  __shared__ unsigned long long keys[2 * kHitBlock];
   uint32_t* const stk = lds_stack + threadIdx.x;
   WaveFeeder feed;
   feed.init();
@@ -106,7 +108,8 @@ closest_hit_kernel(DevScene s, uint32_t n, const float* __restrict__ org,
     if (__ballot(r != 0xFFFFFFFFu) == 0) break;
     bool const act = r != 0xFFFFFFFFu;
 #if XYZ_COOP
-    bool const fin = coop_step<kCount>(s, st, act, stk, kHitBlock, keys + (threadIdx.x & ~63u));
+    bool const fin = coop_step<kCount>(s, st, act, stk, kHitBlock, keys + (threadIdx.x & ~63u),
+                                             reinterpret_cast<uint8_t*>(keys + kHitBlock + (threadIdx.x & ~63u)));
 #else
     bool const fin = act && trace_step<kCount>(s, st, stk, kHitBlock);
 #endif
