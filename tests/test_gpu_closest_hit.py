@@ -29,10 +29,16 @@ def _run_gpu(capi_mod, scene, o, d, counters=True):
 
 
 def _assert_bit_exact(gpu, orc, counters=True):
+    """Bit-identical results.  A ray with NaN components can hit with a NaN t,
+    f or g; IEEE 754 leaves the sign and payload of a NaN result open (x86 and
+    the GPU make different ones), so NaNs are compared as NaN."""
     assert np.array_equal(gpu["prim"], orc["prim"])
     hit = orc["prim"] >= 0
     for k in ("t", "f", "g"):
-        assert np.array_equal(gpu[k][hit].view(np.uint32), orc[k][hit].view(np.uint32)), k
+        a, b = gpu[k][hit], orc[k][hit]
+        nan = np.isnan(b)
+        assert np.array_equal(np.isnan(a), nan), k
+        assert np.array_equal(a[~nan].view(np.uint32), b[~nan].view(np.uint32)), k
     assert np.all(np.isinf(gpu["t"][~hit]))
     assert np.array_equal(gpu["back"][hit], orc["back"][hit])
     if counters:
@@ -89,6 +95,29 @@ def test_small_buckets_deep_tree(oracle_mod, capi_mod, device):
     osc, gsc = _pair(oracle_mod, capi_mod, m["v0"], m["v1"], m["v2"], max_bucket=2, max_depth=24)
     (oc, dc), (oi, di) = S.closest_hit_rays(m, 128, 64, n_incoherent=20000, seed=3)
     for o, d in ((oc, dc), (oi, di)):
+        _assert_bit_exact(_run_gpu(capi_mod, gsc, o, d), osc.closest_hit(o, d, counters=True))
+
+
+@pytest.mark.parametrize("max_depth", [1, 3])
+def test_big_leaves_and_nan_rays(oracle_mod, capi_mod, device, max_depth):
+    """Shallow trees whose leaves hold hundreds of triangles: a leaf then spans
+    several cooperative steps (16 clusters of 16 records each per step), so its
+    best hit is carried from step to step; rays with NaN components take the
+    sequential path."""
+    m = S.closest_hit_mesh(4, 64, seed=5)
+    osc, gsc = _pair(oracle_mod, capi_mod, m["v0"], m["v1"], m["v2"], max_depth=max_depth)
+    biggest = int(osc.tree_dump()["leaf_cnt"].max())
+    assert biggest > 256
+    (oc, dc), (oi, di) = S.closest_hit_rays(m, 96, 64, n_incoherent=8000, seed=4)
+    rng = np.random.default_rng(6)
+    on = oi[:64].copy()
+    dn = di[:64].copy()
+    on[:16, rng.integers(0, 3, 16)] = np.nan
+    dn[16:32, rng.integers(0, 3, 16)] = np.nan
+    dn[32:48] = 0.0
+    print(f"[gpu closest hit] max_depth {max_depth}, {len(m['v0'])} triangles, "
+          f"largest leaf {biggest}", flush=True)
+    for o, d in ((oc, dc), (oi, di), (on, dn)):
         _assert_bit_exact(_run_gpu(capi_mod, gsc, o, d), osc.closest_hit(o, d, counters=True))
 
 
