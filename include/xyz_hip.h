@@ -185,6 +185,15 @@ int xyz_closest_hit(xyz_scene *scene, uint32_t n, const float *org, const float 
 int xyz_closest_hit_work(xyz_scene *scene, uint32_t n, const float *org, const float *dir,
                          int32_t *prim, float *t, unsigned long long *work, void *stream);
 
+/* Self-check of the device arithmetic under xyz_closest_hit (no counterpart in
+ * the reference): the triangle test takes 1/det (xyz/geom/triangle.cpp:139) from the
+ * hardware reciprocal and one Newton step when 1e-6 <= |det| < 2^126 (below
+ * 1e-6 it rejects the triangle whatever the value).  This counts, on the
+ * current device, the binary32 values of that range for which the result
+ * differs from the correctly rounded IEEE quotient, into *mismatches (host);
+ * 0 means the closest hit is exact on this device.  Synchronous. */
+int xyz_check_reciprocal(unsigned long long *mismatches, void *stream);
+
 /* ---- path tracing -------------------------------------------------------------
  * Camera (reference ThinLensCamera, xyz/core/thinlenscamera.cpp:71-205):
  *   focal_length == 0: pinhole with an explicit vertical field of view, the

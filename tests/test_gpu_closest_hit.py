@@ -121,6 +121,14 @@ def test_big_leaves_and_nan_rays(oracle_mod, capi_mod, device, max_depth):
         _assert_bit_exact(_run_gpu(capi_mod, gsc, o, d), osc.closest_hit(o, d, counters=True))
 
 
+def test_fast_reciprocal_is_exact(capi_mod, device):
+    """The triangle test takes 1/det from the hardware reciprocal and one
+    Newton step when 1e-6 <= |det| < 2^126: over every binary32 value of that
+    range the result equals the IEEE quotient."""
+    print("[gpu closest hit] exhaustive reciprocal check", flush=True)
+    assert capi_mod.check_reciprocal() == 0
+
+
 @pytest.fixture(scope="module")
 def full_mesh_pair(oracle_mod, capi_mod, device):
     """The 1,311,488-triangle closest-hit workload mesh (icosphere level 8 + 64 boxes)."""

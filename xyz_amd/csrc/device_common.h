@@ -175,6 +175,26 @@ __device__ __forceinline__ void trace_init(TraceState& st, float3 o, float3 d) {
   st.xt = st.xb = 0;
 }
 
+// 1/det as the reference's IEEE division gives it, wherever tri_test uses the
+// value.  For 1e-6 <= |det| < 2^126 one Newton step from the hardware
+// reciprocal yields the correctly rounded quotient: xyz_check_reciprocal
+// compares the two over every binary32 value of that range.  Below 1e-6
+// tri_test rejects the triangle and ignores the value; 2^126 and above,
+// infinities and NaN take the division.
+constexpr float kRecipFastMax = 0x1p126f;
+__host__ __device__ __forceinline__ float recip_fast(float det) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  float const y = __builtin_amdgcn_rcpf(det);
+  return __builtin_fmaf(__builtin_fmaf(-det, y, 1.0f), y, y);
+#else
+  return 1.0f / det;
+#endif
+}
+__host__ __device__ __forceinline__ float recip_det(float det) {
+  if (fabsf(det) < kRecipFastMax) return recip_fast(det);
+  return 1.0f / det;
+}
+
 // One double-faced ray/triangle test, xyz/geom/triangle.cpp:127-165, written
 // without branches: every quantity is computed in the reference's order and
 // the accept flag combines the reference's reject tests.
@@ -183,7 +203,7 @@ __host__ __device__ __forceinline__ bool tri_test(float3 o, float3 d, float4 A, 
                                          float* g_out, float* det_out) {
   float const nx = A.w, ny = B.w, nz = C.w;
   float const det = d.x * nx + d.y * ny + d.z * nz;
-  float const inv = 1.0f / det;
+  float const inv = recip_det(det);
   float const mx = A.x - o.x, my = A.y - o.y, mz = A.z - o.z;
   float const t = inv * (mx * nx + my * ny + mz * nz);
   float const qx = d.y * mz - d.z * my;

@@ -550,6 +550,23 @@ int scene_create(const xyz_scene_desc* desc, xyz_scene** out) {
   return XYZ_OK;
 }
 
+namespace {
+// Counts the binary32 values x with 1e-6 <= |x| < kRecipFastMax whose fast
+// reciprocal differs from the IEEE quotient 1.0f / x; bit patterns
+// [begin, end) of the 2^32.
+__global__ void k_check_reciprocal(uint64_t begin, uint64_t end, unsigned long long* bad) {
+  uint64_t const stride = uint64_t(gridDim.x) * blockDim.x;
+  unsigned long long n = 0;
+  for (uint64_t i = begin + uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < end; i += stride) {
+    float const x = __uint_as_float(uint32_t(i));
+    float const ax = fabsf(x);
+    if (!(ax >= 1e-6f && ax < kRecipFastMax)) continue;
+    if (__float_as_uint(recip_fast(x)) != __float_as_uint(1.0f / x)) ++n;
+  }
+  if (n != 0) atomicAdd(bad, n);
+}
+}  // namespace
+
 }  // namespace xyzhip
 
 using namespace xyzhip;
@@ -672,6 +689,28 @@ int xyz_closest_hit_work(xyz_scene* scene, uint32_t n, const float* org, const f
   }
   return launch_closest_hit(scene->dev, n, org, dir, prim, t, nullptr, nullptr, nullptr, nullptr,
                             work, static_cast<hipStream_t>(stream));
+}
+
+int xyz_check_reciprocal(unsigned long long* mismatches, void* stream) {
+  if (!mismatches) {
+    set_error("xyz_check_reciprocal: NULL result");
+    return XYZ_E_INVALID;
+  }
+  hipStream_t const st = static_cast<hipStream_t>(stream);
+  unsigned long long* bad = nullptr;
+  hipError_t e = hipMalloc(&bad, sizeof(unsigned long long));
+  if (e != hipSuccess) return hip_fail(e, "xyz_check_reciprocal");
+  e = hipMemsetAsync(bad, 0, sizeof(unsigned long long), st);
+  for (uint64_t b = 0; e == hipSuccess && b < (1ull << 32); b += (1ull << 30)) {
+    hipLaunchKernelGGL(k_check_reciprocal, dim3(4096), dim3(256), 0, st, b, b + (1ull << 30), bad);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(mismatches, bad, sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  hipError_t const ef = hipFree(bad);
+  if (e == hipSuccess) e = ef;
+  return e == hipSuccess ? XYZ_OK : hip_fail(e, "xyz_check_reciprocal");
 }
 
 }  // extern "C"
