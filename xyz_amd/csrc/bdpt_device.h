@@ -93,7 +93,7 @@ __device__ __forceinline__ void basis_about(float3 n, float3* t, float3* b) {
   float3 tt = f3(-n.z, 0.0f, n.x);
   float const l2 = dot3(tt, tt);
   if (l2 > 0.0f) {
-    tt = scl3(tt, 1.0f / sqrtf(l2));
+    tt = scl3(tt, recip_exact(sqrtf(l2)));
     *b = cross3(tt, n);
   } else {
     *b = f3(0.0f, 0.0f, (n.y > 0.0f) ? 1.0f : -1.0f);

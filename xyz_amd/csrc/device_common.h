@@ -194,6 +194,13 @@ __host__ __device__ __forceinline__ float recip_det(float det) {
   if (fabsf(det) < kRecipFastMax) return recip_fast(det);
   return 1.0f / det;
 }
+// 1/x with the bits of the IEEE quotient for every x: the fast form on the
+// range where it is checked to equal the quotient, the division elsewhere.
+__host__ __device__ __forceinline__ float recip_exact(float x) {
+  float const ax = fabsf(x);
+  if (ax >= 1e-6f && ax < kRecipFastMax) return recip_fast(x);
+  return 1.0f / x;
+}
 
 // One double-faced ray/triangle test, xyz/geom/triangle.cpp:127-165, written
 // without branches: every quantity is computed in the reference's order and

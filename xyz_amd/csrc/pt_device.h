@@ -46,8 +46,8 @@ __device__ __forceinline__ float3 cross3(float3 a, float3 b) {
   return f3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
 // libhi/hi/math/math.hpp:146-157,254-257: normalize(a) = a * (1 / sqrt(dot(a, a)))
-__device__ __forceinline__ float rcpf(float x) { return 1.0f / x; }
-__device__ __forceinline__ float3 normalize3(float3 a) { return scl3(a, 1.0f / sqrtf(dot3(a, a))); }
+__device__ __forceinline__ float rcpf(float x) { return recip_exact(x); }
+__device__ __forceinline__ float3 normalize3(float3 a) { return scl3(a, recip_exact(sqrtf(dot3(a, a)))); }
 __device__ __forceinline__ float3 ld3(const float4& v) { return f3(v.x, v.y, v.z); }
 
 // ------------------------------------------------------- deterministic cos/sin
@@ -267,7 +267,7 @@ __device__ __forceinline__ Basis shading_basis(const DevScene& s, int32_t prim, 
   float3 t = f3(-B.ns.z, 0.0f, B.ns.x);
   float const l2 = dot3(t, t);
   if (l2 > 0.0f) {
-    t = scl3(t, 1.0f / sqrtf(l2));
+    t = scl3(t, recip_exact(sqrtf(l2)));
     B.b = cross3(t, B.ns);
   } else {
     B.b = f3(0.0f, 0.0f, (B.ns.y > 0.0f) ? 1.0f : -1.0f);
