@@ -203,8 +203,9 @@ __host__ __device__ __forceinline__ float recip_exact(float x) {
 }
 
 // One double-faced ray/triangle test, xyz/geom/triangle.cpp:127-165, written
-// without branches: every quantity is computed in the reference's order and
-// the accept flag combines the reference's reject tests.
+// without branches (but recip_det's division for out-of-range det): every
+// quantity is computed in the reference's order and the accept flag combines
+// the reference's reject tests.
 __host__ __device__ __forceinline__ bool tri_test(float3 o, float3 d, float4 A, float4 B, float4 C,
                                          float t_min, float t_max, float* t_out, float* f_out,
                                          float* g_out, float* det_out) {
