@@ -7,7 +7,10 @@ on tests/golden/cornell_box.mqo, bit for bit; the oracle itself is pinned by
 closed forms that do not come from either implementation: the reference
 camera's default eye, rotations of the Scene chunk, smooth normals of an
 octahedron and a cube, the CCIR 601-1 row sums, the material-flag table and
-quad areas."""
+quad areas.  The spectral tables it restates are checked against published
+colorimetry and physics: the Macbeth chart's Munsell luminances and measured
+hues, the CIE 1931 functions' equal integrals and peaks, and the mercury
+lines."""
 import math
 import os
 
@@ -288,3 +291,79 @@ def test_shell_commands_and_save(tmp_path):
     sh2.run([l for l in text.splitlines() if not l.startswith(("load", "ren"))])
     assert (sh2.eye, sh2.width, sh2.rendering_time, sh2.algorithm) == ([1, 2, 3], 320, 10, 0)
     assert sh2.run(["quit", "set dim 2 2"]) == "quit" and sh2.width == 320
+
+
+# McCamy, Marcus and Davidson, "A color-rendition chart", J. Appl. Photogr.
+# Eng. 2 (1976) 95-99, table 1: CIE 1931 (x, y) of the 18 chromatic patches
+# under illuminant C (white point 0.3101, 0.3163) and their Y.
+_MCCAMY_1976 = {
+    "mcc-dark-skin": (0.4002, 0.3504, 10.05), "mcc-light-skin": (0.3773, 0.3446, 35.82),
+    "mcc-blue-sky": (0.2470, 0.2514, 19.33), "mcc-foliage": (0.3372, 0.4220, 13.29),
+    "mcc-blue-flower": (0.2651, 0.2400, 24.27), "mcc-bluish-green": (0.2608, 0.3430, 43.06),
+    "mcc-orange": (0.5060, 0.4070, 30.05), "mcc-purplish-blue": (0.2110, 0.1750, 11.97),
+    "mcc-moderate-red": (0.4533, 0.3058, 19.77), "mcc-purple": (0.2845, 0.2020, 6.56),
+    "mcc-yellow-green": (0.3800, 0.4887, 44.29), "mcc-orange-yellow": (0.4729, 0.4375, 43.06),
+    "mcc-blue": (0.1866, 0.1285, 6.11), "mcc-green": (0.3046, 0.4782, 23.39),
+    "mcc-red": (0.5385, 0.3129, 12.00), "mcc-yellow": (0.4480, 0.4703, 59.10),
+    "mcc-magenta": (0.3635, 0.2325, 19.77), "mcc-cyan": (0.1958, 0.2519, 19.77),
+}
+# Munsell values of the neutral series (N 9.5, 8, 6.5, 5, 3.5, 2)
+_NEUTRAL_V = {"mcc-white": 9.5, "mcc-neutral-80": 8.0, "mcc-neutral-65": 6.5,
+              "mcc-neutral-50": 5.0, "mcc-neutral-35": 3.5, "mcc-black": 2.0}
+
+
+def _munsell_y(v):
+    """Luminance factor of Munsell value v (Newhall, Nickerson and Judd 1943)."""
+    return 1.2219 * v - 0.23111 * v ** 2 + 0.23951 * v ** 3 - 0.021009 * v ** 4 + 0.0008404 * v ** 5
+
+
+def test_spectral_tables_against_published_colorimetry(mqo, orc):
+    """The restated spectral tables, checked against colorimetry that comes
+    from neither implementation.  CIE 1931: the three matching functions have
+    equal integrals (equal-energy white is x = y = 1/3), which holds on the
+    400-700 nm window to 0.3 %, and y-bar peaks at 555 nm.  Macbeth chart,
+    under equal energy: the neutral series' Y equals the Munsell renotation's
+    luminance factor of its value within 2 (in %) with achromatic
+    chromaticity; each chromatic patch's hue angle about the white point is
+    within 25 degrees of the published one and its Y within 7 (the published
+    figures are under illuminant C over 380-780 nm).  A swapped or misnamed
+    row moves a hue by far more.  The native loader gives the oracle's XYZ for
+    all 24 names (with the reference's first-substring lookup)."""
+    cmf = np.array(orc._CMF, np.float64)
+    s = cmf.sum(1)
+    assert np.all(np.abs(s / s.mean() - 1) < 0.005)
+    assert 15 <= int(np.argmax(cmf[1])) <= 16  # y-bar: 550 or 560 nm
+    assert int(np.argmax(cmf[0])) == 20 and int(np.argmax(cmf[0][:10])) == 4  # x-bar: 600, 440
+    assert int(np.argmax(cmf[2])) == 5  # z-bar: 450 nm
+    xw, yw = s[0] / s.sum(), s[1] / s.sum()
+    table = dict(orc.MACBETH)
+    assert len(table) == 24 and set(table) == set(_MCCAMY_1976) | set(_NEUTRAL_V)
+    for name, spd in table.items():
+        X, Y, Z = (cmf * np.array(spd)).sum(1) / s[1] * 100
+        x, y = X / (X + Y + Z), Y / (X + Y + Z)
+        if name in _NEUTRAL_V:
+            assert abs(Y - _munsell_y(_NEUTRAL_V[name])) < 2.0, name
+            assert abs(x - xw) < 0.005 and abs(y - yw) < 0.005, name
+        else:
+            px, py, pY = _MCCAMY_1976[name]
+            hue = math.degrees(math.atan2(y - yw, x - xw))
+            phue = math.degrees(math.atan2(py - 0.3163, px - 0.3101))
+            assert abs((hue - phue + 180) % 360 - 180) < 25, name
+            assert abs(Y - pY) < 7, name
+    names = [k for k, _ in orc.MACBETH]
+    m = mqo.loads(document([material_line(k) for k in names], []))
+    for i, k in enumerate(names):
+        want = np.array(orc.material_xyz(k, (1, 1, 1, 1), 0.0), np.float32)
+        got = np.array(m.bsdfs[i]["rgb"], np.float32)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), k
+
+
+def test_mercury_lamp_table_peaks_at_the_mercury_lines(orc):
+    """The mercury arc lamp's spectrum (the reference's light power) has its
+    prominent peaks in the 10 nm bins of mercury's visible emission lines,
+    404.7, 435.8, 546.1 and 577.0/579.1 nm: 410, 440, 550 and 580 nm."""
+    v = np.array(orc.MERCURY)
+    assert len(v) == 31
+    peaks = [400 + 10 * i for i in range(1, 30)
+             if v[i] > v[i - 1] and v[i] > v[i + 1] and v[i] > 1.5 * np.median(v)]
+    assert peaks == [410, 440, 550, 580]
