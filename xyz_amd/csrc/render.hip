@@ -55,6 +55,11 @@ constexpr uint32_t kSegPad = 256;      // queue segments start at multiples of t
 #ifndef XYZ_GUIDED_SHADE_BLOCKS
 #define XYZ_GUIDED_SHADE_BLOCKS 4
 #endif
+// Workgroups per CU the plain and replay shading kernels are compiled for
+// (1: the compiler's register choice).
+#ifndef XYZ_SHADE_BLOCKS
+#define XYZ_SHADE_BLOCKS 1
+#endif
 
 struct RJob {
   DevScene s;
@@ -499,7 +504,7 @@ __device__ __forceinline__ uint32_t lds_wave_append(bool pred, uint32_t key, uin
 // workgroup's appends go to that job's queues: counted in LDS, then one global
 // atomic per queue.
 template <bool kReplay, bool kGuide>
-__global__ void __launch_bounds__(kBlock, kGuide ? XYZ_GUIDED_SHADE_BLOCKS : 1)
+__global__ void __launch_bounds__(kBlock, kGuide ? XYZ_GUIDED_SHADE_BLOCKS : XYZ_SHADE_BLOCKS)
 k_shade(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBufs wb,
         const uint32_t* __restrict__ qcur, uint32_t* qnext, GradArgs ga) {
   __shared__ uint32_t vst[kMaxWaveJobs * 8 + 1];
