@@ -83,3 +83,10 @@ def test_library_exports_every_declared_symbol(capi_mod):
     for name in declared:
         assert hasattr(L, name), name
     assert capi_mod.lib().xyz_version().decode().startswith("xyz-hip")
+
+
+def test_check_reciprocal_rejects_null_without_a_device(capi_mod):
+    """The self-check's argument test runs before any HIP call."""
+    L = capi_mod.lib()
+    assert L.xyz_check_reciprocal(None, None) == capi_mod.XYZ_E_INVALID
+    assert "NULL" in L.xyz_last_error().decode()
