@@ -545,14 +545,19 @@ __device__ __forceinline__ bool coop_step(const DevScene& s, TraceState& st, boo
       }
     }
     wave_sync();
-    uint32_t const mask = masks[2 * lane];
-    for (uint32_t i = 0; i < ncl; ++i) {
-      if ((mask >> i) & 1u) {
-        uint32_t const ci = st.lj + i;
-        list |= static_cast<unsigned long long>(i) << (4u * nsurv);
-        ++nsurv;
-        c += (ci + 1 == st.lend) ? st.ln - ((ci - st.lc0) << kClusterShift) : kClusterSize;
-      }
+    // survivors in ascending order (only bits below ncl are set); every
+    // cluster holds kClusterSize records but the leaf's last, which may be
+    // partly used
+    uint32_t mask = masks[2 * lane];
+    nsurv = uint32_t(__popc(mask));
+    c = nsurv << kClusterShift;
+    uint32_t const last = st.lend - 1u - st.lj;  // the leaf's last cluster, as a bit of this step
+    if (ncl > 0 && last < ncl && ((mask >> last) & 1u))
+      c -= kClusterSize - (st.ln - ((st.lend - 1u - st.lc0) << kClusterShift));
+    for (uint32_t k = 0; mask != 0u; ++k) {
+      uint32_t const i = uint32_t(__ffs(mask)) - 1u;
+      mask &= mask - 1u;
+      list |= static_cast<unsigned long long>(i) << (4u * k);
     }
     wave_sync();
   }
