@@ -736,6 +736,60 @@ struct WaveFeeder {
     }
     return got;
   }
+
+  // The same, over work split into kFeedParts contiguous parts [total k / P,
+  // total (k + 1) / P) with a counter each (counters[k]).  The wave starts on
+  // part `part` (a block's XCD under round-robin dispatch, so that each XCD's
+  // L2 holds the scene data of its own rays) and moves to the next part when
+  // one is used up; after all parts it is exhausted.  Each index is handed
+  // out once; each wave visits each part at most once.
+  static constexpr uint32_t kFeedParts = 8;
+  uint32_t part, parts_left;
+  __device__ __forceinline__ void init_parts(uint32_t first) {
+    init();
+    part = first % kFeedParts;
+    parts_left = kFeedParts;
+  }
+  __device__ __forceinline__ uint32_t refill_parts(bool need, uint32_t* counters, uint32_t total,
+                                                  bool* exhausted) {
+    uint32_t const lane = __lane_id();
+    unsigned long long const below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint32_t got = 0xFFFFFFFFu;
+    for (;;) {
+      unsigned long long const m = __ballot(need);
+      if (m == 0) break;
+      if (next == count) {
+        if (parts_left == 0) {
+          if (need) *exhausted = true;
+          break;
+        }
+        uint32_t const pt = __builtin_amdgcn_readfirstlane(part);
+        uint32_t const lo = uint32_t((uint64_t(total) * pt) / kFeedParts);
+        uint32_t const size = uint32_t((uint64_t(total) * (pt + 1)) / kFeedParts) - lo;
+        uint32_t b = 0;
+        if (lane == 0) b = atomicAdd(&counters[pt], 64u);
+        b = __builtin_amdgcn_readfirstlane(b);
+        base = lo + b;
+        count = (b >= size) ? 0u : ((size - b < 64u) ? size - b : 64u);
+        next = 0;
+        if (count == 0) {
+          part = (pt + 1) % kFeedParts;
+          parts_left = __builtin_amdgcn_readfirstlane(parts_left) - 1u;
+          continue;
+        }
+      }
+      uint32_t const avail = count - next;
+      uint32_t const rank = uint32_t(__popcll(m & below));
+      uint32_t const want = uint32_t(__popcll(m));
+      uint32_t const take = want < avail ? want : avail;
+      if (need && rank < take) {
+        got = base + next + rank;
+        need = false;
+      }
+      next += take;
+    }
+    return got;
+  }
 };
 
 // Wave-level work distribution for persistent kernels: the lanes with `need`

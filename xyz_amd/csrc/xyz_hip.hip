@@ -71,8 +71,11 @@ constexpr uint32_t kHitBlocksPerCu = XYZ_HIT_BLOCKS_PER_CU;
 
 // Persistent closest-hit kernel: every lane keeps a ray in flight; when its
 // query finishes the lane takes the next ray of its wave's batch (batches of 64
-// consecutive rays from a global counter), so a wave's SIMD lanes stay busy
-// whatever the spread of per-ray cost.
+// consecutive rays), so a wave's SIMD lanes stay busy whatever the spread of
+// per-ray cost.  The rays are split into 8 contiguous parts with a counter
+// each; a block starts on part blockIdx.x % 8, the XCD that round-robin
+// dispatch gives it, so each XCD's L2 serves neighbouring rays, and moves on to
+// the other parts when its own is used up.
 #ifndef XYZ_HIT_WAVES_PER_EU
 #define XYZ_HIT_WAVES_PER_EU 0
 #endif
@@ -92,14 +95,14 @@ closest_hit_kernel(DevScene s, uint32_t n, const float* __restrict__ org,
   __shared__ unsigned long long keys[2 * kHitBlock];
   uint32_t* const stk = lds_stack + threadIdx.x;
   WaveFeeder feed;
-  feed.init();
+  feed.init_parts(blockIdx.x);
   TraceState st;
   uint32_t r = 0xFFFFFFFFu;
   bool exhausted = false;
   unsigned long long sum[4] = {0, 0, 0, 0};  // counting mode: this lane's work
   for (;;) {
     bool const need = (r == 0xFFFFFFFFu) && !exhausted;
-    uint32_t const got = feed.refill(need, work, n, &exhausted);
+    uint32_t const got = feed.refill_parts(need, work, n, &exhausted);
     if (got != 0xFFFFFFFFu) {
       r = got;
       trace_init(st, make_float3(org[3 * size_t(r)], org[3 * size_t(r) + 1], org[3 * size_t(r) + 2]),
@@ -144,8 +147,9 @@ closest_hit_kernel(DevScene s, uint32_t n, const float* __restrict__ org,
   }
 }
 
+// Clears the closest hit's work counters (one per part of the rays).
 __global__ void k_zero_u32(uint32_t* p) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
+  if (blockIdx.x == 0 && threadIdx.x < WaveFeeder::kFeedParts) p[threadIdx.x] = 0;
 }
 
 int g_hit_cu = 0;
@@ -165,8 +169,8 @@ int launch_closest_hit(const DevScene& s, uint32_t n, const float* org, const fl
   uint32_t const need_blocks = (n + kHitBlock - 1) / kHitBlock;
   uint32_t const grid = std::min<uint32_t>(need_blocks, uint32_t(g_hit_cu) * kHitBlocksPerCu);
   void* work = nullptr;
-  hipError_t e = pool_alloc(&work, sizeof(uint32_t), stream);
-  if (e != hipSuccess) return hip_fail(e, "pool_alloc(work counter)");
+  hipError_t e = pool_alloc(&work, sizeof(uint32_t) * WaveFeeder::kFeedParts, stream);
+  if (e != hipSuccess) return hip_fail(e, "pool_alloc(work counters)");
   uint32_t* w = static_cast<uint32_t*>(work);
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, w);
   if (counters || totals) {
