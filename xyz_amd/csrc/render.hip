@@ -45,6 +45,7 @@ struct alignas(256) LineU32 {
 };
 
 constexpr uint32_t kMaxWaveJobs = 64;  // jobs one wave may hold
+constexpr uint32_t kFeedParts = 8;     // parts of the trace kernels' work, one per XCD
 constexpr uint32_t kSegPad = 256;      // queue segments start at multiples of this position
 
 // One render job -- scene, camera, integrator constants, film -- on the device.
@@ -88,8 +89,8 @@ struct WaveJobs {
 // aligned batch of positions always lies in one segment, i.e. one job.
 struct WaveCounters {
   LineU32 append[kMaxWaveJobs * 9];     // [9 l + o]: o < 8 next octant queue, o = 8 shadow requests
-  LineU32 work_extend;                  // persistent extend kernel's work counter
-  LineU32 work_shadow;                  // persistent shadow kernel's work counter
+  LineU32 work_extend[kFeedParts];      // persistent extend kernel's work counters, one per part
+  LineU32 work_shadow[kFeedParts];      // persistent shadow kernel's work counters
   uint32_t cur[kMaxWaveJobs * 8];       // current extension queue sizes
   uint32_t vext[kMaxWaveJobs * 8 + 1];  // padded prefix of cur
   uint32_t n_active;                    // sum of cur
@@ -281,15 +282,22 @@ __device__ __forceinline__ uint32_t wave_ray_limit(uint32_t total) {
 // all of the wave's lanes are idle.  All lanes call take() together.
 struct SegFeeder {
   uint32_t base, count, next, seg;
+  uint32_t part, parts_left;
   bool done;
-  __device__ __forceinline__ void init() {
+  // The positions [0, vtotal) are split into kFeedParts contiguous parts on
+  // kSegPad boundaries (so no claim of lim <= 64 positions crosses one), with
+  // a counter each; the wave starts on part `first` (its block's XCD under
+  // round-robin dispatch) and moves to the next part when one is used up.
+  __device__ __forceinline__ void init(uint32_t first) {
     base = count = next = seg = 0;
+    part = first % kFeedParts;
+    parts_left = kFeedParts;
     done = false;
   }
   // Lanes with `need` receive the entry r of segment *lane_seg (returned), or
   // UINT32_MAX.  cnt[g]: real size of segment g; segs_per_job segments per
   // job; *job: the wave's job, *switched: set when it changed.
-  __device__ __forceinline__ uint32_t take(bool need, uint32_t* counter, uint32_t vtotal,
+  __device__ __forceinline__ uint32_t take(bool need, LineU32* counters, uint32_t vtotal,
                                            uint32_t lim, const uint32_t* vst, uint32_t nseg,
                                            const uint32_t* cnt, uint32_t segs_per_job,
                                            uint32_t* job, bool* switched, uint32_t* lane_seg) {
@@ -302,12 +310,22 @@ struct SegFeeder {
       unsigned long long const m = __ballot(need);
       if (m == 0 || room == 0 || done) break;
       if (next == count) {
-        uint32_t b = 0;
-        if (lane == 0) b = atomicAdd(counter, lim);
-        b = __builtin_amdgcn_readfirstlane(__shfl(b, 0));
-        if (b >= vtotal) {
+        if (parts_left == 0) {
           done = true;
           break;
+        }
+        uint32_t const pt = __builtin_amdgcn_readfirstlane(part);
+        uint32_t const units = vtotal / kSegPad;
+        uint32_t const lo = (units * pt / kFeedParts) * kSegPad;
+        uint32_t const hi = (units * (pt + 1u) / kFeedParts) * kSegPad;
+        uint32_t b = 0;
+        if (lane == 0) b = atomicAdd(&counters[pt].v, lim);
+        b = __builtin_amdgcn_readfirstlane(__shfl(b, 0)) + lo;
+        if (b >= hi) {
+          part = (pt + 1u) % kFeedParts;
+          parts_left = __builtin_amdgcn_readfirstlane(parts_left) - 1u;
+          count = next = 0;
+          continue;
         }
         uint32_t const g = seg_of(vst, nseg, b);
         uint32_t const r0 = b - vst[g];
@@ -389,8 +407,10 @@ __global__ void __launch_bounds__(512) k_init_wave(const WaveJobs* __restrict__ 
   if (k == 0) {
     ctr->vext[nseg] = tot;
     ctr->n_active = wj.start[wj.nj];
-    ctr->work_extend.v = 0u;
-    ctr->work_shadow.v = 0u;
+    for (uint32_t p = 0; p < kFeedParts; ++p) {
+      ctr->work_extend[p].v = 0u;
+      ctr->work_shadow[p].v = 0u;
+    }
   }
 }
 
@@ -434,7 +454,7 @@ k_extend(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBu
   uint32_t const lim = wave_ray_limit(wb.ctr->n_active);
   uint32_t* const stk = lds_stack + threadIdx.x;
   SegFeeder feed;
-  feed.init();
+  feed.init(blockIdx.x);
   uint32_t job = 0;
   const DevScene* sp = &jobs[wj.jlo].s;  // the wave's job (uniform)
   TraceState st;
@@ -442,7 +462,7 @@ k_extend(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBu
   for (;;) {
     bool switched = false;
     uint32_t lseg = 0;
-    uint32_t const r = feed.take(slot == 0xFFFFFFFFu, &wb.ctr->work_extend.v, vtotal, lim, vst,
+    uint32_t const r = feed.take(slot == 0xFFFFFFFFu, wb.ctr->work_extend, vtotal, lim, vst,
                                  nseg, cnt, 8u, &job, &switched, &lseg);
     if (switched) sp = &jobs[wj.jlo + __builtin_amdgcn_readfirstlane(job)].s;
     DevScene const& s = *sp;
@@ -644,7 +664,7 @@ k_shadow(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBu
   uint32_t const lim = wave_ray_limit(n_req);
   uint32_t* const stk = lds_stack + threadIdx.x;
   SegFeeder feed;
-  feed.init();
+  feed.init(blockIdx.x);
   uint32_t job = 0;
   const DevScene* sp = &jobs[wj.jlo].s;  // the wave's job (uniform)
   TraceState st;
@@ -652,7 +672,7 @@ k_shadow(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBu
   for (;;) {
     bool switched = false;
     uint32_t lseg = 0;
-    uint32_t const r = feed.take(rq == 0xFFFFFFFFu, &wb.ctr->work_shadow.v, vtotal, lim, vst, nseg,
+    uint32_t const r = feed.take(rq == 0xFFFFFFFFu, wb.ctr->work_shadow, vtotal, lim, vst, nseg,
                                  cnt, 1u, &job, &switched, &lseg);
     if (switched) sp = &jobs[wj.jlo + __builtin_amdgcn_readfirstlane(job)].s;
     DevScene const& s = *sp;
@@ -736,8 +756,10 @@ __global__ void __launch_bounds__(512) k_prep(const WaveJobs* __restrict__ wjp, 
     ctr->vext[nseg] = tot;
     ctr->n_active = nsum;
     ctr->queries += qsum;
-    ctr->work_extend.v = 0u;
-    ctr->work_shadow.v = 0u;
+    for (uint32_t p = 0; p < kFeedParts; ++p) {
+      ctr->work_extend[p].v = 0u;
+      ctr->work_shadow[p].v = 0u;
+    }
   }
 }
 
