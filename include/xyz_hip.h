@@ -194,6 +194,13 @@ int xyz_closest_hit_work(xyz_scene *scene, uint32_t n, const float *org, const f
  * 0 means the closest hit is exact on this device.  Synchronous. */
 int xyz_check_reciprocal(unsigned long long *mismatches, void *stream);
 
+/* Self-check of the Metal lobe's pow (no counterpart in the reference): the
+ * reference evaluates std::pow on floats (xyz/stdafx.hpp:160 sampling cos
+ * theta_h = xi^(1/s), xyz/bsdf/metal.cpp:101,146 the lobe density's nh^s);
+ * the device evaluates it in double by a fixed series so that every build
+ * gives the same bits.  out[i] = pow(x[i], y[i]) for n device floats. */
+int xyz_eval_pow(uint32_t n, const float *x, const float *y, float *out, void *stream);
+
 /* ---- path tracing -------------------------------------------------------------
  * Camera (reference ThinLensCamera, xyz/core/thinlenscamera.cpp:71-205):
  *   focal_length == 0: pinhole with an explicit vertical field of view, the

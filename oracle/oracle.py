@@ -52,6 +52,8 @@ def lib():
                                              ctypes.c_float, _f32p]
         L.orc_philox.argtypes = [_u32p, _u32p, _u32p]
         L.orc_cos_sin.argtypes = [ctypes.c_uint64, _f32p, _f32p, _f32p]
+        L.orc_pow.argtypes = [ctypes.c_uint64, _f32p, _f32p, _f32p]
+        L.orc_glossy_half.argtypes = [ctypes.c_uint64, _f32p, _f32p, ctypes.c_float, _f32p]
         L.orc_shading_basis.argtypes = [_vp, ctypes.c_uint32, ctypes.c_float, ctypes.c_float,
                                         ctypes.c_int, _f32p]
         L.orc_sample_light.restype = ctypes.c_int
@@ -387,6 +389,26 @@ def cos_sin(x):
     s = np.empty_like(x)
     lib().orc_cos_sin(x.size, _p(x, _f32p), _p(c, _f32p), _p(s, _f32p))
     return c, s
+
+
+def pow32(x, y):
+    """The oracle's deterministic float pow (Metal lobe) of arrays x, y."""
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(np.broadcast_to(np.asarray(y, np.float32), x.shape), np.float32)
+    out = np.empty_like(x)
+    lib().orc_pow(x.size, _p(x, _f32p), _p(y, _f32p), _p(out, _f32p))
+    return out
+
+
+def glossy_half(mu_t, mu_p, shininess):
+    """The Metal sampler's half vectors (n, 3) for uniforms mu_t, mu_p in the
+    basis t = +x, n = +y, b = +z."""
+    mu_t = np.ascontiguousarray(mu_t, np.float32)
+    mu_p = np.ascontiguousarray(mu_p, np.float32)
+    out = np.empty((mu_t.size, 3), np.float32)
+    lib().orc_glossy_half(mu_t.size, _p(mu_t, _f32p), _p(mu_p, _f32p), np.float32(shininess),
+                          _p(out, _f32p))
+    return out
 
 
 def camera12(camera):

@@ -182,6 +182,77 @@ float xyz_sin(float a) {
   return float(v);
 }
 
+// Deterministic float pow for the Metal lobe (xyz/stdafx.hpp:160,
+// xyz/bsdf/metal.cpp:101,146), evaluated in double so that CPU and GPU agree
+// bit for bit:
+//   ln x  = E ln2 + 2 atanh(z), z = (m - 1)/(m + 1), x = m 2^E, m in [sqrt(1/2), sqrt(2)],
+//           atanh by its series z (1 + z^2/3 + ... + z^24/25) (|z| <= 0.1716, truncation
+//           below 1e-19);
+//   exp t = 2^k exp(r), r = t - k ln2 with |r| <= ln2/2, exp(r) by the Taylor
+//           series through r^15 (truncation below 1e-19);
+// ln2 split in two parts (ln2_hi has 32 significant bits, so E ln2_hi and
+// k ln2_hi are exact).  The special cases follow C's powf.
+const double kAtanhInv[13] = {1.0, 0.3333333333333333, 0.2, 0.14285714285714285, 0.1111111111111111,
+                              0.09090909090909091, 0.07692307692307693, 0.06666666666666667,
+                              0.058823529411764705, 0.05263157894736842, 0.047619047619047616,
+                              0.043478260869565216, 0.04};
+const double kExpInvFact[16] = {1.0, 1.0, 0.5, 0.16666666666666666, 0.041666666666666664,
+                                0.008333333333333333, 0.001388888888888889, 0.0001984126984126984,
+                                2.48015873015873e-05, 2.7557319223985893e-06, 2.755731922398589e-07,
+                                2.505210838544172e-08, 2.08767569878681e-09, 1.6059043836821613e-10,
+                                1.1470745597729725e-11, 7.647163731819816e-13};
+double dbl_from_bits(uint64_t b) { double d; std::memcpy(&d, &b, 8); return d; }
+// x > 0 finite (normal or subnormal), y finite.
+float pow_positive(float x, float y) {
+  uint32_t b;
+  std::memcpy(&b, &x, 4);
+  uint32_t m = b & 0x7FFFFFu;
+  int E = int((b >> 23) & 0xFFu) - 127;
+  if ((b >> 23) == 0u) {  // subnormal: shift the leading one into the hidden bit
+    E = -126;
+    while (!(m & 0x800000u)) { m <<= 1; --E; }
+    m &= 0x7FFFFFu;
+  }
+  double md = dbl_from_bits((uint64_t(0x3FF) << 52) | (uint64_t(m) << 29));
+  if (md > 1.4142135623730951) { md = md * 0.5; E += 1; }
+  double const ln2_hi = 6.93147180369123816490e-01, ln2_lo = 1.90821492927058770002e-10;
+  double const z = (md - 1.0) / (md + 1.0);
+  double const z2 = z * z;
+  double p = kAtanhInv[12];
+  for (int k = 11; k >= 0; --k) p = kAtanhInv[k] + z2 * p;
+  double const lnm = (2.0 * z) * p;
+  double const L = (double(E) * ln2_hi) + ((double(E) * ln2_lo) + lnm);
+  double const t = double(y) * L;
+  if (t > 100.0) return std::numeric_limits<float>::infinity();
+  if (t < -110.0) return 0.0f;
+  double const k = std::floor(t * 1.44269504088896338700 + 0.5);
+  double const r = (t - k * ln2_hi) - k * ln2_lo;
+  double q = kExpInvFact[15];
+  for (int n = 14; n >= 0; --n) q = kExpInvFact[n] + r * q;
+  return float(q * dbl_from_bits(uint64_t(int64_t(k) + 1023) << 52));
+}
+float xyz_pow(float x, float y) {
+  if (y == 0.0f || x == 1.0f) return 1.0f;
+  if (std::isnan(x) || std::isnan(y)) return x + y;
+  bool neg = false;
+  if (std::signbit(x)) {
+    if (std::isinf(y)) {
+      if (x == -1.0f) return 1.0f;
+      return ((-x < 1.0f) == (y > 0.0f)) ? 0.0f : std::numeric_limits<float>::infinity();
+    }
+    bool const integer = std::floor(y) == y;
+    if (!integer && x != 0.0f && !std::isinf(x)) return std::numeric_limits<float>::quiet_NaN();
+    neg = integer && std::fabs(y) < 16777216.0f && (int64_t(y) & 1) != 0;
+    x = -x;
+  }
+  float r;
+  if (std::isinf(y)) r = ((x < 1.0f) == (y > 0.0f)) ? 0.0f : std::numeric_limits<float>::infinity();
+  else if (x == 0.0f) r = (y > 0.0f) ? 0.0f : std::numeric_limits<float>::infinity();
+  else if (std::isinf(x)) r = (y > 0.0f) ? std::numeric_limits<float>::infinity() : 0.0f;
+  else r = pow_positive(x, y);
+  return neg ? -r : r;
+}
+
 // --------------------------------------------------------------- Philox4x32-10
 // Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3", SC'11.
 struct U4 { uint32_t v[4]; };
@@ -627,6 +698,24 @@ V3 albedo_at(Scene const& sc, Bsdf const& b, Tri const& tr, Hit const& h, int* t
   return V3(tx.texels[size_t(idx) * 3 + 0], tx.texels[size_t(idx) * 3 + 1], tx.texels[size_t(idx) * 3 + 2]);
 }
 
+// xyz/stdafx.hpp:50-63 (cos/sin as the path tracer's)
+real diffused(real mu_t, real mu_p, V3 const& t, V3 const& n, V3 const& b, V3* out) {
+  real const phi = real(2 * M_PI) * mu_p;
+  real const ct = std::sqrt(mu_t);
+  real const st = std::sqrt(1 - mu_t);
+  *out = (t * (st * xyz_cos(phi)) + b * (st * xyz_sin(phi))) + n * ct;
+  return ct;
+}
+
+// xyz/stdafx.hpp:154-168
+real glossy_half(real mu_t, real mu_p, real shin, V3 const& t, V3 const& n, V3 const& b, V3* h) {
+  real const phi = real(2 * M_PI) * mu_p;
+  real const ct = xyz_pow(mu_t, rcp(shin));
+  real const st = std::sqrt(real(1) - ct * ct);
+  *h = (t * (st * xyz_cos(phi)) + b * (st * xyz_sin(phi))) + n * ct;
+  return ct;
+}
+
 struct PTParams {
   uint32_t kmax;        // reference XYZ_CONFIG_kMaxRandomWalkDepth = max depth D + 1
   uint32_t rr_start;    // RR applies at vertices j > rr_start (0 = reference)
@@ -726,10 +815,8 @@ bool bounds_importance(Bsdf const& b, bool back, V3 const& ns, V3 const& ng, V3 
       real const nk1 = -dot(*dir, ns);
       if (nk1 <= 0) return false;
       real const cg = -dot(*dir, ng);
-      real const phi = real(2 * M_PI) * u.u[2];
-      real const ch = std::pow(u.u[1], rcp(b.shininess));
-      real const sh = std::sqrt(real(1) - ch * ch);
-      V3 const hv = (t * (sh * xyz_cos(phi)) + bn * (sh * xyz_sin(phi))) + ns * ch;
+      V3 hv;
+      glossy_half(u.u[1], u.u[2], b.shininess, t, ns, bn, &hv);
       real const hk = -dot(hv, *dir);
       *dir = *dir + (2 * hk) * hv;
       real const nk2 = dot(*dir, ns);
@@ -1015,7 +1102,7 @@ V3 pt_path(Scene const& sc, Camera const& cam, PTParams const& pp, uint32_t W, u
               real const nh = dot(ev.ns, hv);
               real const nk1 = -dot(ev.ns, ev.dir);
               real const fsn = real(double(b.shininess + 1) * (M_1_PI / 8) *
-                                    double(std::pow(nh, b.shininess)) / double(hk));
+                                    double(xyz_pow(nh, b.shininess)) / double(hk));
               real const dens = fsn * wgt * A;
               real const fres = fake_fresnel(hk);
               V3 const pe = lv.power * ev.power;
@@ -1097,10 +1184,8 @@ V3 pt_path(Scene const& sc, Camera const& cam, PTParams const& pp, uint32_t W, u
       ev.f_next = 1;
     } else if (b.type == METAL) {
       // xyz/bsdf/metal.cpp:122-153, xyz/stdafx.hpp:154-168
-      real const phi = real(2 * M_PI) * dj.u[2];
-      real const cos_h = std::pow(dj.u[1], rcp(b.shininess));
-      real const sin_h = std::sqrt(real(1) - cos_h * cos_h);
-      V3 const hv = (ev.t * (sin_h * xyz_cos(phi)) + ev.b * (sin_h * xyz_sin(phi))) + ev.ns * cos_h;
+      V3 hv;
+      real const cos_h = glossy_half(dj.u[1], dj.u[2], b.shininess, ev.t, ev.ns, ev.b, &hv);
       real const nk1 = -dot(ev.ns, ev.dir);
       real const hk = -dot(hv, ev.dir);
       ev.dir = ev.dir + (2 * hk) * hv;
@@ -1110,7 +1195,7 @@ V3 pt_path(Scene const& sc, Camera const& cam, PTParams const& pp, uint32_t W, u
       real const sc1 = fres / hmax(nk1, nk2) * nk2;
       ev.power = ev.power * sc1;
       ev.specular = false;
-      ev.f_next = real(double(b.shininess + 1) * (M_1_PI / 8) * double(std::pow(cos_h, b.shininess)) / double(hk));
+      ev.f_next = real(double(b.shininess + 1) * (M_1_PI / 8) * double(xyz_pow(cos_h, b.shininess)) / double(hk));
     } else {
       break;  // Light seen from behind: xyz/bsdf/light.cpp:27-31
     }
@@ -1140,24 +1225,6 @@ struct BVert {
 };
 
 constexpr uint32_t kBdptStreamLight = 2, kBdptStreamEye = 3;
-
-// xyz/stdafx.hpp:50-63 (cos/sin as the path tracer's)
-real diffused(real mu_t, real mu_p, V3 const& t, V3 const& n, V3 const& b, V3* out) {
-  real const phi = real(2 * M_PI) * mu_p;
-  real const ct = std::sqrt(mu_t);
-  real const st = std::sqrt(1 - mu_t);
-  *out = (t * (st * xyz_cos(phi)) + b * (st * xyz_sin(phi))) + n * ct;
-  return ct;
-}
-
-// xyz/stdafx.hpp:154-168
-real glossy_half(real mu_t, real mu_p, real shin, V3 const& t, V3 const& n, V3 const& b, V3* h) {
-  real const phi = real(2 * M_PI) * mu_p;
-  real const ct = std::pow(mu_t, rcp(shin));
-  real const st = std::sqrt(real(1) - ct * ct);
-  *h = (t * (st * xyz_cos(phi)) + b * (st * xyz_sin(phi))) + n * ct;
-  return ct;
-}
 
 // Triangle::GeometricBasis (xyz/geom/triangle.cpp:167-192)
 void geometric_basis(Tri const& tr, Hit const& h, BVert* v) {
@@ -1263,7 +1330,7 @@ bool lights_to_camera(Scene const& sc, Bsdf const& bs, Tri const& tr, Hit const&
       v->bsdf_x_ipdf = fake_fresnel(hk) * v->cos_in_sh / hmax(cos_sh, v->cos_in_sh) *
                        v->cos_out_geo / cig;
       v->s_prev = v->s_next = real(double(bs.shininess + 1) * (M_1_PI / 8) *
-                                   double(std::pow(nh, bs.shininess)) / double(hk));
+                                   double(xyz_pow(nh, bs.shininess)) / double(hk));
       v->s_prev *= rcp(v->cos_in_sh);
       v->s_next *= rcp(v->cos_out_geo);
       return true;
@@ -1346,7 +1413,7 @@ bool camera_to_lights(Scene const& sc, Bsdf const& bs, Tri const& tr, Hit const&
       real const cos_sh = dot(v->ns, v->wout);
       v->bsdf_x_ipdf = fake_fresnel(hk) * v->cos_in_sh / hmax(cos_sh, v->cos_in_sh);
       v->s_prev = v->s_next = real(double(bs.shininess + 1) * (M_1_PI / 8) *
-                                   double(std::pow(nh, bs.shininess)) / double(hk));
+                                   double(xyz_pow(nh, bs.shininess)) / double(hk));
       v->s_prev *= rcp(v->cos_in_sh);
       v->s_next *= rcp(v->cos_out_geo);
       return true;
@@ -1799,6 +1866,19 @@ void orc_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
 
 void orc_cos_sin(uint64_t n, const float* x, float* c, float* s) {
   for (uint64_t i = 0; i < n; ++i) { c[i] = xyz_cos(x[i]); s[i] = xyz_sin(x[i]); }
+}
+// The Metal lobe's half-vector sampler (xyz/stdafx.hpp:154-168) in the basis
+// t = +x, n = +y, b = +z: out = n * 3 floats.
+void orc_glossy_half(uint64_t n, const float* mu_t, const float* mu_p, float shininess, float* out) {
+  V3 const t(1, 0, 0), nn(0, 1, 0), b(0, 0, 1);
+  for (uint64_t i = 0; i < n; ++i) {
+    V3 h;
+    glossy_half(mu_t[i], mu_p[i], shininess, t, nn, b, &h);
+    for (int c = 0; c < 3; ++c) out[3 * i + c] = h[c];
+  }
+}
+void orc_pow(uint64_t n, const float* x, const float* y, float* out) {
+  for (uint64_t i = 0; i < n; ++i) out[i] = xyz_pow(x[i], y[i]);
 }
 
 // Shading basis (xyz/geom/triangle.cpp:194-219) of triangle `tri` at (f, g, back).

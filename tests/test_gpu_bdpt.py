@@ -4,8 +4,8 @@ on the same seeds.
 Each pixel's own colours are summed in sample order by one thread, so the own
 film is compared bit for bit; light-tracing contributions reach their pixels
 through float atomics, whose order changes the last bits, so the splat film is
-compared to 1e-5 relative.  METAL's lobe uses powf, whose last bit may differ
-between host and device libm: metal scenes are compared statistically."""
+compared to 1e-5 relative.  METAL's lobe uses the deterministic pow that the
+oracle evaluates too, so metal scenes are compared the same way."""
 import os
 
 import numpy as np
@@ -73,7 +73,7 @@ def test_mirror_wall_matches_oracle(oracle_mod, capi_mod, device, camera):
     assert spl.sum() > 0.1 * own.sum()
 
 
-def test_mqo_scene_with_metal_statistical(oracle_mod, capi_mod, device):
+def test_mqo_scene_with_metal(oracle_mod, capi_mod, device):
     from xyz_amd import mqo
     m = mqo.load(os.path.join(ROOT, "tests", "golden", "cornell_box.mqo"))
     sc = dict(v0=m.v0, v1=m.v1, v2=m.v2, tri_bsdf=m.tri_bsdf, bsdfs=m.bsdfs,
@@ -83,11 +83,7 @@ def test_mqo_scene_with_metal_statistical(oracle_mod, capi_mod, device):
     print("[gpu bdpt] mqo scene", flush=True)
     ref_own, ref_spl = o.render_bdpt(cam, 32, 24, 16, 15, seed=4)
     own, spl, _ = gpu_bdpt(g, cam, 32, 24, 16, 15, seed=4)
-    same = (own.view(np.uint32) == ref_own.view(np.uint32)).all(-1).mean()
-    print(f"identical own pixels {same:.3f}")
-    assert same > 0.5
-    for a, b in ((own, ref_own), (spl, ref_spl)):
-        assert abs(a.sum() - b.sum()) < 0.02 * b.sum()
+    assert_films(own, spl, ref_own, ref_spl)
 
 
 def test_one_film_and_passes(capi_mod, device):

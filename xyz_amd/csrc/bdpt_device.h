@@ -80,7 +80,7 @@ __device__ __forceinline__ float bd_diffused(float mu_t, float mu_p, float3 t, f
 __device__ __forceinline__ float bd_glossy_half(float mu_t, float mu_p, float shin, float3 t,
                                                 float3 n, float3 b, float3* h) {
   float const phi = 6.28318530717958647692f * mu_p;
-  float const ct = powf(mu_t, rcpf(shin));
+  float const ct = pow_det(mu_t, rcpf(shin));
   float const st = sqrtf(1.0f - ct * ct);
   float cp, sp;
   cos_sin(phi, &cp, &sp);
@@ -205,7 +205,7 @@ __device__ __forceinline__ bool lights_to_camera(const DevScene& s, const DevBsd
       w.bsdf_x_ipdf = fake_fresnel(hk) * w.cos_in_sh / hmaxf(cos_sh, w.cos_in_sh) *
                       w.cos_out_geo / cig;
       v.s_prev = v.s_next = float(double(bs.shininess + 1.0f) * (kInvPi / 8) *
-                                  double(powf(nh, bs.shininess)) / double(hk));
+                                  double(pow_det(nh, bs.shininess)) / double(hk));
       v.s_prev *= rcpf(w.cos_in_sh);
       v.s_next *= rcpf(w.cos_out_geo);
       return true;
@@ -290,7 +290,7 @@ __device__ __forceinline__ bool camera_to_lights(const DevScene& s, const DevBsd
       float const cos_sh = dot3(v.ns, w.wout);
       w.bsdf_x_ipdf = fake_fresnel(hk) * w.cos_in_sh / hmaxf(cos_sh, w.cos_in_sh);
       v.s_prev = v.s_next = float(double(bs.shininess + 1.0f) * (kInvPi / 8) *
-                                  double(powf(nh, bs.shininess)) / double(hk));
+                                  double(pow_det(nh, bs.shininess)) / double(hk));
       v.s_prev *= rcpf(w.cos_in_sh);
       v.s_next *= rcpf(w.cos_out_geo);
       return true;

@@ -117,6 +117,94 @@ __device__ __forceinline__ void cos_sin(float a, float* c, float* s) {
   *s = float(sv);
 }
 
+// ------------------------------------------------------- deterministic pow
+// powf of the Metal lobe (xyz/stdafx.hpp:160, xyz/bsdf/metal.cpp:101,146) in
+// double, so that every build computes the same bits:
+//   ln x  = E ln2 + 2 atanh(z), z = (m - 1)/(m + 1), x = m 2^E with m in
+//           [sqrt(1/2), sqrt(2)]; atanh by its series through z^25
+//           (|z| <= 0.1716, truncation below 1e-19);
+//   exp t = 2^k exp(r), r = t - k ln2, |r| <= ln2/2; exp(r) by its Taylor
+//           series through r^15 (truncation below 1e-19).
+// ln2 is split in two (ln2_hi has 32 significant bits, so E ln2_hi and
+// k ln2_hi are exact).  Special cases follow C's powf.
+__device__ __forceinline__ float pow_positive(float x, float y) {
+  uint32_t const b = __float_as_uint(x);
+  uint32_t m = b & 0x7FFFFFu;
+  int E = int((b >> 23) & 0xFFu) - 127;
+  if ((b >> 23) == 0u) {  // subnormal: the leading one moved into the hidden bit
+    int const sh = __clz(m) - 8;
+    m = (m << sh) & 0x7FFFFFu;
+    E = -126 - sh;
+  }
+  double md = __longlong_as_double((0x3FFll << 52) | ((long long)m << 29));
+  if (md > 1.4142135623730951) {
+    md = md * 0.5;
+    E += 1;
+  }
+  double const ln2_hi = 6.93147180369123816490e-01, ln2_lo = 1.90821492927058770002e-10;
+  double const z = (md - 1.0) / (md + 1.0);
+  double const z2 = z * z;
+  double p = 0.04;  // 1/25, then 1/23, ..., 1/3, 1
+  p = 0.043478260869565216 + z2 * p;
+  p = 0.047619047619047616 + z2 * p;
+  p = 0.05263157894736842 + z2 * p;
+  p = 0.058823529411764705 + z2 * p;
+  p = 0.06666666666666667 + z2 * p;
+  p = 0.07692307692307693 + z2 * p;
+  p = 0.09090909090909091 + z2 * p;
+  p = 0.1111111111111111 + z2 * p;
+  p = 0.14285714285714285 + z2 * p;
+  p = 0.2 + z2 * p;
+  p = 0.3333333333333333 + z2 * p;
+  p = 1.0 + z2 * p;
+  double const lnm = (2.0 * z) * p;
+  double const L = (double(E) * ln2_hi) + ((double(E) * ln2_lo) + lnm);
+  double const t = double(y) * L;
+  if (t > 100.0) return __int_as_float(0x7F800000);
+  if (t < -110.0) return 0.0f;
+  double const k = floor(t * 1.44269504088896338700 + 0.5);
+  double const r = (t - k * ln2_hi) - k * ln2_lo;
+  double q = 7.647163731819816e-13;  // 1/15!, then 1/14!, ..., 1
+  q = 1.1470745597729725e-11 + r * q;
+  q = 1.6059043836821613e-10 + r * q;
+  q = 2.08767569878681e-09 + r * q;
+  q = 2.505210838544172e-08 + r * q;
+  q = 2.755731922398589e-07 + r * q;
+  q = 2.7557319223985893e-06 + r * q;
+  q = 2.48015873015873e-05 + r * q;
+  q = 0.0001984126984126984 + r * q;
+  q = 0.001388888888888889 + r * q;
+  q = 0.008333333333333333 + r * q;
+  q = 0.041666666666666664 + r * q;
+  q = 0.16666666666666666 + r * q;
+  q = 0.5 + r * q;
+  q = 1.0 + r * q;
+  q = 1.0 + r * q;
+  return float(q * __longlong_as_double(((long long)(k) + 1023ll) << 52));
+}
+__device__ __forceinline__ float pow_det(float x, float y) {
+  float const inf = __int_as_float(0x7F800000);
+  if (y == 0.0f || x == 1.0f) return 1.0f;
+  if (isnan(x) || isnan(y)) return x + y;
+  bool neg = false;
+  if (signbit(x)) {
+    if (isinf(y)) {
+      if (x == -1.0f) return 1.0f;
+      return ((-x < 1.0f) == (y > 0.0f)) ? 0.0f : inf;
+    }
+    bool const integer = floorf(y) == y;
+    if (!integer && x != 0.0f && !isinf(x)) return __int_as_float(0x7FC00000);
+    neg = integer && fabsf(y) < 16777216.0f && ((long long)(y) & 1ll) != 0;
+    x = -x;
+  }
+  float r;
+  if (isinf(y)) r = ((x < 1.0f) == (y > 0.0f)) ? 0.0f : inf;
+  else if (x == 0.0f) r = (y > 0.0f) ? 0.0f : inf;
+  else if (isinf(x)) r = (y > 0.0f) ? inf : 0.0f;
+  else r = pow_positive(x, y);
+  return neg ? -r : r;
+}
+
 // ----------------------------------------------------------- Philox4x32-10
 // Salmon et al., SC'11.  key = (pixel, seed_lo), counter = (sample, bounce, seed_hi, 0).
 __device__ __forceinline__ uint4 philox(uint4 c, uint32_t k0, uint32_t k1) {
@@ -745,7 +833,7 @@ __device__ __forceinline__ bool shade_end(const DevScene& s, const PTConst& c, P
           float const nh = dot3(B.ns, hv);
           float const nk1 = -dot3(B.ns, ps.dir);
           sr->fsn = float(double(b.shininess + 1.0f) * (kInvPi / 8) *
-                          double(powf(nh, b.shininess)) / double(hk));
+                          double(pow_det(nh, b.shininess)) / double(hk));
           sr->fres = fake_fresnel(hk);
           sr->mx = (nk1 < nk2) ? nk2 : nk1;
         }
@@ -823,7 +911,7 @@ __device__ __forceinline__ bool shade_end(const DevScene& s, const PTConst& c, P
   if (b.type == XYZ_METAL) {
     // xyz/bsdf/metal.cpp:122-153, xyz/stdafx.hpp:154-168
     float const phi = 6.28318530717958647692f * u.z;
-    float const cos_h = powf(u.y, rcpf(b.shininess));
+    float const cos_h = pow_det(u.y, rcpf(b.shininess));
     float const sin_h = sqrtf(1.0f - cos_h * cos_h);
     float cp, sp;
     cos_sin(phi, &cp, &sp);
@@ -838,7 +926,7 @@ __device__ __forceinline__ bool shade_end(const DevScene& s, const PTConst& c, P
     float const sc = fres / mx * nk2;
     ps.power = scl3(ps.power, sc);
     ps.specular = false;
-    ps.f_next = float(double(b.shininess + 1.0f) * (kInvPi / 8) * double(powf(cos_h, b.shininess)) /
+    ps.f_next = float(double(b.shininess + 1.0f) * (kInvPi / 8) * double(pow_det(cos_h, b.shininess)) /
                       double(hk));
     return true;
   }

@@ -1863,6 +1863,13 @@ int render_bdpt_impl(xyz_scene* sc, const xyz_camera* cam, const xyz_render_para
   return XYZ_OK;
 }
 
+// Self-check of the Metal lobe's pow: out[i] = pow_det(x[i], y[i]).
+__global__ void k_eval_pow(uint32_t n, const float* __restrict__ x, const float* __restrict__ y,
+                           float* __restrict__ out) {
+  uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = pow_det(x[i], y[i]);
+}
+
 }  // namespace
 
 }  // namespace xyzhip
@@ -1870,6 +1877,18 @@ int render_bdpt_impl(xyz_scene* sc, const xyz_camera* cam, const xyz_render_para
 using namespace xyzhip;
 
 extern "C" {
+
+int xyz_eval_pow(uint32_t n, const float* x, const float* y, float* out, void* stream) {
+  if (n > 0 && (!x || !y || !out)) {
+    set_error("xyz_eval_pow: NULL buffer");
+    return XYZ_E_INVALID;
+  }
+  if (n == 0) return XYZ_OK;
+  hipLaunchKernelGGL(k_eval_pow, dim3((n + 255) / 256), dim3(256), 0, static_cast<hipStream_t>(stream),
+                     n, x, y, out);
+  hipError_t const e = hipGetLastError();
+  return e == hipSuccess ? XYZ_OK : hip_fail(e, "xyz_eval_pow");
+}
 
 int xyz_render(xyz_scene* scene, const xyz_camera* camera, const xyz_render_params* params,
                float* film, xyz_render_stats* stats, void* stream) {
