@@ -723,12 +723,29 @@ struct PTParams {
   uint64_t seed;
 };
 
-// Per-vertex record used by the adjoint replay.
+// What the reference gradient needs of one sample's path.  A sample's colour
+// is a sum of terms, each the path throughput times a local factor; vertex i
+// (a hit that reaches the NEE step, scene_pathtracing.cpp:111-142) multiplies
+// the throughput by its albedo a_i (Lambert, Metal; 1 for specular BSDFs),
+// adds its NEE term beta_i a_i Y_i, then scales the throughput by sigma_i
+// (roulette's 1/q, Metal's Fresnel weight).  A path that ends on a light adds
+// beta_n E.  With R_n = E and R_i = a_i (Y_i + sigma_i R_{i+1}), the colour's
+// derivative with respect to a_i is beta_i (Y_i + sigma_i R_{i+1}) -- no
+// division by a_i.  Every emission term is linear in its light's L_e through
+// Phi = L_e pi A_lights, so dcolour/dL_e is the term with pi A_lights in place
+// of Phi.
 struct VertexRecord {
-  int texture;  // -1 if albedo is constant
+  int texture;  // -1 unless a textured Lambert
   int texel;
-  V3 albedo;
-  V3 prefix;    // color accumulated before this vertex's albedo was applied
+  V3 albedo;    // a_i
+  V3 beta;      // throughput before a_i
+  V3 nee;       // Y_i (0 when the NEE sample is not visible)
+  real sigma;   // throughput scale after the NEE step
+};
+struct PathRecord {
+  std::vector<VertexRecord> v;
+  V3 emission;                    // E of the light the path ends on, per unit throughput
+  std::vector<double> d_emission;  // 3 per BSDF: dcolour / dL_e
 };
 
 struct PathStats { uint32_t queries = 0; Counters c; };
@@ -970,10 +987,10 @@ real guided_sample(real const cdf[kN + 1], Vertex const& v, real u_bin, real u_t
 }
 
 // xyz/core/scene_pathtracing.cpp:11-144.  Returns the sample's colour.
-// Optional: records Lambert vertices and per-light emission contributions.
+// Optional: records what the reference gradient needs (PathRecord).
 V3 pt_path(Scene const& sc, Camera const& cam, PTParams const& pp, uint32_t W, uint32_t H,
            uint32_t px, uint32_t py, uint32_t sample, PathStats* st,
-           std::vector<VertexRecord>* rec, std::vector<V3>* light_contrib,
+           PathRecord* rec,
            std::vector<Importon> const* guide = nullptr) {
   uint32_t const pixel = py * W + px;
   Draw d0 = draw(pixel, pp.seed, sample, 0);
@@ -1017,10 +1034,12 @@ V3 pt_path(Scene const& sc, Camera const& cam, PTParams const& pp, uint32_t W, u
     if ((b.type == LIGHT) && !ev.back) {
       V3 const lp = b.phi;
       V3 add;
+      double wfac;  // add = (Phi * power) * wfac
       if (ev.specular) {
         double const w = M_1_PI / double(A);
         V3 const pe = lp * ev.power;
         add = V3(real(double(pe[0]) * w), real(double(pe[1]) * w), real(double(pe[2]) * w));
+        wfac = w;
       } else {
         real const cos_t = std::fabs(dot(ev.dir, ev.ns));
         real const dist2 = h.t_max * h.t_max;
@@ -1032,10 +1051,15 @@ V3 pt_path(Scene const& sc, Camera const& cam, PTParams const& pp, uint32_t W, u
         V3 const pe = lp * ev.power;
         add = V3(real(double(pe[0]) * w1 * w2), real(double(pe[1]) * w1 * w2),
                  real(double(pe[2]) * w1 * w2));
+        wfac = w1 * w2;
       }
-      if (light_contrib) {
-        // emission of the light that was hit
-        (*light_contrib)[size_t(tr.bsdf)] = (*light_contrib)[size_t(tr.bsdf)] + add;
+      if (rec) {
+        // the light that was hit: dPhi/dL_e = pi A_lights
+        double const unit = M_PI * double(A);
+        for (int c = 0; c < 3; ++c) {
+          rec->emission[c] = real(double(lp[c]) * wfac);
+          rec->d_emission[3 * size_t(tr.bsdf) + size_t(c)] += double(ev.power[c]) * unit * wfac;
+        }
       }
       color = color + add;
       break;
@@ -1051,17 +1075,20 @@ V3 pt_path(Scene const& sc, Camera const& cam, PTParams const& pp, uint32_t W, u
     real cdf[kN + 1];
     if (guided) importance_cdf(*guide, ev, cdf);
 
+    VertexRecord* vr = nullptr;
+    if (rec) {
+      rec->v.push_back(VertexRecord{-1, -1, V3(1, 1, 1), ev.power, V3(0, 0, 0), real(1)});
+      vr = &rec->v.back();
+    }
+
     // ---- NEE: Bsdf::CalculateWeightedDirectLighting
     if (b.type == LAMBERT || b.type == METAL) {
       int texel = -1;
       V3 const alb = albedo_at(sc, b, tr, h, &texel);
-      if (rec) {
-        VertexRecord r;
-        r.texture = (b.type == LAMBERT) ? b.tex : -1;
-        r.texel = texel;
-        r.albedo = alb;
-        r.prefix = color;
-        rec->push_back(r);
+      if (vr) {
+        vr->texture = (b.type == LAMBERT) ? b.tex : -1;
+        vr->texel = texel;
+        vr->albedo = alb;
       }
       ev.power = ev.power * alb;
       if (lv.tri >= 0) {
@@ -1081,6 +1108,7 @@ V3 pt_path(Scene const& sc, Camera const& cam, PTParams const& pp, uint32_t W, u
           if (st) { st->c.nodes += c2.nodes; st->c.tris += c2.tris; }
           if (hit == ev.tri) {
             V3 add;
+            double nee_w = 0;
             if (b.type == LAMBERT) {
               // xyz/bsdf/lambert.cpp:157-172
               real const G = nk2 * nk3 / (sh.t_max * sh.t_max);
@@ -1093,6 +1121,7 @@ V3 pt_path(Scene const& sc, Camera const& cam, PTParams const& pp, uint32_t W, u
               double const wgt = (double(G) * (M_1_PI * M_1_PI)) * double(rcp(1 + dens * dens));
               V3 const pe = lv.power * ev.power;
               add = V3(real(double(pe[0]) * wgt), real(double(pe[1]) * wgt), real(double(pe[2]) * wgt));
+              nee_w = wgt;
             } else {
               // xyz/bsdf/metal.cpp:100-116
               real const wgt = nk3 / (sh.t_max * sh.t_max);
@@ -1111,10 +1140,17 @@ V3 pt_path(Scene const& sc, Camera const& cam, PTParams const& pp, uint32_t W, u
               double const rw = double(rcp(1 + dens * dens));
               for (int ch = 0; ch < 3; ++ch)
                 add[ch] = real(((((double(pe[ch]) * s1) * double(fsn)) / mx) * double(fres)) * rw);
+              nee_w = (((s1 * double(fsn)) / mx) * double(fres)) * rw;
             }
-            if (light_contrib && light_n >= 0) {
+            if (vr) {
+              // add = (Phi * power) * nee_w: per unit throughput Phi nee_w; the
+              // sampled light's dPhi/dL_e = pi A_lights
               uint32_t const lb = sc.tris[size_t(lv.tri)].bsdf;
-              (*light_contrib)[lb] = (*light_contrib)[lb] + add;
+              double const unit = M_PI * double(A);
+              for (int c = 0; c < 3; ++c) {
+                vr->nee[c] = real(double(lv.power[c]) * nee_w);
+                rec->d_emission[3 * size_t(lb) + size_t(c)] += double(ev.power[c]) * unit * nee_w;
+              }
             }
             color = color + add;
           }
@@ -1130,6 +1166,7 @@ V3 pt_path(Scene const& sc, Camera const& cam, PTParams const& pp, uint32_t W, u
         if (dj.u[0] >= q) break;
         real const inv = rcp(q);
         ev.power = ev.power * inv;
+        if (vr) vr->sigma = vr->sigma * inv;
       }
     }
 
@@ -1137,6 +1174,7 @@ V3 pt_path(Scene const& sc, Camera const& cam, PTParams const& pp, uint32_t W, u
     if (guided) {
       real const density = guided_sample(cdf, ev, dj.u[3], dj.u[1], dj.u[2], &ev.dir, &ev.f_next);
       ev.power = ev.power * rcp(density);
+      if (vr) vr->sigma = vr->sigma * rcp(density);
       ev.specular = false;
       if (!(dot(ev.dir, ev.ng) > 0)) break;
     } else if (b.type == LAMBERT) {
@@ -1194,6 +1232,7 @@ V3 pt_path(Scene const& sc, Camera const& cam, PTParams const& pp, uint32_t W, u
       real const fres = fake_fresnel(hk);
       real const sc1 = fres / hmax(nk1, nk2) * nk2;
       ev.power = ev.power * sc1;
+      if (vr) vr->sigma = vr->sigma * sc1;
       ev.specular = false;
       ev.f_next = real(double(b.shininess + 1) * (M_1_PI / 8) * double(xyz_pow(cos_h, b.shininess)) / double(hk));
     } else {
@@ -1942,7 +1981,7 @@ double orc_render(void* s, const float* cam, uint32_t W, uint32_t H, uint32_t sp
     for (uint32_t x = 0; x < W; ++x) {
       float* px = film + (size_t(y) * W + x) * 3;
       for (uint32_t smp = sample0; smp < sample0 + spp; ++smp) {
-        V3 col = pt_path(sc, c, pp, W, H, x, uint32_t(y), smp, stats ? &ps : nullptr, nullptr, nullptr);
+        V3 col = pt_path(sc, c, pp, W, H, x, uint32_t(y), smp, stats ? &ps : nullptr, nullptr);
         px[0] += col[0]; px[1] += col[1]; px[2] += col[2];
       }
     }
@@ -1994,12 +2033,11 @@ double orc_render_bdpt(void* s, const float* cam, uint32_t W, uint32_t H, uint32
 
 // Reference gradient of the image loss sum_p w_p . S_p, where S_p is the SUM of
 // the `spp` sample colours of pixel p (the film) and w = d_film (W*H*3) =
-// dLoss/dS, with respect to every texel
-// of every texture (d_tex, concatenated like the scene's texels) and to each
-// BSDF's emission L_e (d_emit, 3 per BSDF; non-zero for lights only).
-// Each sample's colour is a sum of terms; a term added after Lambert vertex i
-// carries that vertex's albedo as a factor, so d(term)/d(albedo_i) = term/albedo_i
-// per channel.  Emission terms are linear in the emitting light's L_e.
+// dLoss/dS, with respect to every texel of every texture (d_tex, concatenated
+// like the scene's texels) and to each BSDF's emission L_e (d_emit, 3 per
+// BSDF; non-zero for lights only).  Each sample's path is recorded
+// (PathRecord) and swept from its end: R = E, then for vertex i from the last
+// to the first g_i = Y_i + sigma_i R, dS/da_i += beta_i g_i, R = a_i g_i.
 // Russian-roulette probabilities and discrete choices are held fixed.
 double orc_render_backward(void* s, const float* cam, uint32_t W, uint32_t H, uint32_t spp,
                            uint32_t sample0, uint32_t max_depth, uint32_t rr_start,
@@ -2017,8 +2055,9 @@ double orc_render_backward(void* s, const float* cam, uint32_t W, uint32_t H, ui
     tex_off[i + 1] = tex_off[i] + size_t(sc.textures[i].w) * sc.textures[i].h * 3;
   int const nt = n_threads(threads);
   auto t0 = std::chrono::steady_clock::now();
+  size_t const nb = sc.bsdfs.size();
   std::vector<std::vector<double>> gt(size_t(nt), std::vector<double>(tex_off.back(), 0.0));
-  std::vector<std::vector<double>> ge(size_t(nt), std::vector<double>(sc.bsdfs.size() * 3, 0.0));
+  std::vector<std::vector<double>> ge(size_t(nt), std::vector<double>(nb * 3, 0.0));
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
   for (int64_t y = 0; y < int64_t(H); ++y) {
 #ifdef _OPENMP
@@ -2026,35 +2065,36 @@ double orc_render_backward(void* s, const float* cam, uint32_t W, uint32_t H, ui
 #else
     int const tid = 0;
 #endif
-    std::vector<VertexRecord> rec;
-    std::vector<V3> lc(sc.bsdfs.size());
+    PathRecord rec;
     for (uint32_t x = 0; x < W; ++x) {
       size_t const p = size_t(y) * W + x;
       double const w[3] = {d_film[3 * p], d_film[3 * p + 1], d_film[3 * p + 2]};
       for (uint32_t smp = sample0; smp < sample0 + spp; ++smp) {
-        rec.clear();
-        for (V3& v : lc) v = V3(0, 0, 0);
-        V3 const col = pt_path(sc, c, pp, W, H, x, uint32_t(y), smp, nullptr, &rec, &lc);
-        for (VertexRecord const& r : rec) {
-          if (r.texture < 0) continue;
-          size_t const base = tex_off[size_t(r.texture)] + size_t(r.texel) * 3;
-          for (int ch = 0; ch < 3; ++ch) {
-            double const suffix = double(col[ch]) - double(r.prefix[ch]);
-            gt[size_t(tid)][base + ch] += w[ch] * suffix / double(r.albedo[ch]);
+        rec.v.clear();
+        rec.emission = V3(0, 0, 0);
+        rec.d_emission.assign(nb * 3, 0.0);
+        pt_path(sc, c, pp, W, H, x, uint32_t(y), smp, nullptr, &rec);
+        double R[3] = {double(rec.emission[0]), double(rec.emission[1]), double(rec.emission[2])};
+        for (size_t i = rec.v.size(); i-- > 0;) {
+          VertexRecord const& r = rec.v[i];
+          double g[3];
+          for (int ch = 0; ch < 3; ++ch) g[ch] = double(r.nee[ch]) + double(r.sigma) * R[ch];
+          if (r.texture >= 0) {
+            size_t const base = tex_off[size_t(r.texture)] + size_t(r.texel) * 3;
+            for (int ch = 0; ch < 3; ++ch) gt[size_t(tid)][base + ch] += w[ch] * double(r.beta[ch]) * g[ch];
           }
+          for (int ch = 0; ch < 3; ++ch) R[ch] = double(r.albedo[ch]) * g[ch];
         }
-        for (size_t b = 0; b < sc.bsdfs.size(); ++b) {
+        for (size_t b = 0; b < nb; ++b) {
           if (sc.bsdfs[b].type != LIGHT) continue;
-          for (int ch = 0; ch < 3; ++ch)
-            if (sc.bsdfs[b].rgb[ch] != 0)
-              ge[size_t(tid)][3 * b + ch] += w[ch] * double(lc[b][ch]) / double(sc.bsdfs[b].rgb[ch]);
+          for (int ch = 0; ch < 3; ++ch) ge[size_t(tid)][3 * b + ch] += w[ch] * rec.d_emission[3 * b + ch];
         }
       }
     }
   }
   for (int t = 0; t < nt; ++t) {
     for (size_t i = 0; i < tex_off.back(); ++i) d_tex[i] += gt[size_t(t)][i];
-    for (size_t i = 0; i < sc.bsdfs.size() * 3; ++i) d_emit[i] += ge[size_t(t)][i];
+    for (size_t i = 0; i < nb * 3; ++i) d_emit[i] += ge[size_t(t)][i];
   }
   return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -2166,7 +2206,7 @@ double orc_render_idpt(void* s, void* m, const float* cam, uint32_t W, uint32_t 
       float* px = film + (size_t(y) * W + x) * 3;
       for (uint32_t smp = sample0; smp < sample0 + spp; ++smp) {
         V3 col = pt_path(sc, c, pp, W, H, x, uint32_t(y), smp, stats ? &ps : nullptr, nullptr,
-                         nullptr, guide);
+                         guide);
         px[0] += col[0]; px[1] += col[1]; px[2] += col[2];
       }
     }

@@ -3,12 +3,15 @@
 // progressive path-tracing pass, write a PFM.  Mode "host" touches no GPU.
 #include <cstdio>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "xyz_hip.hpp"
 
 int main(int argc, char** argv) {
-  bool const gpu = argc > 1 && std::strcmp(argv[1], "gpu") == 0;
+  // "films <prefix>": render on the device and write the film files
+  bool const films = argc > 2 && std::strcmp(argv[1], "films") == 0;
+  bool const gpu = films || (argc > 1 && std::strcmp(argv[1], "gpu") == 0);
   // two triangles of a unit quad at z = 1 facing -z, one light quad above
   std::vector<float> v0 = {0, 0, 1, 0, 0, 1, 0.2f, 2, 0.2f, 0.2f, 2, 0.2f};
   // light winding gives cross(e2, e1) = -y: it faces down toward the quad
@@ -39,6 +42,23 @@ int main(int argc, char** argv) {
   scene.FindIntersection(2, o, dir, prim, hit);
   std::printf("hits: %d t=%g, %d t=%g\n", prim[0], hit[0].t_max, prim[1], hit[1].t_max);
   if (prim[0] != 1 || prim[1] != 0 || hit[0].t_max != 1.0f) return 2;
+  if (films) {
+    xyz_camera fc = {{0.5f, 0.5f, -2}, {0.5f, 0.5f, 1}, {0, 1, 0}, 40.0f, 48, 40};
+    xyz_render_params fp = {};
+    fp.spp = 8;
+    fp.max_depth = 4;
+    fp.rr_threshold = 0.2f;
+    fp.seed = 3;
+    std::vector<float> f;
+    scene.Render(fc, fp, &f);
+    std::string const pre = argv[2];
+    bool ok = xyz_hip::SaveAsPfm((pre + ".pfm").c_str(), 48, 40, f, 8.0f);
+    ok = ok && xyz_hip::SaveAsPfmRgb((pre + ".rgb.pfm").c_str(), 48, 40, f, 8.0f);
+    ok = ok && xyz_hip::SaveAsFloat((pre + ".float").c_str(), 48, 40, f, 8.0f);
+    ok = ok && xyz_hip::SaveAsHdr((pre + ".hdr").c_str(), 48, 40, f, 8.0f);
+    std::printf("films written: %d\n", int(ok));
+    return ok ? 0 : 6;
+  }
   xyz_camera cam = {{0.5f, 0.5f, -2}, {0.5f, 0.5f, 1}, {0, 1, 0}, 40.0f, 16, 16};
   xyz_render_params prm = {};
   prm.spp = 4;

@@ -410,11 +410,14 @@ struct ShadowReq {
   float3 pe;         // light power * path power
   float fsn, fres, mx;  // METAL only
   float gd;          // LAMBERT: the importance histogram's density of w (1 when unguided)
+  uint32_t vj;       // index of the shading vertex on its path (0 = first hit)
 };
 
 // Contribution of a visible NEE sample (xyz/bsdf/lambert.cpp:157-172,
-// xyz/bsdf/metal.cpp:100-116), given the shadow ray's hit distance t.
-__device__ __forceinline__ float3 nee_contribution(const ShadowReq& r, float t, float A) {
+// xyz/bsdf/metal.cpp:100-116), given the shadow ray's hit distance t.  *wfac:
+// the contribution per unit of r.pe (for the adjoint).
+__device__ __forceinline__ float3 nee_contribution(const ShadowReq& r, float t, float A,
+                                                   double* wfac = nullptr) {
   if (r.type == XYZ_LAMBERT) {
     float const G = r.nk2 * r.nk3 / (t * t);
     // lambert.cpp:161 and, guided, :228-229 (M_1_PI * G * density)
@@ -422,6 +425,7 @@ __device__ __forceinline__ float3 nee_contribution(const ShadowReq& r, float t, 
     float const p_exp = rcpf(A);
     float const dens = p_imp / p_exp;
     double const wgt = (double(G) * (kInvPi * kInvPi)) * double(rcpf(1.0f + dens * dens));
+    if (wfac) *wfac = wgt;
     return f3(float(double(r.pe.x) * wgt), float(double(r.pe.y) * wgt),
               float(double(r.pe.z) * wgt));
   }
@@ -433,6 +437,7 @@ __device__ __forceinline__ float3 nee_contribution(const ShadowReq& r, float t, 
   double const rw = double(rcpf(1.0f + dens * dens));
   double const fr = double(r.fres);
   double const fs = double(r.fsn);
+  if (wfac) *wfac = (((s1 * fs) / mx) * fr) * rw;
   return f3(float(((((double(r.pe.x) * s1) * fs) / mx) * fr) * rw),
             float(((((double(r.pe.y) * s1) * fs) / mx) * fr) * rw),
             float(((((double(r.pe.z) * s1) * fs) / mx) * fr) * rw));
@@ -719,10 +724,17 @@ __device__ __forceinline__ bool guided_scatter(const uint32_t c[kImpN], float4 u
   return dot3(ps.dir, B.ng) > 0.0f;
 }
 
-// Gradient hooks of the path-replay adjoint; the primal uses NoGrad.
+// Gradient hooks of the adjoint's replay; the primal uses NoGrad.
+//   emission(bsdf, power, wfac, b, k): the path ends on light `bsdf` and adds
+//       (b.phi * power) * wfac; k is the vertex counter;
+//   vertex(vj, power, alb, texture, texel): shading vertex vj multiplies the
+//       throughput `power` by its albedo (1 for specular BSDFs; texture >= 0:
+//       a textured Lambert, albedo from `texel`);
+//   scale(s): the throughput is scaled by s after that (roulette, Metal).
 struct NoGrad {
-  __device__ __forceinline__ void albedo(int32_t, int32_t, float3, float3) {}
-  __device__ __forceinline__ void emission(uint32_t, float3) {}
+  __device__ __forceinline__ void emission(uint32_t, float3, double, const DevBsdf&, uint32_t) {}
+  __device__ __forceinline__ void vertex(uint32_t, float3, float3, int32_t, int32_t) {}
+  __device__ __forceinline__ void scale(float) {}
 };
 
 // One vertex of Scene::EvalutePathTracing (xyz/core/scene_pathtracing.cpp:52-142)
@@ -760,9 +772,11 @@ __device__ __forceinline__ bool shade_begin(const DevScene& s, const PTConst& c,
   if (b.type == XYZ_LIGHT && !h.back) {
     float3 const pe = mul3(f3(b.phi[0], b.phi[1], b.phi[2]), ps.power);
     float3 add;
+    double wfac;
     if (ps.specular) {
       double const w = kInvPi / double(A);
       add = f3(float(double(pe.x) * w), float(double(pe.y) * w), float(double(pe.z) * w));
+      wfac = w;
     } else {
       float const cos_t = fabsf(dot3(ps.dir, B.ns));
       float const dist2 = h.t * h.t;
@@ -773,8 +787,9 @@ __device__ __forceinline__ bool shade_begin(const DevScene& s, const PTConst& c,
       double const w2 = double(rcpf(1.0f + dens * dens));
       add = f3(float(double(pe.x) * w1 * w2), float(double(pe.y) * w1 * w2),
                float(double(pe.z) * w1 * w2));
+      wfac = w1 * w2;
     }
-    grad.emission(bsdf_id, add);
+    grad.emission(bsdf_id, ps.power, wfac, b, ps.k);
     ps.color = add3(ps.color, add);
     return false;
   }
@@ -799,10 +814,11 @@ __device__ __forceinline__ bool shade_end(const DevScene& s, const PTConst& c, P
   bool const guided = kGuide && x.guided;
 
   // NEE: Bsdf::CalculateWeightedDirectLighting
+  uint32_t const vj = ps.k - 2u;  // this vertex on the path (first hit: 0)
   if (b.type == XYZ_LAMBERT || b.type == XYZ_METAL) {
     int32_t texel;
     float3 const alb = albedo_at(s, b, h.prim, h.f, h.g, &texel);
-    if (b.type == XYZ_LAMBERT && texel >= 0) grad.albedo(b.texture, texel, alb, ps.color);
+    grad.vertex(vj, ps.power, alb, b.type == XYZ_LAMBERT ? b.texture : -1, texel);
     ps.power = mul3(ps.power, alb);
     if (lv.prim >= 0) {
       float3 const w = normalize3(sub3(ps.pos, lv.pos));
@@ -824,6 +840,7 @@ __device__ __forceinline__ bool shade_end(const DevScene& s, const PTConst& c, P
         sr->pe = mul3(lv.power, ps.power);
         sr->fsn = sr->fres = sr->mx = 0.0f;
         sr->gd = 1.0f;
+        sr->vj = vj;
         if constexpr (kGuide) {
           if (guided) sr->gd = importance_density(hc, w, B);
         }
@@ -840,6 +857,8 @@ __device__ __forceinline__ bool shade_end(const DevScene& s, const PTConst& c, P
         *has_sr = true;
       }
     }
+  } else {
+    grad.vertex(vj, ps.power, f3(1.0f, 1.0f, 1.0f), -1, -1);  // specular: no albedo factor
   }
 
   // Russian roulette: xyz/core/scene_pathtracing.cpp:128-137
@@ -850,6 +869,7 @@ __device__ __forceinline__ bool shade_end(const DevScene& s, const PTConst& c, P
       if (u.x >= q) return false;
       float const inv = rcpf(q);
       ps.power = scl3(ps.power, inv);
+      grad.scale(inv);
     }
   }
 
@@ -925,6 +945,7 @@ __device__ __forceinline__ bool shade_end(const DevScene& s, const PTConst& c, P
     float const mx = (nk1 < nk2) ? nk2 : nk1;
     float const sc = fres / mx * nk2;
     ps.power = scl3(ps.power, sc);
+    grad.scale(sc);
     ps.specular = false;
     ps.f_next = float(double(b.shininess + 1.0f) * (kInvPi / 8) * double(pow_det(cos_h, b.shininess)) /
                       double(hk));

@@ -36,6 +36,9 @@ constexpr uint32_t kBlock = 256;
 #define XYZ_WAVE_CAP_LOG2 25
 #endif
 constexpr uint64_t kWaveCap = 1ull << XYZ_WAVE_CAP_LOG2;
+// Bytes of adjoint vertex tape per internal stream (the adjoint's waves are
+// cut to fit).
+constexpr uint64_t kTapeBudget = 6ull << 30;
 
 // A counter alone in its 256-byte line: device-scope atomics on one line are
 // serialised, so counters that many workgroups hit at once do not share one.
@@ -103,7 +106,7 @@ struct WaveBufs {
   float4* pw;     // power.xyz
   float4* col;    // colour.xyz
   float4* hit;    // bits(prim | back << 31), t, f, g
-  float4* ctot;   // adjoint: the slot's primal colour
+  float4* ctot;   // adjoint: emission per unit throughput of the light the path ends on, bits(vertex count)
   float4* so;     // shadow: origin.xyz, nk2
   float4* sw;     // shadow: w.xyz, nk3
   float4* spe;    // shadow: pe.xyz, bits(slot)
@@ -113,6 +116,13 @@ struct WaveBufs {
   uint32_t* qb;   // 8 * cap
   WaveCounters* ctr;
   uint32_t cap;
+  // adjoint: per (vertex vj, slot) at [vj * cap + slot]: (throughput before
+  // the albedo, bits(texel key)), (albedo, scale after the NEE step), (NEE
+  // term per unit throughput after the albedo, 0); tape_depth vertices
+  float4* tape_a;
+  float4* tape_b;
+  float4* tape_c;
+  uint32_t tape_depth;
 };
 
 struct GradArgs {
@@ -121,33 +131,50 @@ struct GradArgs {
   float* d_emit;
 };
 
-// Per-workgroup aggregation of the adjoint's gradient deposits in LDS: the
-// emission gradient is a dense array (a handful of lights, hit by every path),
-// texel gradients go to a small open-addressing table; one global atomic per
-// touched entry is issued when the workgroup ends.  Entries that do not fit go
-// straight to global memory.
+// Per-workgroup aggregation of the adjoint's gradient deposits in LDS.  The
+// emission gradient is a dense array (a handful of lights, reached by every
+// path); texel gradients go to a small open-addressing table.  One global
+// atomic per touched entry is issued when the workgroup ends; entries that do
+// not fit go straight to global memory.
 constexpr uint32_t kTexSlots = 1024;
 constexpr uint32_t kTexProbe = 8;
 constexpr uint32_t kEmitLds = 3 * 256;
 constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;
 
-struct GradLds {
-  uint32_t keys[kTexSlots];
-  float vals[kTexSlots * 3];
+struct EmitLds {
   float emit[kEmitLds];
 };
+struct TexLds {
+  uint32_t keys[kTexSlots];
+  float vals[kTexSlots * 3];
+};
 
-__device__ __forceinline__ void grad_lds_init(GradLds& L) {
+__device__ __forceinline__ void emit_lds_init(EmitLds& L) {
+  for (uint32_t i = threadIdx.x; i < kEmitLds; i += blockDim.x) L.emit[i] = 0.0f;
+  __syncthreads();
+}
+__device__ __forceinline__ void emit_lds_flush(EmitLds& L, float* d_emit, uint32_t n_emit) {
+  __syncthreads();
+  uint32_t const m = n_emit < kEmitLds ? n_emit : kEmitLds;
+  for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
+    float const v = L.emit[i];
+    if (v != 0.0f) unsafeAtomicAdd(d_emit + i, v);
+  }
+}
+__device__ __forceinline__ void emit_add(EmitLds* L, float* d_emit, uint32_t i, float v) {
+  if (v == 0.0f) return;
+  if (i < kEmitLds) atomicAdd(&L->emit[i], v);
+  else unsafeAtomicAdd(d_emit + i, v);
+}
+
+__device__ __forceinline__ void tex_lds_init(TexLds& L) {
   for (uint32_t i = threadIdx.x; i < kTexSlots; i += blockDim.x) {
     L.keys[i] = kEmptyKey;
     L.vals[3 * i] = L.vals[3 * i + 1] = L.vals[3 * i + 2] = 0.0f;
   }
-  for (uint32_t i = threadIdx.x; i < kEmitLds; i += blockDim.x) L.emit[i] = 0.0f;
   __syncthreads();
 }
-
-__device__ __forceinline__ void grad_lds_flush(GradLds& L, float* d_tex, float* d_emit,
-                                               uint32_t n_emit) {
+__device__ __forceinline__ void tex_lds_flush(TexLds& L, float* d_tex) {
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < kTexSlots; i += blockDim.x) {
     uint32_t const k = L.keys[i];
@@ -157,58 +184,82 @@ __device__ __forceinline__ void grad_lds_flush(GradLds& L, float* d_tex, float* 
       if (v != 0.0f) unsafeAtomicAdd(d_tex + 3 * size_t(k) + c, v);
     }
   }
-  uint32_t const m = n_emit < kEmitLds ? n_emit : kEmitLds;
-  for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
-    float const v = L.emit[i];
-    if (v != 0.0f) unsafeAtomicAdd(d_emit + i, v);
+}
+// Adds g to texel `key` (the texel's index in the concatenated textures).
+__device__ __forceinline__ void tex_add(TexLds* L, float* d_tex, uint32_t key, float3 g) {
+  if (g.x == 0.0f && g.y == 0.0f && g.z == 0.0f) return;
+  uint32_t const h = (key * 2654435761u) >> 22;  // 10 bits
+  for (uint32_t p = 0; p < kTexProbe; ++p) {
+    uint32_t const slot = (h + p) & (kTexSlots - 1);
+    uint32_t const prev = atomicCAS(&L->keys[slot], kEmptyKey, key);
+    if (prev == kEmptyKey || prev == key) {
+      atomicAdd(&L->vals[3 * slot], g.x);
+      atomicAdd(&L->vals[3 * slot + 1], g.y);
+      atomicAdd(&L->vals[3 * slot + 2], g.z);
+      return;
+    }
   }
+  size_t const base = 3 * size_t(key);
+  unsafeAtomicAdd(d_tex + base, g.x);
+  unsafeAtomicAdd(d_tex + base + 1, g.y);
+  unsafeAtomicAdd(d_tex + base + 2, g.z);
 }
 
+// Gradient hooks of the adjoint's replay (see NoGrad).  Each shading vertex
+// writes its record to the slot's tape; a path that ends on a light writes
+// that light's emission per unit throughput to ctot.  The L_e derivative of
+// an emission term is the term with pi A_lights in place of Phi
+// (Phi = L_e pi A_lights) and is deposited at once.
 struct GradReplay {
   const DevTex* tex;
-  const DevBsdf* bsdfs;
-  float3 w;     // dLoss/d(film sum) of this path's pixel
-  float3 ctot;  // the path's total colour from the primal pass
-  float* d_tex;
+  float3 w;       // dLoss/d(film sum) of this path's pixel
+  double unit;    // pi A_lights
   float* d_emit;
-  GradLds* lds;
-  __device__ __forceinline__ void albedo(int32_t tex, int32_t texel, float3 alb, float3 prefix) {
-    // every later term carries this albedo as a factor: d/dalb = suffix / alb
-    uint32_t const key = (tex_offset(tex) / 3) + uint32_t(texel);
-    float const sx = ctot.x - prefix.x, sy = ctot.y - prefix.y, sz = ctot.z - prefix.z;
-    float const gx = (sx != 0.0f) ? w.x * sx / alb.x : 0.0f;
-    float const gy = (sy != 0.0f) ? w.y * sy / alb.y : 0.0f;
-    float const gz = (sz != 0.0f) ? w.z * sz / alb.z : 0.0f;
-    if (gx == 0.0f && gy == 0.0f && gz == 0.0f) return;
-    uint32_t h = (key * 2654435761u) >> 22;  // 10 bits
-    for (uint32_t p = 0; p < kTexProbe; ++p) {
-      uint32_t const slot = (h + p) & (kTexSlots - 1);
-      uint32_t const prev = atomicCAS(&lds->keys[slot], kEmptyKey, key);
-      if (prev == kEmptyKey || prev == key) {
-        atomicAdd(&lds->vals[3 * slot], gx);
-        atomicAdd(&lds->vals[3 * slot + 1], gy);
-        atomicAdd(&lds->vals[3 * slot + 2], gz);
-        return;
-      }
-    }
-    size_t const base = 3 * size_t(key);
-    unsafeAtomicAdd(d_tex + base, gx);
-    unsafeAtomicAdd(d_tex + base + 1, gy);
-    unsafeAtomicAdd(d_tex + base + 2, gz);
+  EmitLds* lds;
+  float4* ctot;
+  float4* tape_a;
+  float4* tape_b;
+  float4* tape_c;
+  uint32_t cap, tape_depth;
+  uint32_t slot;
+  uint32_t vj;    // the current vertex
+  float sigma;    // its throughput scale after the NEE step
+  __device__ __forceinline__ void bind(const WaveBufs& wb, uint32_t s) {
+    ctot = wb.ctot;
+    tape_a = wb.tape_a;
+    tape_b = wb.tape_b;
+    tape_c = wb.tape_c;
+    cap = wb.cap;
+    tape_depth = wb.tape_depth;
+    slot = s;
+    vj = 0;
+    sigma = 1.0f;
   }
-  __device__ __forceinline__ uint32_t tex_offset(int32_t t) const { return tex[t].offset; }
-  __device__ __forceinline__ void emission(uint32_t bsdf, float3 add) {
-    // Phi = L_e * pi * A, every emission term is linear in its light's L_e
-    DevBsdf const& b = bsdfs[bsdf];
-    float const g[3] = {(b.rgb[0] != 0.0f && add.x != 0.0f) ? w.x * add.x / b.rgb[0] : 0.0f,
-                        (b.rgb[1] != 0.0f && add.y != 0.0f) ? w.y * add.y / b.rgb[1] : 0.0f,
-                        (b.rgb[2] != 0.0f && add.z != 0.0f) ? w.z * add.z / b.rgb[2] : 0.0f};
-    for (int c = 0; c < 3; ++c) {
-      if (g[c] == 0.0f) continue;
-      uint32_t const i = 3 * bsdf + uint32_t(c);
-      if (i < kEmitLds) atomicAdd(&lds->emit[i], g[c]);
-      else unsafeAtomicAdd(d_emit + i, g[c]);
-    }
+  __device__ __forceinline__ void emission(uint32_t bsdf, float3 power, double wfac,
+                                           const DevBsdf& b, uint32_t k) {
+    ctot[slot] = make_float4(float(double(b.phi[0]) * wfac), float(double(b.phi[1]) * wfac),
+                             float(double(b.phi[2]) * wfac), __uint_as_float(k - 1u));
+    double const u = unit * wfac;
+    emit_add(lds, d_emit, 3 * bsdf, w.x * float(double(power.x) * u));
+    emit_add(lds, d_emit, 3 * bsdf + 1, w.y * float(double(power.y) * u));
+    emit_add(lds, d_emit, 3 * bsdf + 2, w.z * float(double(power.z) * u));
+  }
+  __device__ __forceinline__ void vertex(uint32_t j, float3 power, float3 alb, int32_t texture,
+                                         int32_t texel) {
+    vj = j;
+    sigma = 1.0f;
+    if (j >= tape_depth) return;  // the depth cap keeps j < tape_depth
+    uint32_t const key = (texture >= 0 && texel >= 0)
+                             ? tex[texture].offset / 3u + uint32_t(texel) : kEmptyKey;
+    size_t const i = size_t(j) * cap + slot;
+    tape_a[i] = make_float4(power.x, power.y, power.z, __uint_as_float(key));
+    tape_b[i] = make_float4(alb.x, alb.y, alb.z, 1.0f);
+    tape_c[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    ctot[slot] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(j + 1u));
+  }
+  __device__ __forceinline__ void scale(float sc) {
+    sigma = sigma * sc;
+    if (vj < tape_depth) reinterpret_cast<float*>(&tape_b[size_t(vj) * cap + slot])[3] = sigma;
   }
 };
 
@@ -431,6 +482,7 @@ k_camera(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBu
   wb.dir[i] = make_float4(d.x, d.y, d.z, __uint_as_float(pack_k(1u, true)));
   wb.pw[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
   wb.col[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (wb.ctot) wb.ctot[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // no vertex, no emission
   wb.qa[8u * wj.start[l] + (i - wj.start[l])] = i;
 }
 
@@ -529,11 +581,11 @@ k_shade(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBuf
         const uint32_t* __restrict__ qcur, uint32_t* qnext, GradArgs ga) {
   __shared__ uint32_t vst[kMaxWaveJobs * 8 + 1];
   __shared__ uint32_t bcnt[9], bbase[9];  // this workgroup's appends: 8 octant queues + shadow
-  GradLds* glds = nullptr;
+  EmitLds* glds = nullptr;
   if constexpr (kReplay) {
-    __shared__ GradLds glds_mem;  // only the replay instantiation reserves it
+    __shared__ EmitLds glds_mem;  // only the replay instantiation reserves it
     glds = &glds_mem;
-    grad_lds_init(glds_mem);
+    emit_lds_init(glds_mem);
   }
   WaveJobs const& wj = *wjp;
   uint32_t const nseg = wj.nj * 8u;
@@ -592,13 +644,12 @@ k_shade(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBuf
       if (kReplay) {
         GradReplay gr;
         gr.tex = J.s.tex;
-        gr.bsdfs = J.s.bsdfs;
         gr.w = f3(ga.d_film[3 * size_t(pixel)], ga.d_film[3 * size_t(pixel) + 1],
                   ga.d_film[3 * size_t(pixel) + 2]);
-        gr.ctot = ld3(wb.ctot[slot]);
-        gr.d_tex = ga.d_tex;
+        gr.unit = kPi * double(J.s.light_area);
         gr.d_emit = ga.d_emit;
         gr.lds = glds;
+        gr.bind(wb, slot);
         cont = shade_vertex(J.s, J.c, ps, h, pixel, sample, lv, &sr, &has_sr, gr);
       } else {
         NoGrad ng;
@@ -624,11 +675,11 @@ k_shade(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBuf
       wb.so[rq] = make_float4(sr.o.x, sr.o.y, sr.o.z, sr.nk2);
       wb.sw[rq] = make_float4(sr.w.x, sr.w.y, sr.w.z, sr.nk3);
       wb.spe[rq] = make_float4(sr.pe.x, sr.pe.y, sr.pe.z, __uint_as_float(slot));
-      wb.smeta[rq] = make_uint4(uint32_t(sr.expected), sr.type, sr.light_bsdf, 0u);
+      wb.smeta[rq] = make_uint4(uint32_t(sr.expected), sr.type, sr.light_bsdf, sr.vj);
       wb.smet[rq] = make_float4(sr.fsn, sr.fres, sr.mx, sr.gd);
     }
   }
-  if constexpr (kReplay) grad_lds_flush(*glds, ga.d_tex, ga.d_emit, 3 * jobs[wj.jlo].s.n_bsdfs);
+  if constexpr (kReplay) emit_lds_flush(*glds, ga.d_emit, 3 * jobs[wj.jlo].s.n_bsdfs);
 }
 
 template <bool kReplay>
@@ -637,11 +688,11 @@ k_shadow(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBu
   extern __shared__ uint32_t lds_stack[];
   __shared__ uint32_t vst[kMaxWaveJobs + 1];
   __shared__ uint32_t cnt[kMaxWaveJobs];
-  GradLds* glds = nullptr;
+  EmitLds* glds = nullptr;
   if constexpr (kReplay) {
-    __shared__ GradLds glds_mem;
+    __shared__ EmitLds glds_mem;
     glds = &glds_mem;
-    grad_lds_init(glds_mem);
+    emit_lds_init(glds_mem);
   }
   __shared__ unsigned long long keys[2 * kBlock];  // keys, then batch_owner's tables
   WaveJobs const& wj = *wjp;
@@ -701,26 +752,69 @@ k_shadow(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBu
     sr.fres = met.y;
     sr.mx = met.z;
     sr.gd = met.w;
-    float3 const add = nee_contribution(sr, h.t, s.light_area);
+    double wfac = 0.0;
+    float3 const add = nee_contribution(sr, h.t, s.light_area, kReplay ? &wfac : nullptr);
     uint32_t const slot = __float_as_uint(pe.w);
     if constexpr (kReplay) {
+      // the vertex's NEE term per unit throughput, and the sampled light's
+      // L_e derivative (pi A_lights in place of Phi)
       RJob const& J = jobs[wj.jlo + job];
       unsigned long long const item = wj.q0 + slot - J.base;
       uint32_t const pixel = uint32_t(item % J.wh);
-      GradReplay gr;
-      gr.tex = s.tex;
-      gr.bsdfs = s.bsdfs;
-      gr.w = f3(ga.d_film[3 * size_t(pixel)], ga.d_film[3 * size_t(pixel) + 1],
-                ga.d_film[3 * size_t(pixel) + 2]);
-      gr.d_tex = ga.d_tex;
-      gr.d_emit = ga.d_emit;
-      gr.lds = glds;
-      gr.emission(meta.z, add);
+      uint32_t const vj = meta.w;
+      if (vj < wb.tape_depth) {
+        size_t const ti = size_t(vj) * wb.cap + slot;
+        float4 const ta = wb.tape_a[ti], tb = wb.tape_b[ti];
+        float3 const pw = mul3(ld3(ta), ld3(tb));  // the throughput after the albedo
+        DevBsdf const& lb = s.bsdfs[meta.z];
+        wb.tape_c[ti] = make_float4(float(double(lb.phi[0]) * wfac), float(double(lb.phi[1]) * wfac),
+                                    float(double(lb.phi[2]) * wfac), 0.0f);
+        double const u = kPi * double(s.light_area) * wfac;
+        emit_add(glds, ga.d_emit, 3 * meta.z, ga.d_film[3 * size_t(pixel)] * float(double(pw.x) * u));
+        emit_add(glds, ga.d_emit, 3 * meta.z + 1, ga.d_film[3 * size_t(pixel) + 1] * float(double(pw.y) * u));
+        emit_add(glds, ga.d_emit, 3 * meta.z + 2, ga.d_film[3 * size_t(pixel) + 2] * float(double(pw.z) * u));
+      }
     }
     float4 const cc = wb.col[slot];
     wb.col[slot] = make_float4(cc.x + add.x, cc.y + add.y, cc.z + add.z, 0.0f);
   }
-  if constexpr (kReplay) grad_lds_flush(*glds, ga.d_tex, ga.d_emit, 3 * jobs[wj.jlo].s.n_bsdfs);
+  if constexpr (kReplay) emit_lds_flush(*glds, ga.d_emit, 3 * jobs[wj.jlo].s.n_bsdfs);
+}
+
+// The adjoint's reverse sweep over each slot's recorded path (one thread per
+// slot): with R = the emission of the light the path ended on (per unit
+// throughput), for vertex j from the last to the first
+//   g = Y_j + sigma_j R,   dS/d(albedo_j) += beta_j g,   R = albedo_j g,
+// where S is the pixel's film sum; the texel gradient is w * beta_j * g with w
+// = dLoss/dS.  No factor is divided out, so albedo channels at 0 get their
+// derivative too.
+__global__ void __launch_bounds__(kBlock)
+k_adjoint_sweep(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBufs wb,
+                GradArgs ga, uint32_t n) {
+  __shared__ TexLds lds;
+  tex_lds_init(lds);
+  uint32_t const i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) {
+    WaveJobs const& wj = *wjp;
+    RJob const& J = jobs[wj.jlo + job_of_slot(wj, i)];
+    unsigned long long const item = wj.q0 + i - J.base;
+    uint32_t const pixel = uint32_t(item % J.wh);
+    float3 const w = f3(ga.d_film[3 * size_t(pixel)], ga.d_film[3 * size_t(pixel) + 1],
+                        ga.d_film[3 * size_t(pixel) + 2]);
+    float4 const e = wb.ctot[i];
+    uint32_t cnt = __float_as_uint(e.w);
+    cnt = cnt < wb.tape_depth ? cnt : wb.tape_depth;
+    float3 R = ld3(e);
+    for (uint32_t j = cnt; j-- > 0;) {
+      size_t const ti = size_t(j) * wb.cap + i;
+      float4 const a = wb.tape_a[ti], b = wb.tape_b[ti], c = wb.tape_c[ti];
+      float3 const g = add3(ld3(c), scl3(R, b.w));
+      uint32_t const key = __float_as_uint(a.w);
+      if (key != kEmptyKey) tex_add(&lds, ga.d_tex, key, mul3(w, mul3(ld3(a), g)));
+      R = mul3(ld3(b), g);
+    }
+  }
+  tex_lds_flush(lds, ga.d_tex);
 }
 
 // End of a bounce: the next queues become current; their padded prefix for
@@ -797,10 +891,6 @@ k_accumulate(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp,
   film[3 * size_t(p) + 2] = fz;
 }
 
-__global__ void __launch_bounds__(kBlock) k_copy_col(float4* dst, const float4* src, uint32_t n) {
-  uint32_t const i = blockIdx.x * kBlock + threadIdx.x;
-  if (i < n) dst[i] = src[i];
-}
 
 // ------------------------------------------------------------------ host side
 int g_num_cu = 0;
@@ -907,11 +997,15 @@ struct DevMem {
 
 size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
 
-int alloc_wave(uint32_t cap, bool replay, hipStream_t st, DevMem* mem, WaveBufs* wb) {
+// tape_depth > 0: the adjoint's buffers (ctot and the vertex tape).
+int alloc_wave(uint32_t cap, uint32_t tape_depth, hipStream_t st, DevMem* mem, WaveBufs* wb) {
+  bool const replay = tape_depth > 0;
   size_t const f4 = align_up(size_t(cap) * 16);
   size_t const u4 = align_up(size_t(cap) * 16);
   size_t const qsz = align_up(size_t(cap) * 8 * 4);
-  size_t const total = f4 * (replay ? 10 : 9) + u4 + 2 * qsz + align_up(sizeof(WaveCounters));
+  size_t const tape = replay ? align_up(size_t(cap) * tape_depth * 16) : 0;
+  size_t const total = f4 * (replay ? 10 : 9) + u4 + 2 * qsz + align_up(sizeof(WaveCounters)) +
+                       3 * tape;
   hipError_t e = pool_alloc(&mem->p, total, st);
   if (e != hipSuccess) return hip_fail(e, "pool_alloc(wave)");
   mem->st = st;
@@ -936,6 +1030,10 @@ int alloc_wave(uint32_t cap, bool replay, hipStream_t st, DevMem* mem, WaveBufs*
   wb->qb = reinterpret_cast<uint32_t*>(take(qsz));
   wb->ctr = reinterpret_cast<WaveCounters*>(take(align_up(sizeof(WaveCounters))));
   wb->cap = cap;
+  wb->tape_a = replay ? reinterpret_cast<float4*>(take(tape)) : nullptr;
+  wb->tape_b = replay ? reinterpret_cast<float4*>(take(tape)) : nullptr;
+  wb->tape_c = replay ? reinterpret_cast<float4*>(take(tape)) : nullptr;
+  wb->tape_depth = tape_depth;
   return XYZ_OK;
 }
 
@@ -978,6 +1076,36 @@ struct EventSet {
   }
   ~EventSet() {
     for (hipEvent_t x : e) (void)hipEventDestroy(x);
+  }
+};
+
+// After a fork onto internal streams: records an event on each lane and
+// makes the caller's stream wait for it.  join() does so and reports errors;
+// an armed guard that goes out of scope does so on the way out (error paths),
+// so that memory released on the caller's stream afterwards is not in use.
+struct LaneJoin {
+  hipStream_t st = nullptr;
+  const hipStream_t* ls = nullptr;
+  const hipEvent_t* done = nullptr;
+  int n = 0;
+  void arm(hipStream_t s, const hipStream_t* lanes, const hipEvent_t* ev, int count) {
+    st = s;
+    ls = lanes;
+    done = ev;
+    n = count;
+  }
+  hipError_t join() {
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < n; ++i) {
+      hipError_t r = hipEventRecord(done[i], ls[i]);
+      if (r == hipSuccess) r = hipStreamWaitEvent(st, done[i], 0);
+      if (r != hipSuccess && e == hipSuccess) e = r;
+    }
+    n = 0;
+    return e;
+  }
+  ~LaneJoin() {
+    if (n > 0) (void)join();
   }
 };
 
@@ -1039,7 +1167,16 @@ int render_jobs(std::vector<JobSpec> js, hipStream_t st, bool backward, GradArgs
     x.j.base = total;
     total += x.items;
   }
-  uint32_t const cap = uint32_t(std::min<unsigned long long>(total, kWaveCap));
+  // the adjoint's vertex tape: 48 bytes per slot and shading vertex, at most
+  // kTapeBudget bytes per lane
+  uint32_t tape_depth = 0;
+  unsigned long long slot_cap = kWaveCap;
+  if (backward) {
+    for (JobSpec const& x : jobs) tape_depth = std::max(tape_depth, x.max_depth);
+    slot_cap = std::max<unsigned long long>(
+        kBlock, std::min<unsigned long long>(kWaveCap, kTapeBudget / (48ull * tape_depth)));
+  }
+  uint32_t const cap = uint32_t(std::min<unsigned long long>(total, slot_cap));
   // wave plan
   std::vector<WaveJobs> waves;
   std::vector<uint32_t> wdepth, wstack, wmaxwh;
@@ -1093,16 +1230,22 @@ int render_jobs(std::vector<JobSpec> js, hipStream_t st, bool backward, GradArgs
   DevMem mem[kLanes];
   WaveBufs wb[kLanes];
   for (int i = 0; i < lanes; ++i) {
-    rc = alloc_wave(cap, backward, st, &mem[i], &wb[i]);
+    rc = alloc_wave(cap, tape_depth, st, &mem[i], &wb[i]);
     if (rc != XYZ_OK) return rc;
     hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, wb[i].ctr);
   }
   EventSet ev;
+  // Once the lanes are forked, every return makes the caller's stream wait
+  // for them before the wave buffers and tables above are released (their
+  // destructors free on the caller's stream).
+  LaneJoin join;
   if (lanes > 1) {
     rc = ev.create(2 * kLanes + 1);
     if (rc != XYZ_OK) return rc;
     hipEvent_t const fork = ev.e[2 * kLanes];
     e = hipEventRecord(fork, st);
+    if (e != hipSuccess) return hip_fail(e, "render fork");
+    join.arm(st, ls, &ev.e[kLanes], lanes);
     for (int i = 0; i < lanes && e == hipSuccess; ++i) e = hipStreamWaitEvent(ls[i], fork, 0);
     if (e != hipSuccess) return hip_fail(e, "render fork");
   }
@@ -1115,9 +1258,17 @@ int render_jobs(std::vector<JobSpec> js, hipStream_t st, bool backward, GradArgs
     uint32_t const grid = (n + kBlock - 1) / kBlock;
     hipLaunchKernelGGL(k_init_wave, dim3(1), dim3(512), 0, s, wj, wb[i].ctr);
     hipLaunchKernelGGL(k_camera, dim3(grid), dim3(kBlock), 0, s, jobs_dev, wj, wb[i], n);
+    if (backward) {
+      // the paths are traced again from their seeds, each slot recording its
+      // vertices, then swept in reverse
+      rc = run_bounces<true>(jobs_dev, wj, wb[i], n, wdepth[k], wstack[k], ga, s);
+      if (rc != XYZ_OK) return rc;
+      hipLaunchKernelGGL(k_adjoint_sweep, dim3(grid), dim3(kBlock), 0, s, jobs_dev, wj, wb[i], ga, n);
+      continue;
+    }
     rc = run_bounces<false>(jobs_dev, wj, wb[i], n, wdepth[k], wstack[k], none, s, guided);
     if (rc != XYZ_OK) return rc;
-    if (!backward) {
+    {
       // wave k is added to the films after wave k - 1 (sample order per pixel)
       if (lanes > 1 && k > 0) {
         e = hipStreamWaitEvent(s, ev.e[(k - 1) % size_t(lanes)], 0);
@@ -1129,22 +1280,11 @@ int render_jobs(std::vector<JobSpec> js, hipStream_t st, bool backward, GradArgs
         e = hipEventRecord(ev.e[i], s);
         if (e != hipSuccess) return hip_fail(e, "render record");
       }
-    } else {
-      // replay: same paths from their seeds, now with the primal colours known
-      hipLaunchKernelGGL(k_copy_col, dim3(grid), dim3(kBlock), 0, s, wb[i].ctot, wb[i].col, n);
-      hipLaunchKernelGGL(k_init_wave, dim3(1), dim3(512), 0, s, wj, wb[i].ctr);
-      hipLaunchKernelGGL(k_camera, dim3(grid), dim3(kBlock), 0, s, jobs_dev, wj, wb[i], n);
-      rc = run_bounces<true>(jobs_dev, wj, wb[i], n, wdepth[k], wstack[k], ga, s);
-      if (rc != XYZ_OK) return rc;
     }
   }
   if (lanes > 1) {  // join: the caller's stream waits for every lane
-    for (int i = 0; i < lanes; ++i) {
-      hipEvent_t const done = ev.e[kLanes + i];
-      e = hipEventRecord(done, ls[i]);
-      if (e == hipSuccess) e = hipStreamWaitEvent(st, done, 0);
-      if (e != hipSuccess) return hip_fail(e, "render join");
-    }
+    e = join.join();
+    if (e != hipSuccess) return hip_fail(e, "render join");
   }
   if (queries) {
     unsigned long long q[kLanes] = {};
