@@ -182,6 +182,18 @@ int xyz_closest_hit(xyz_scene *scene, uint32_t n, const float *org, const float 
  * triangle tests executed, cluster box tests executed.  The last two differ
  * from the second because leaf clusters that provably cannot hold an accepted
  * hit are skipped. */
+/* Allocation-free xyz_closest_hit (no counterpart in the reference), for
+ * callers that cannot allocate device memory: code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc.  Same arguments and results
+ * as xyz_closest_hit; the kernel's work counters (its only scratch memory) live
+ * in a fixed 2 KiB device array instead of pool memory, so the call makes no
+ * allocation.  At most 2^31 rays per call (XYZ_E_UNSUPPORTED beyond).  Every
+ * call on one GPU shares that GPU's array: calls that overlap on one GPU (on
+ * different streams) race and must not be issued. */
+int xyz_closest_hit_noalloc(xyz_scene *scene, uint32_t n, const float *org, const float *dir,
+                            int32_t *prim, float *t, float *f, float *g, uint8_t *back,
+                            uint32_t *counters, void *stream);
+
 int xyz_closest_hit_work(xyz_scene *scene, uint32_t n, const float *org, const float *dir,
                          int32_t *prim, float *t, unsigned long long *work, void *stream);
 

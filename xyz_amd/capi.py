@@ -113,7 +113,7 @@ class BatchTuning(ctypes.Structure):
 EXPORTS = ["xyz_kdtree_build", "xyz_kdtree_build_device", "xyz_kdtree_info", "xyz_kdtree_export", "xyz_kdtree_destroy",
            "xyz_mqo_load", "xyz_mqo_load_file", "xyz_mqo_describe", "xyz_mqo_destroy",
            "xyz_scene_create", "xyz_scene_get_info", "xyz_scene_destroy", "xyz_closest_hit",
-           "xyz_closest_hit_work", "xyz_check_reciprocal", "xyz_eval_pow",
+           "xyz_closest_hit_work", "xyz_closest_hit_noalloc", "xyz_check_reciprocal", "xyz_eval_pow",
            "xyz_importon_map_create", "xyz_importon_map_from_host", "xyz_importon_map_info",
            "xyz_importon_map_export", "xyz_importance_query", "xyz_importon_map_destroy",
            "xyz_render", "xyz_render_idpt", "xyz_render_bdpt", "xyz_render_backward", "xyz_render_batch",
@@ -156,6 +156,7 @@ def lib():
         L.xyz_closest_hit.argtypes = [_vp, ctypes.c_uint32, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                       _vp, _vp]
         L.xyz_closest_hit_work.argtypes = [_vp, ctypes.c_uint32, _vp, _vp, _vp, _vp, _vp, _vp]
+        L.xyz_closest_hit_noalloc.argtypes = L.xyz_closest_hit.argtypes
         L.xyz_check_reciprocal.argtypes = [ctypes.POINTER(ctypes.c_ulonglong), _vp]
         L.xyz_eval_pow.argtypes = [ctypes.c_uint32, _vp, _vp, _vp, _vp]
         L.xyz_render.argtypes = [_vp, ctypes.POINTER(Camera), ctypes.POINTER(RenderParams), _vp,

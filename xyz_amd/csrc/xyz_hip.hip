@@ -186,6 +186,76 @@ int launch_closest_hit(const DevScene& s, uint32_t n, const float* org, const fl
   return XYZ_OK;
 }
 
+// ------------------------------------------------- allocation-free variant
+// The closest hit's work counters (one per part of the rays) in a fixed device
+// array instead of pool memory, for callers that cannot allocate.  Each GPU
+// holds its own copy; its address comes from hipGetSymbolAddress once per
+// device, and host code uses that address, never the array's name.
+constexpr uint32_t kScratchWords = 64u * WaveFeeder::kFeedParts;  // 2 KiB
+static __device__ __align__(256) uint32_t scratch[kScratchWords];
+// Rays per call: the counters are 32-bit and overshoot their part's end by at
+// most one batch of 64 per wave of the grid, so 2^31 rays leave them far from
+// wrapping.
+constexpr uint64_t kNoAllocMaxRays = 1ull << 31;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_ptr[kMaxDevices] = {};
+
+// The device's copy of `scratch`, looked up once per device (the device is
+// current).
+int scratch_address(uint32_t** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("closest_hit_noalloc: device index out of range");
+    return XYZ_E_UNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[dev] = static_cast<uint32_t*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return XYZ_OK;
+}
+
+int launch_closest_hit_noalloc(const DevScene& s, uint32_t n, const float* org, const float* dir,
+                               int32_t* prim, float* t, float* f, float* g, uint8_t* back,
+                               uint32_t* counters, hipStream_t stream) {
+  if (uint64_t(n) > kNoAllocMaxRays) {
+    set_error("xyz_closest_hit_noalloc: n exceeds the maximum of 2^31 rays per call");
+    return XYZ_E_UNSUPPORTED;
+  }
+  uint32_t* w = nullptr;
+  int rc = scratch_address(&w);
+  if (rc != XYZ_OK) return rc;
+  if (n == 0) return XYZ_OK;
+  if (g_hit_cu == 0) {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
+      v = 256;
+    g_hit_cu = v;
+  }
+  size_t const lds = size_t(s.stack_cap) * kHitBlock * sizeof(uint32_t);
+  uint32_t const need_blocks = (n + kHitBlock - 1) / kHitBlock;
+  uint32_t const grid = std::min<uint32_t>(need_blocks, uint32_t(g_hit_cu) * kHitBlocksPerCu);
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, w);  // a kernel, not a memset
+  if (counters) {
+    hipLaunchKernelGGL(closest_hit_kernel<true>, dim3(grid), dim3(kHitBlock), lds, stream, s, n,
+                       org, dir, prim, t, f, g, back, counters, nullptr, w);
+  } else {
+    hipLaunchKernelGGL(closest_hit_kernel<false>, dim3(grid), dim3(kHitBlock), lds, stream, s, n,
+                       org, dir, prim, t, f, g, back, counters, nullptr, w);
+  }
+  hipError_t const e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "closest_hit_noalloc launch");
+  return XYZ_OK;
+}
+
 // ------------------------------------------------------------ scene upload
 namespace {
 
@@ -679,6 +749,21 @@ int xyz_closest_hit(xyz_scene* scene, uint32_t n, const float* org, const float*
   }
   return launch_closest_hit(scene->dev, n, org, dir, prim, t, f, g, back, counters, nullptr,
                             static_cast<hipStream_t>(stream));
+}
+
+int xyz_closest_hit_noalloc(xyz_scene* scene, uint32_t n, const float* org, const float* dir,
+                            int32_t* prim, float* t, float* f, float* g, uint8_t* back,
+                            uint32_t* counters, void* stream) {
+  if (!scene) {
+    set_error("xyz_closest_hit_noalloc: NULL scene");
+    return XYZ_E_INVALID;
+  }
+  if (n > 0 && (!org || !dir || !prim)) {
+    set_error("xyz_closest_hit_noalloc: NULL ray or prim buffer");
+    return XYZ_E_INVALID;
+  }
+  return launch_closest_hit_noalloc(scene->dev, n, org, dir, prim, t, f, g, back, counters,
+                                    static_cast<hipStream_t>(stream));
 }
 
 int xyz_closest_hit_work(xyz_scene* scene, uint32_t n, const float* org, const float* dir,
