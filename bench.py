@@ -19,6 +19,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# VALU issue: 256 CUs x 4 SIMDs; a wave64 VALU instruction occupies a 16-lane
+# SIMD for 4 cycles; 2.4 GHz peak engine clock
+N_SIMDS, CLOCK_GHZ, CYCLES_PER_WAVE_INSTR = 1024, 2.4, 4
+VALU_PEAK_GINSTR = N_SIMDS * CLOCK_GHZ / CYCLES_PER_WAVE_INSTR  # 614.4 G wave instructions/s
 
 
 def parse():
@@ -43,7 +47,7 @@ def parse():
                     help="skip the bidirectional path-tracing leg")
     ap.add_argument("--skip-idpt", action="store_true",
                     help="skip the importance-driven path-tracing leg")
-    ap.add_argument("--pt-spp", type=int, default=16,
+    ap.add_argument("--pt-spp", type=int, default=256,
                     help="samples per pixel timed in the 1080p leg (the workload renders 256)")
     ap.add_argument("--pt-cpu-rows", type=int, default=128,
                     help="image rows the CPU baseline renders at 1 spp in the 1080p leg")
@@ -156,26 +160,8 @@ def main():
                            tris_per_ray=round(float(c[n + a:n + b].mean()), 2))
     log(f"[bench] split: {split}")
 
-    achieved = alg_bytes / (ms_avg * 1e-3) / 1e9
-    achieved_x = exec_bytes / (ms_avg * 1e-3) / 1e9
-    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
-                    unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                    executed=dict(achieved=round(achieved_x, 1),
-                                  frac=round(achieved_x / HBM_PEAK_GBS, 4),
-                                  bytes_per_launch=int(exec_bytes),
-                                  tris_tested_per_ray=round(w_tris / n, 2),
-                                  boxes_tested_per_ray=round(w_boxes / n, 2)))
-    # compulsory-only floor: the rays in and the results out (32 B per ray) plus
-    # one pass over the scene's device data
-    floor_bytes = 32 * n + int(info["device_bytes"])
-    roofline["floor"] = dict(bytes_per_launch=floor_bytes,
-                             ms_at_peak=round(floor_bytes / (HBM_PEAK_GBS * 1e9) * 1e3, 4),
-                             frac_of_launch=round(floor_bytes / (HBM_PEAK_GBS * 1e9) /
-                                                  (ms_avg * 1e-3), 4))
-    pmc = os.path.join(ROOT, "profiles", "closest_hit_pmc.json")
-    if os.path.exists(pmc):
-        with open(pmc) as fh:
-            roofline["traffic"] = json.load(fh).get("hbm_bytes_per_launch")
+    roofline = closest_hit_roofline(n, ms_avg, alg_bytes, exec_bytes, w_tris, w_boxes,
+                                    int(info["device_bytes"]))
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -421,6 +407,52 @@ def bench_idpt(args, hip, capi, scenes):
     return out
 
 
+def closest_hit_roofline(n, ms_avg, alg_bytes, exec_bytes, w_tris, w_boxes, device_bytes):
+    """The closest-hit kernel's ceilings.  It is bound by VALU issue (divergent
+    wave-cooperative traversal), not by HBM: `achieved` is its VALU wave
+    instructions per launch (rocprofv3 SQ_INSTS_VALU, profiles/
+    closest_hit_pmc.json) over the launch time measured here, against 1024
+    SIMDs x 2.4 GHz / 4 cycles.  The HBM fraction (measured FETCH_SIZE +
+    WRITE_SIZE bytes over the same time), the L2 hit rate, and the byte counts
+    of the reference's work and of the work executed sit beside it."""
+    s = ms_avg * 1e-3
+    pmc_path = os.path.join(ROOT, "profiles", "closest_hit_pmc.json")
+    pmc = {}
+    if os.path.exists(pmc_path):
+        with open(pmc_path) as fh:
+            pmc = json.load(fh)
+    valu = pmc.get("sq_insts_valu_per_launch")
+    traffic = pmc.get("hbm_bytes_per_launch")
+    r = dict(bound="valu", achieved=None, peak=VALU_PEAK_GINSTR, unit="G wave-instr/s", frac=None,
+             traffic=traffic, source="profiles/closest_hit_pmc.json")
+    if valu:
+        a = valu / s / 1e9
+        r.update(achieved=round(a, 1), frac=round(a / VALU_PEAK_GINSTR, 4),
+                 valu_instr_per_ray=round(valu / n, 1),
+                 lanes_active_per_valu_instr=pmc.get("lanes_active_per_valu_instruction"),
+                 valu_busy=pmc.get("valu_busy"))
+    if traffic:
+        r["hbm"] = dict(achieved=round(traffic / s / 1e9, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                        frac=round(traffic / s / 1e9 / HBM_PEAK_GBS, 4))
+    if "l2_hit_rate" in pmc:
+        r["l2_hit_rate"] = pmc["l2_hit_rate"]
+    # the reference's work at SURVEY-style bytes (32 + 8 nodes + 40 triangle tests
+    # per ray): culling executes 38 % of those tests, so this rate is not a bound
+    r["reference_work"] = dict(bytes_per_launch=int(alg_bytes),
+                               gb_per_s=round(alg_bytes / s / 1e9, 1))
+    r["executed"] = dict(bytes_per_launch=int(exec_bytes),
+                         gb_per_s=round(exec_bytes / s / 1e9, 1),
+                         tris_tested_per_ray=round(w_tris / n, 2),
+                         boxes_tested_per_ray=round(w_boxes / n, 2))
+    # compulsory-only floor: the rays in and the results out (32 B per ray) plus
+    # one pass over the scene's device data
+    floor_bytes = 32 * n + device_bytes
+    r["floor"] = dict(bytes_per_launch=floor_bytes,
+                      ms_at_peak=round(floor_bytes / (HBM_PEAK_GBS * 1e9) * 1e3, 4),
+                      frac_of_launch=round(floor_bytes / (HBM_PEAK_GBS * 1e9) / s, 4))
+    return r
+
+
 def bench_pt(args, hip, capi, scenes):
     """Path tracing at 1920x1080 on the ~2M-triangle production scene: max depth
     16, Russian roulette after depth 4.  Msamples/s over `--pt-spp` samples per
@@ -440,10 +472,11 @@ def bench_pt(args, hip, capi, scenes):
     samples = W * H * args.pt_spp
     rate = samples / (ms * 1e-3) / 1e6
     res = dict(msamples_per_s=round(rate, 2), ms=round(ms, 1), spp=args.pt_spp,
-               frame_256spp_s=round(W * H * 256 / (rate * 1e6), 2),
+               frame_s=round(ms * 1e-3, 3),
                queries_per_sample=round(st["queries"] / samples, 3),
                mrays_per_s=round(st["queries"] / (ms * 1e-3) / 1e6, 1),
                n_tris=int(info["n_tris"]), config="1920x1080, depth 16, RR after 4")
+    res["roofline"] = pt_roofline(rate)
     log(f"[bench] pt 1080p: {res}")
     if not args.no_cpu_baseline:
         from oracle import oracle
@@ -473,8 +506,42 @@ def bench_pt(args, hip, capi, scenes):
             single_thread=dict(value=round(n1 / ost1["seconds"] / 1e6, 4), cores=1,
                                sample=f"rows {rows1[0]}-{rows1[1] - 1} at 1 spp ({n1} samples)"))
         res["vs_cpu"] = round(rate / res["cpu_baseline"]["value"], 1)
+        # the reference's work per sample (SURVEY-style bytes), counted by the
+        # oracle on its band: 32 + 8 nodes + 40 triangle tests per query, 64 per
+        # shaded vertex, the film's 12 bytes per pixel over the spp
+        bs = (32 * ost["queries"] + 8 * ost["nodes"] + 40 * ost["tris"] + 64 * ost["vertices"]) / n
+        bs += 12.0 / args.pt_spp
+        res["roofline"]["reference_work"] = dict(
+            bytes_per_sample=round(bs, 1), gb_per_s=round(bs * rate * 1e6 / 1e9, 1),
+            nodes_per_query=round(ost["nodes"] / ost["queries"], 1),
+            tris_per_query=round(ost["tris"] / ost["queries"], 1),
+            vertices_per_sample=round(ost["vertices"] / n, 3),
+            sample=f"oracle counts on rows {rows[0]}-{rows[1] - 1} at 1 spp")
         log(f"[bench] pt cpu baseline: {res['cpu_baseline']}")
     return res
+
+
+def pt_roofline(rate_msamples):
+    """The path tracer's kernels (extend, shade, shadow) at the frame rate just
+    measured: HBM bytes per sample from rocprofv3 FETCH_SIZE + WRITE_SIZE on the
+    same frame (profiles/pt_1080p_pmc.json) give the HBM fraction; each
+    kernel's VALU busy, lanes per instruction and L2 hit rate come from the same
+    file.  None of the three kernels saturates HBM or VALU issue: the traversal
+    kernels stall on memory latency (L2 hit rate 0.57-0.74)."""
+    path = os.path.join(ROOT, "profiles", "pt_1080p_pmc.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        pmc = json.load(fh)
+    bps = pmc["hbm_bytes_per_sample"]
+    a = bps * rate_msamples * 1e6 / 1e9
+    return dict(bound="hbm", achieved=round(a, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(a / HBM_PEAK_GBS, 4), traffic_bytes_per_sample=bps,
+                kernels={k: dict(valu_busy=v["valu_busy"],
+                                 lanes_active_per_valu_instr=v["lanes_active_per_valu_instruction"],
+                                 l2_hit_rate=v["l2_hit_rate"])
+                         for k, v in pmc["kernels"].items()},
+                source="profiles/pt_1080p_pmc.json")
 
 
 def bench_adjoint(args, hip, capi, scenes):
@@ -507,8 +574,11 @@ def bench_adjoint(args, hip, capi, scenes):
 
 def bench_batch(args, hip, capi, scenes):
     """64 generated scenes (about 1e3..5e5 triangles, 128^2..1024^2, 16..256 spp,
-    depth 8): one persistent graph-captured launch (autotuned) against the same
-    renders one after another through the wavefront entry point."""
+    depth 8): xyz_render_batch with the autotuner's choice (the shared-wave
+    engine, replayed from a captured hipGraph; the first call captures, the
+    timed call is one graph launch) against the same renders one after another
+    through xyz_render.  `ideal_batched_over_sequential` is the ratio if every
+    job ran at the rate of the job with the most samples."""
     D = 8
     jobs = [scenes.batch_scene(s) for s in range(64)]
     gs = [capi.Scene.from_dict(j) for j in jobs]
@@ -524,11 +594,18 @@ def bench_batch(args, hip, capi, scenes):
     for f in films:
         f.zero()
     e0, e1 = hip.Event(), hip.Event()
+    ev = [hip.Event() for _ in range(len(gs) + 1)]
     e0.record(None)
-    for s, c, f, sp, sd in zip(gs, cams, films, spps, seeds):
+    ev[0].record(None)
+    for k, (s, c, f, sp, sd) in enumerate(zip(gs, cams, films, spps, seeds)):
         s.render(c, f, sp, D, rr_start=3, seed=sd)
+        ev[k + 1].record(None)
     e1.record(None)
     seq_ms = e0.elapsed_ms(e1)
+    job_ms = [ev[k].elapsed_ms(ev[k + 1]) for k in range(len(gs))]
+    job_samples = [j["width"] * j["height"] * j["spp"] for j in jobs]
+    big = int(np.argmax(job_samples))
+    big_rate = job_samples[big] / (job_ms[big] * 1e-3)
     seq_films = [f.numpy() for f in films]
     # batched: one graph launch (first call captures; the timed call replays)
     for f in films:
@@ -547,6 +624,11 @@ def bench_batch(args, hip, capi, scenes):
                batched_msamples_per_s=round(samples / (bat_ms * 1e-3) / 1e6, 1),
                sequential_msamples_per_s=round(samples / (seq_ms * 1e-3) / 1e6, 1),
                samples=int(samples), films_bit_identical=bool(same),
+               ideal_batched_over_sequential=round(seq_ms / (samples / big_rate * 1e3), 3),
+               largest_job=dict(samples=int(job_samples[big]), ms=round(job_ms[big], 1),
+                                msamples_per_s=round(big_rate / 1e6, 1)),
+               smallest_job_msamples_per_s=round(
+                   min(a / (b * 1e-3) for a, b in zip(job_samples, job_ms)) / 1e6, 1),
                tuning=dict(engine=["persistent megakernel", "shared wavefront"][best.engine],
                            tile_pixels=best.tile_pixels, blocks_per_cu=best.blocks_per_cu,
                            block_threads=best.block_threads, use_graph=best.use_graph),

@@ -234,12 +234,13 @@ class OracleScene:
         if film is None:
             film = np.zeros((H, W, 3), np.float32)
         y0, y1 = (0, H) if rows is None else rows
-        st = np.zeros(3, np.uint64)
+        st = np.zeros(4, np.uint64)
         secs = lib().orc_render(self.h, _p(cam, _f32p), W, H, spp, sample0, max_depth, rr_start,
                                 rr_threshold, seed, y0, y1, _p(film, _f32p), _p(st, _u64p),
                                 threads)
         if stats:
-            return film, dict(seconds=secs, queries=int(st[0]), nodes=int(st[1]), tris=int(st[2]))
+            return film, dict(seconds=secs, queries=int(st[0]), nodes=int(st[1]), tris=int(st[2]),
+                              vertices=int(st[3]))
         return film
 
     def render_bdpt(self, camera, W, H, spp, max_depth, seed=0, sample0=0, threads=0,

@@ -748,7 +748,7 @@ struct PathRecord {
   std::vector<double> d_emission;  // 3 per BSDF: dcolour / dL_e
 };
 
-struct PathStats { uint32_t queries = 0; Counters c; };
+struct PathStats { uint32_t queries = 0; uint32_t vertices = 0; Counters c; };
 
 // ------------------------------------------------------------------ IDPT
 // Importance-driven path tracing (Scene::EvaluteImportanceDrivenPathTracing,
@@ -1065,6 +1065,7 @@ V3 pt_path(Scene const& sc, Camera const& cam, PTParams const& pp, uint32_t W, u
       break;
     }
     if (++k >= pp.kmax) break;
+    if (st) st->vertices++;
 
     uint32_t const j = k - 1;  // vertex index, 1-based
     Draw dj = draw(pixel, pp.seed, sample, j);
@@ -1959,8 +1960,8 @@ void orc_primary_dir(const float* cam, uint32_t W, uint32_t H, float s, float t,
 // Progressive path tracing (xyz/integrator/integrator.cpp:119-128) over rows
 // [y0, y1) for samples [sample0, sample0 + spp).  film (W*H*3) receives the
 // per-pixel SUM of sample colours, added sample pass by sample pass.
-// stats[0] = closest-hit queries, stats[1] = nodes visited, stats[2] = tris tested.
-// Returns wall seconds.
+// stats[0] = closest-hit queries, stats[1] = nodes visited, stats[2] = tris tested,
+// stats[3] = shaded vertices (hits that reach the NEE step).  Returns wall seconds.
 double orc_render(void* s, const float* cam, uint32_t W, uint32_t H, uint32_t spp,
                   uint32_t sample0, uint32_t max_depth, uint32_t rr_start, float rr_threshold,
                   uint64_t seed, uint32_t y0, uint32_t y1, float* film, uint64_t* stats,
@@ -1973,7 +1974,7 @@ double orc_render(void* s, const float* cam, uint32_t W, uint32_t H, uint32_t sp
   pp.rr_threshold = rr_threshold;
   pp.seed = seed;
   int const nt = n_threads(threads);
-  std::atomic<uint64_t> q(0), nn(0), ntr(0);
+  std::atomic<uint64_t> q(0), nn(0), ntr(0), nv(0);
   auto t0 = std::chrono::steady_clock::now();
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
   for (int64_t y = int64_t(y0); y < int64_t(y1); ++y) {
@@ -1985,9 +1986,9 @@ double orc_render(void* s, const float* cam, uint32_t W, uint32_t H, uint32_t sp
         px[0] += col[0]; px[1] += col[1]; px[2] += col[2];
       }
     }
-    if (stats) { q += ps.queries; nn += ps.c.nodes; ntr += ps.c.tris; }
+    if (stats) { q += ps.queries; nn += ps.c.nodes; ntr += ps.c.tris; nv += ps.vertices; }
   }
-  if (stats) { stats[0] = q; stats[1] = nn; stats[2] = ntr; }
+  if (stats) { stats[0] = q; stats[1] = nn; stats[2] = ntr; stats[3] = nv; }
   return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 

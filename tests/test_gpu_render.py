@@ -160,21 +160,26 @@ def test_batch_equals_sequential(oracle_mod, capi_mod, device):
         f = DeviceArray.zeros((H, W, 3), np.float32)
         sc.render(cam, f, spp, 8, rr_start=3, seed=11)
         seq.append(f.numpy())
-    for engine, use_graph in ((0, 0), (0, 1), (1, 0)):
+    for engine, use_graph in ((0, 0), (0, 1), (1, 0), (1, 1)):
         films = [DeviceArray.zeros((j[2], j[1], 3), np.float32) for j in jobs]
         t = capi_mod.BatchTuning(64, 2, 128, use_graph, engine)
         capi_mod.render_batch(scenes, cams, films, [j[3] for j in jobs], 8, rr_start=3,
                               seeds=[11, 11, 11], tuning=t)
         for f, s in zip(films, seq):
             assert np.array_equal(f.numpy().view(np.uint32), s.view(np.uint32))
-    # graph replays: each replay adds the same samples again (film += sum)
-    films = [DeviceArray.zeros((j[2], j[1], 3), np.float32) for j in jobs]
-    t = capi_mod.BatchTuning(64, 2, 128, 1)
-    for rep in range(3):
-        capi_mod.render_batch(scenes, cams, films, [j[3] for j in jobs], 8, rr_start=3,
-                              seeds=[11, 11, 11], tuning=t)
-        for f, s in zip(films, seq):
-            assert np.allclose(f.numpy(), s * (rep + 1), rtol=1e-5, atol=1e-6)
+    # graph replays of both engines: each replay adds the same samples again
+    # (film += sum); eager renders and synchronisations between the replays
+    for engine in (0, 1):
+        films = [DeviceArray.zeros((j[2], j[1], 3), np.float32) for j in jobs]
+        t = capi_mod.BatchTuning(64, 2, 128, 1, engine)
+        for rep in range(3):
+            capi_mod.render_batch(scenes, cams, films, [j[3] for j in jobs], 8, rr_start=3,
+                                  seeds=[11, 11, 11], tuning=t)
+            for f, s in zip(films, seq):
+                assert np.allclose(f.numpy(), s * (rep + 1), rtol=1e-5, atol=1e-6), (engine, rep)
+            other = DeviceArray.zeros((16, 16, 3), np.float32)
+            scenes[0].render(cams[0], other, 2, 8, rr_start=3, seed=rep)
+            other.numpy()
 
 
 def test_render_errors(capi_mod, device):

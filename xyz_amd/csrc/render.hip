@@ -997,19 +997,25 @@ struct DevMem {
 
 size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
 
-// tape_depth > 0: the adjoint's buffers (ctot and the vertex tape).
-int alloc_wave(uint32_t cap, uint32_t tape_depth, hipStream_t st, DevMem* mem, WaveBufs* wb) {
+// Bytes of one lane's wave buffers; tape_depth > 0: with the adjoint's
+// buffers (ctot and the vertex tape).
+size_t wave_bytes(uint32_t cap, uint32_t tape_depth) {
   bool const replay = tape_depth > 0;
   size_t const f4 = align_up(size_t(cap) * 16);
   size_t const u4 = align_up(size_t(cap) * 16);
   size_t const qsz = align_up(size_t(cap) * 8 * 4);
   size_t const tape = replay ? align_up(size_t(cap) * tape_depth * 16) : 0;
-  size_t const total = f4 * (replay ? 10 : 9) + u4 + 2 * qsz + align_up(sizeof(WaveCounters)) +
-                       3 * tape;
-  hipError_t e = pool_alloc(&mem->p, total, st);
-  if (e != hipSuccess) return hip_fail(e, "pool_alloc(wave)");
-  mem->st = st;
-  char* b = static_cast<char*>(mem->p);
+  return f4 * (replay ? 10 : 9) + u4 + 2 * qsz + align_up(sizeof(WaveCounters)) + 3 * tape;
+}
+
+// Lays one lane's wave buffers out in `mem` (wave_bytes(cap, tape_depth) bytes).
+void carve_wave(void* mem, uint32_t cap, uint32_t tape_depth, WaveBufs* wb) {
+  bool const replay = tape_depth > 0;
+  size_t const f4 = align_up(size_t(cap) * 16);
+  size_t const u4 = align_up(size_t(cap) * 16);
+  size_t const qsz = align_up(size_t(cap) * 8 * 4);
+  size_t const tape = replay ? align_up(size_t(cap) * tape_depth * 16) : 0;
+  char* b = static_cast<char*>(mem);
   auto take = [&](size_t n) {
     char* r = b;
     b += n;
@@ -1034,6 +1040,14 @@ int alloc_wave(uint32_t cap, uint32_t tape_depth, hipStream_t st, DevMem* mem, W
   wb->tape_b = replay ? reinterpret_cast<float4*>(take(tape)) : nullptr;
   wb->tape_c = replay ? reinterpret_cast<float4*>(take(tape)) : nullptr;
   wb->tape_depth = tape_depth;
+}
+
+// One lane's wave buffers from the stream-ordered pool.
+int alloc_wave(uint32_t cap, uint32_t tape_depth, hipStream_t st, DevMem* mem, WaveBufs* wb) {
+  hipError_t const e = pool_alloc(&mem->p, wave_bytes(cap, tape_depth), st);
+  if (e != hipSuccess) return hip_fail(e, "pool_alloc(wave)");
+  mem->st = st;
+  carve_wave(mem->p, cap, tape_depth, wb);
   return XYZ_OK;
 }
 
@@ -1148,20 +1162,24 @@ struct JobSpec {
   uint32_t max_depth;
 };
 
-// Renders jobs with the wavefront kernels.  The jobs' work items are laid end
-// to end and cut into waves of up to kWaveCap path slots and kMaxWaveJobs jobs;
-// consecutive waves alternate between internal streams, so that the tail of
-// one wave (a few long paths) overlaps the full bounces of the next.  Each
-// job's film receives its waves in sample order (an event per wave).
-// backward: the adjoint of the single job (ga).  queries: total closest-hit
-// queries (may be NULL; then nothing waits on the device).
-int render_jobs(std::vector<JobSpec> js, hipStream_t st, bool backward, GradArgs ga,
-                unsigned long long* queries, bool guided = false) {
+// The waves of a render: the jobs' work items laid end to end and cut into
+// waves of up to `cap` path slots and kMaxWaveJobs jobs.
+struct RenderPlan {
   std::vector<JobSpec> jobs;
-  for (JobSpec& x : js)
-    if (x.items > 0) jobs.push_back(x);
-  if (queries) *queries = 0;
-  if (jobs.empty()) return XYZ_OK;
+  std::vector<WaveJobs> waves;
+  std::vector<uint32_t> wdepth, wstack, wmaxwh;
+  uint32_t cap = 0, tape_depth = 0;
+  int lanes = 1;
+  bool backward = false;
+};
+
+void plan_render(std::vector<JobSpec> const& js, bool backward, RenderPlan* p) {
+  p->jobs.clear();
+  for (JobSpec const& x : js)
+    if (x.items > 0) p->jobs.push_back(x);
+  p->backward = backward;
+  std::vector<JobSpec>& jobs = p->jobs;
+  if (jobs.empty()) return;
   unsigned long long total = 0;
   for (JobSpec& x : jobs) {
     x.j.base = total;
@@ -1169,78 +1187,75 @@ int render_jobs(std::vector<JobSpec> js, hipStream_t st, bool backward, GradArgs
   }
   // the adjoint's vertex tape: 48 bytes per slot and shading vertex, at most
   // kTapeBudget bytes per lane
-  uint32_t tape_depth = 0;
   unsigned long long slot_cap = kWaveCap;
   if (backward) {
-    for (JobSpec const& x : jobs) tape_depth = std::max(tape_depth, x.max_depth);
+    for (JobSpec const& x : jobs) p->tape_depth = std::max(p->tape_depth, x.max_depth);
     slot_cap = std::max<unsigned long long>(
-        kBlock, std::min<unsigned long long>(kWaveCap, kTapeBudget / (48ull * tape_depth)));
+        kBlock, std::min<unsigned long long>(kWaveCap, kTapeBudget / (48ull * p->tape_depth)));
   }
   uint32_t const cap = uint32_t(std::min<unsigned long long>(total, slot_cap));
-  // wave plan
-  std::vector<WaveJobs> waves;
-  std::vector<uint32_t> wdepth, wstack, wmaxwh;
-  {
-    unsigned long long q = 0;
-    size_t j = 0;
-    while (q < total) {
-      WaveJobs w;
-      std::memset(&w, 0, sizeof(w));
-      w.q0 = q;
-      w.jlo = uint32_t(j);
-      uint32_t slots = 0, depth = 0, stack = 1, maxwh = 0;
-      while (slots < cap && j < jobs.size() && w.nj < kMaxWaveJobs) {
-        unsigned long long const end = jobs[j].j.base + jobs[j].items;
-        uint32_t const take = uint32_t(std::min<unsigned long long>(cap - slots, end - q));
-        w.start[w.nj++] = slots;
-        slots += take;
-        q += take;
-        depth = std::max(depth, jobs[j].max_depth);
-        stack = std::max(stack, jobs[j].j.s.stack_cap);
-        maxwh = std::max(maxwh, jobs[j].j.wh);
-        if (q == end) ++j; else break;
-      }
-      w.start[w.nj] = slots;
-      waves.push_back(w);
-      wdepth.push_back(depth);
-      wstack.push_back(stack);
-      wmaxwh.push_back(maxwh);
+  p->cap = cap;
+  unsigned long long q = 0;
+  size_t j = 0;
+  while (q < total) {
+    WaveJobs w;
+    std::memset(&w, 0, sizeof(w));
+    w.q0 = q;
+    w.jlo = uint32_t(j);
+    uint32_t slots = 0, depth = 0, stack = 1, maxwh = 0;
+    while (slots < cap && j < jobs.size() && w.nj < kMaxWaveJobs) {
+      unsigned long long const end = jobs[j].j.base + jobs[j].items;
+      uint32_t const take = uint32_t(std::min<unsigned long long>(cap - slots, end - q));
+      w.start[w.nj++] = slots;
+      slots += take;
+      q += take;
+      depth = std::max(depth, jobs[j].max_depth);
+      stack = std::max(stack, jobs[j].j.s.stack_cap);
+      maxwh = std::max(maxwh, jobs[j].j.wh);
+      if (q == end) ++j; else break;
     }
+    w.start[w.nj] = slots;
+    p->waves.push_back(w);
+    p->wdepth.push_back(depth);
+    p->wstack.push_back(stack);
+    p->wmaxwh.push_back(maxwh);
   }
-  // device tables, written by kernels from their arguments (stream-ordered)
-  DevMem tables;
-  size_t const jb = align_up(jobs.size() * sizeof(RJob));
-  hipError_t e = pool_alloc(&tables.p, jb + waves.size() * sizeof(WaveJobs), st);
-  if (e != hipSuccess) return hip_fail(e, "pool_alloc(render tables)");
-  tables.st = st;
-  RJob* const jobs_dev = static_cast<RJob*>(tables.p);
-  WaveJobs* const waves_dev = reinterpret_cast<WaveJobs*>(static_cast<char*>(tables.p) + jb);
-  for (size_t i = 0; i < jobs.size(); ++i)
-    hipLaunchKernelGGL(k_put_job, dim3(1), dim3(1), 0, st, jobs[i].j, jobs_dev + i);
-  for (size_t k = 0; k < waves.size(); ++k)
-    hipLaunchKernelGGL(k_put_wave, dim3(1), dim3(1), 0, st, waves[k], waves_dev + k);
-  int const lanes = waves.size() > 1 ? kLanes : 1;
+  p->lanes = p->waves.size() > 1 ? kLanes : 1;
+}
+
+// Bytes of a render's device tables (jobs, then waves).
+size_t tables_bytes(const RenderPlan& p) {
+  return align_up(p.jobs.size() * sizeof(RJob)) + p.waves.size() * sizeof(WaveJobs);
+}
+
+// Issues a planned render on `st` (and the internal lane streams, joined back
+// into `st` before the return): tables at `tables`, lane i's wave buffers wb[i].
+int enqueue_render(const RenderPlan& p, void* tables, WaveBufs* wb, hipStream_t st, GradArgs ga,
+                   bool guided, EventSet* ext_ev = nullptr) {
+  RJob* const jobs_dev = static_cast<RJob*>(tables);
+  WaveJobs* const waves_dev =
+      reinterpret_cast<WaveJobs*>(static_cast<char*>(tables) + align_up(p.jobs.size() * sizeof(RJob)));
+  for (size_t i = 0; i < p.jobs.size(); ++i)
+    hipLaunchKernelGGL(k_put_job, dim3(1), dim3(1), 0, st, p.jobs[i].j, jobs_dev + i);
+  for (size_t k = 0; k < p.waves.size(); ++k)
+    hipLaunchKernelGGL(k_put_wave, dim3(1), dim3(1), 0, st, p.waves[k], waves_dev + k);
+  int const lanes = p.lanes;
+  for (int i = 0; i < lanes; ++i)
+    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, wb[i].ctr);
   hipStream_t ls[kLanes];
   for (int i = 0; i < kLanes; ++i) ls[i] = st;
   int rc = XYZ_OK;
+  hipError_t e = hipSuccess;
+  // events of the fork, the per-wave film order and the join; a caller that
+  // captures the render into a graph keeps them until the capture has ended
+  EventSet own_ev;
+  EventSet& ev = ext_ev ? *ext_ev : own_ev;
+  // once the lanes are forked, every return joins them back into `st`
+  LaneJoin join;
   if (lanes > 1) {
     rc = lane_streams(ls);
     if (rc != XYZ_OK) return rc;
-  }
-  DevMem mem[kLanes];
-  WaveBufs wb[kLanes];
-  for (int i = 0; i < lanes; ++i) {
-    rc = alloc_wave(cap, tape_depth, st, &mem[i], &wb[i]);
-    if (rc != XYZ_OK) return rc;
-    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, wb[i].ctr);
-  }
-  EventSet ev;
-  // Once the lanes are forked, every return makes the caller's stream wait
-  // for them before the wave buffers and tables above are released (their
-  // destructors free on the caller's stream).
-  LaneJoin join;
-  if (lanes > 1) {
-    rc = ev.create(2 * kLanes + 1);
+    if (ev.e.size() < size_t(2 * kLanes + 1)) rc = ev.create(2 * kLanes + 1);
     if (rc != XYZ_OK) return rc;
     hipEvent_t const fork = ev.e[2 * kLanes];
     e = hipEventRecord(fork, st);
@@ -1250,53 +1265,84 @@ int render_jobs(std::vector<JobSpec> js, hipStream_t st, bool backward, GradArgs
     if (e != hipSuccess) return hip_fail(e, "render fork");
   }
   GradArgs none{nullptr, nullptr, nullptr};
-  for (size_t k = 0; k < waves.size(); ++k) {
+  for (size_t k = 0; k < p.waves.size(); ++k) {
     int const i = int(k % size_t(lanes));
     hipStream_t const s = ls[i];
     WaveJobs const* const wj = waves_dev + k;
-    uint32_t const n = waves[k].start[waves[k].nj];
+    uint32_t const n = p.waves[k].start[p.waves[k].nj];
     uint32_t const grid = (n + kBlock - 1) / kBlock;
     hipLaunchKernelGGL(k_init_wave, dim3(1), dim3(512), 0, s, wj, wb[i].ctr);
     hipLaunchKernelGGL(k_camera, dim3(grid), dim3(kBlock), 0, s, jobs_dev, wj, wb[i], n);
-    if (backward) {
+    if (p.backward) {
       // the paths are traced again from their seeds, each slot recording its
       // vertices, then swept in reverse
-      rc = run_bounces<true>(jobs_dev, wj, wb[i], n, wdepth[k], wstack[k], ga, s);
+      rc = run_bounces<true>(jobs_dev, wj, wb[i], n, p.wdepth[k], p.wstack[k], ga, s);
       if (rc != XYZ_OK) return rc;
       hipLaunchKernelGGL(k_adjoint_sweep, dim3(grid), dim3(kBlock), 0, s, jobs_dev, wj, wb[i], ga, n);
       continue;
     }
-    rc = run_bounces<false>(jobs_dev, wj, wb[i], n, wdepth[k], wstack[k], none, s, guided);
+    rc = run_bounces<false>(jobs_dev, wj, wb[i], n, p.wdepth[k], p.wstack[k], none, s, guided);
     if (rc != XYZ_OK) return rc;
-    {
-      // wave k is added to the films after wave k - 1 (sample order per pixel)
-      if (lanes > 1 && k > 0) {
-        e = hipStreamWaitEvent(s, ev.e[(k - 1) % size_t(lanes)], 0);
-        if (e != hipSuccess) return hip_fail(e, "render wait");
-      }
-      hipLaunchKernelGGL(k_accumulate, dim3((wmaxwh[k] + kBlock - 1) / kBlock, waves[k].nj),
-                         dim3(kBlock), 0, s, jobs_dev, wj, wb[i].col);
-      if (lanes > 1) {
-        e = hipEventRecord(ev.e[i], s);
-        if (e != hipSuccess) return hip_fail(e, "render record");
-      }
+    // wave k is added to the films after wave k - 1 (sample order per pixel)
+    if (lanes > 1 && k > 0) {
+      e = hipStreamWaitEvent(s, ev.e[(k - 1) % size_t(lanes)], 0);
+      if (e != hipSuccess) return hip_fail(e, "render wait");
+    }
+    hipLaunchKernelGGL(k_accumulate, dim3((p.wmaxwh[k] + kBlock - 1) / kBlock, p.waves[k].nj),
+                       dim3(kBlock), 0, s, jobs_dev, wj, wb[i].col);
+    if (lanes > 1) {
+      e = hipEventRecord(ev.e[i], s);
+      if (e != hipSuccess) return hip_fail(e, "render record");
     }
   }
   if (lanes > 1) {  // join: the caller's stream waits for every lane
     e = join.join();
     if (e != hipSuccess) return hip_fail(e, "render join");
   }
-  if (queries) {
-    unsigned long long q[kLanes] = {};
-    e = hipSuccess;
-    for (int i = 0; i < lanes && e == hipSuccess; ++i)
-      e = hipMemcpyAsync(&q[i], &wb[i].ctr->queries, sizeof(q[i]), hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e != hipSuccess) return hip_fail(e, "render stats");
-    for (int i = 0; i < lanes; ++i) *queries += q[i];
-  }
   e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "render");
+  return XYZ_OK;
+}
+
+// Closest-hit queries counted by a finished render's lanes (synchronises `st`).
+int render_queries(const WaveBufs* wb, int lanes, hipStream_t st, unsigned long long* queries) {
+  unsigned long long q[kLanes] = {};
+  hipError_t e = hipSuccess;
+  for (int i = 0; i < lanes && e == hipSuccess; ++i)
+    e = hipMemcpyAsync(&q[i], &wb[i].ctr->queries, sizeof(q[i]), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return hip_fail(e, "render stats");
+  *queries = 0;
+  for (int i = 0; i < lanes; ++i) *queries += q[i];
+  return XYZ_OK;
+}
+
+// Renders jobs with the wavefront kernels (see RenderPlan); consecutive waves
+// alternate between internal streams, so that the tail of one wave (a few
+// long paths) overlaps the full bounces of the next.  Each job's film
+// receives its waves in sample order (an event per wave).  Workspace comes
+// from the stream-ordered pool.  backward: the adjoint of the single job (ga).
+// queries: total closest-hit queries (may be NULL; then nothing waits on the
+// device).
+int render_jobs(std::vector<JobSpec> js, hipStream_t st, bool backward, GradArgs ga,
+                unsigned long long* queries, bool guided = false) {
+  if (queries) *queries = 0;
+  RenderPlan p;
+  plan_render(js, backward, &p);
+  if (p.jobs.empty()) return XYZ_OK;
+  DevMem tables;
+  hipError_t e = pool_alloc(&tables.p, tables_bytes(p), st);
+  if (e != hipSuccess) return hip_fail(e, "pool_alloc(render tables)");
+  tables.st = st;
+  DevMem mem[kLanes];
+  WaveBufs wb[kLanes];
+  for (int i = 0; i < p.lanes; ++i) {
+    int const rc = alloc_wave(p.cap, p.tape_depth, st, &mem[i], &wb[i]);
+    if (rc != XYZ_OK) return rc;
+  }
+  int rc = enqueue_render(p, tables.p, wb, st, ga, guided);
+  if (rc != XYZ_OK) return rc;
+  if (queries) return render_queries(wb, p.lanes, st, queries);
   return XYZ_OK;
 }
 
@@ -1575,16 +1621,74 @@ xyz_batch_tuning default_tuning() {
   return t;
 }
 
+// A captured batch and the device memory its nodes address.  Replays of one
+// entry share that memory, so each replay waits for the previous one (`done`)
+// on whatever stream it is issued.
 struct GraphEntry {
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
+  hipEvent_t done = nullptr;
+  // engine 0: the job table and the work counter
   void* jobs_dev = nullptr;
   uint32_t* work = nullptr;
+  // engine 1: the render tables and each lane's wave buffers
+  void* tables = nullptr;
+  void* wave_mem[kLanes] = {};
+  WaveBufs wb[kLanes];
+  int lanes = 0;
+  uint64_t last_use = 0;
 };
 
 std::mutex g_graph_mu;
 std::map<std::string, GraphEntry> g_graphs;
+uint64_t g_graph_clock = 0;
 hipStream_t g_capture_stream = nullptr;
+// Cached graphs at most (a shared-wave entry holds its wave buffers, up to
+// ~17 GB for 2^25-slot waves on two lanes).
+constexpr size_t kMaxCachedGraphs = 4;
+
+void free_graph_entry(GraphEntry& ge) {
+  if (ge.done) (void)hipEventSynchronize(ge.done);  // its last replay has finished
+  if (ge.exec) (void)hipGraphExecDestroy(ge.exec);
+  if (ge.graph) (void)hipGraphDestroy(ge.graph);
+  if (ge.done) (void)hipEventDestroy(ge.done);
+  if (ge.jobs_dev) (void)hipFree(ge.jobs_dev);
+  if (ge.work) (void)hipFree(ge.work);
+  if (ge.tables) (void)hipFree(ge.tables);
+  for (void* m : ge.wave_mem)
+    if (m) (void)hipFree(m);
+  ge = GraphEntry();
+}
+
+// Makes room for one more entry (call with g_graph_mu held): the least recently
+// used entries go, each after its last replay.
+void evict_graphs() {
+  while (g_graphs.size() >= kMaxCachedGraphs) {
+    auto victim = g_graphs.begin();
+    for (auto it = g_graphs.begin(); it != g_graphs.end(); ++it)
+      if (it->second.last_use < victim->second.last_use) victim = it;
+    free_graph_entry(victim->second);
+    g_graphs.erase(victim);
+  }
+}
+
+// Replays a cached entry on `st` after its previous replay.
+int replay_graph(GraphEntry& ge, hipStream_t st) {
+  hipError_t e = hipStreamWaitEvent(st, ge.done, 0);
+  if (e == hipSuccess) e = hipGraphLaunch(ge.exec, st);
+  if (e == hipSuccess) e = hipEventRecord(ge.done, st);
+  ge.last_use = ++g_graph_clock;
+  return e == hipSuccess ? XYZ_OK : hip_fail(e, "graph replay");
+}
+
+int capture_stream(hipStream_t* out) {
+  if (!g_capture_stream) {
+    hipError_t const e = hipStreamCreateWithFlags(&g_capture_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
+  }
+  *out = g_capture_stream;
+  return XYZ_OK;
+}
 
 int launch_batch_raw(const BatchPlan& plan, const xyz_batch_tuning& t, void* jobs_dev,
                      uint32_t* work, hipStream_t st) {
@@ -1617,7 +1721,7 @@ int check_tuning(const xyz_batch_tuning& t) {
 // Engine 1: all jobs through the wavefront kernels in shared waves.
 int render_batch_wavefront(uint32_t n, xyz_scene* const* scenes, const xyz_camera* cams,
                            const xyz_render_params* prm, float* const* films,
-                           xyz_render_stats* stats, hipStream_t st) {
+                           xyz_render_stats* stats, hipStream_t st, bool use_graph) {
   std::vector<JobSpec> js(n);
   unsigned long long paths = 0;
   for (uint32_t i = 0; i < n; ++i) {
@@ -1651,8 +1755,63 @@ int render_batch_wavefront(uint32_t n, xyz_scene* const* scenes, const xyz_camer
     (void)hipEventRecord(e0, st);
   }
   unsigned long long q = 0;
-  int const rc = render_jobs(std::move(js), st, false, GradArgs{nullptr, nullptr, nullptr},
-                             stats ? &q : nullptr);
+  int rc = XYZ_OK;
+  if (use_graph) {
+    // one capture per job table: the render's tables and wave buffers are
+    // allocated before the capture and stay with the cached graph; a repeated
+    // batch is one hipGraphLaunch
+    std::string key("engine1:");
+    for (JobSpec const& x : js) {
+      key.append(reinterpret_cast<const char*>(&x.j), sizeof(x.j));
+      key.append(reinterpret_cast<const char*>(&x.items), sizeof(x.items));
+      key.append(reinterpret_cast<const char*>(&x.max_depth), sizeof(x.max_depth));
+    }
+    std::lock_guard<std::mutex> lk(g_graph_mu);
+    auto it = g_graphs.find(key);
+    if (it == g_graphs.end()) {
+      evict_graphs();
+      RenderPlan p;
+      plan_render(js, false, &p);
+      GraphEntry ge;
+      ge.lanes = p.lanes;
+      hipStream_t cs = nullptr;
+      rc = capture_stream(&cs);
+      if (rc != XYZ_OK) return rc;
+      hipError_t e = hipMalloc(&ge.tables, std::max<size_t>(tables_bytes(p), 256));
+      for (int i = 0; i < p.lanes && e == hipSuccess; ++i) {
+        e = hipMalloc(&ge.wave_mem[i], wave_bytes(p.cap, 0));
+        if (e == hipSuccess) carve_wave(ge.wave_mem[i], p.cap, 0, &ge.wb[i]);
+      }
+      if (e == hipSuccess) e = hipEventCreateWithFlags(&ge.done, hipEventDisableTiming);
+      if (e == hipSuccess) e = hipEventRecord(ge.done, cs);
+      // the work of earlier calls on the caller's stream is ordered by the
+      // replay; the capture stream itself has nothing in flight here
+      if (e == hipSuccess) e = hipStreamSynchronize(cs);
+      if (e != hipSuccess) {
+        free_graph_entry(ge);
+        return hip_fail(e, "render_batch graph set-up");
+      }
+      {
+        EventSet ev;  // kept until the capture has ended
+        e = hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed);
+        if (e == hipSuccess)
+          rc = enqueue_render(p, ge.tables, ge.wb, cs, GradArgs{nullptr, nullptr, nullptr}, false, &ev);
+        hipError_t const ee = e == hipSuccess ? hipStreamEndCapture(cs, &ge.graph) : e;
+        if (e == hipSuccess) e = ee;
+      }
+      if (rc == XYZ_OK && e == hipSuccess) e = hipGraphInstantiate(&ge.exec, ge.graph, nullptr, nullptr, 0);
+      if (rc != XYZ_OK || e != hipSuccess) {
+        free_graph_entry(ge);
+        return rc != XYZ_OK ? rc : hip_fail(e, "render_batch capture");
+      }
+      it = g_graphs.emplace(key, ge).first;
+    }
+    rc = replay_graph(it->second, st);
+    if (rc == XYZ_OK && stats) rc = render_queries(it->second.wb, it->second.lanes, st, &q);
+  } else {
+    rc = render_jobs(std::move(js), st, false, GradArgs{nullptr, nullptr, nullptr},
+                     stats ? &q : nullptr);
+  }
   if (rc != XYZ_OK) return rc;
   if (stats) {
     (void)hipEventRecord(e1, st);
@@ -1679,7 +1838,8 @@ int render_batch_impl(uint32_t n, xyz_scene* const* scenes, const xyz_camera* ca
   xyz_batch_tuning const t = tuning ? *tuning : default_tuning();
   int rc = check_tuning(t);
   if (rc != XYZ_OK) return rc;
-  if (t.engine == 1) return render_batch_wavefront(n, scenes, cams, prm, films, stats, st);
+  if (t.engine == 1)
+    return render_batch_wavefront(n, scenes, cams, prm, films, stats, st, t.use_graph != 0);
   BatchPlan plan;
   rc = plan_batch(n, scenes, cams, prm, films, t.tile_pixels, &plan);
   if (rc != XYZ_OK) return rc;
@@ -1696,41 +1856,38 @@ int render_batch_impl(uint32_t n, xyz_scene* const* scenes, const xyz_camera* ca
     // key: the job table bytes and the tuning; the graph replays one launch
     std::string key(reinterpret_cast<const char*>(plan.jobs.data()), plan.jobs.size() * sizeof(BatchJob));
     key.append(reinterpret_cast<const char*>(&t), sizeof(t));
+    key.insert(0, "engine0:");
     std::lock_guard<std::mutex> lk(g_graph_mu);
     auto it = g_graphs.find(key);
     if (it == g_graphs.end()) {
+      evict_graphs();
       GraphEntry ge;
       size_t const jb = plan.jobs.size() * sizeof(BatchJob);
+      hipStream_t cs = nullptr;
+      rc = capture_stream(&cs);
+      if (rc != XYZ_OK) return rc;
       hipError_t e = hipMalloc(&ge.jobs_dev, jb);
       if (e == hipSuccess) e = hipMemcpy(ge.jobs_dev, plan.jobs.data(), jb, hipMemcpyHostToDevice);
       if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&ge.work), sizeof(uint32_t));
-      if (e != hipSuccess) return hip_fail(e, "render_batch graph buffers");
-      if (!g_capture_stream) {
-        e = hipStreamCreate(&g_capture_stream);
-        if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
+      if (e == hipSuccess) e = hipEventCreateWithFlags(&ge.done, hipEventDisableTiming);
+      if (e == hipSuccess) e = hipEventRecord(ge.done, cs);
+      if (e == hipSuccess) e = hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed);
+      if (e != hipSuccess) {
+        free_graph_entry(ge);
+        return hip_fail(e, "render_batch graph set-up");
       }
-      e = hipStreamBeginCapture(g_capture_stream, hipStreamCaptureModeRelaxed);
-      if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
-      rc = launch_batch_raw(plan, t, ge.jobs_dev, ge.work, g_capture_stream);
-      e = hipStreamEndCapture(g_capture_stream, &ge.graph);
-      if (rc != XYZ_OK) return rc;
-      if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
-      e = hipGraphInstantiate(&ge.exec, ge.graph, nullptr, nullptr, 0);
-      if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
-      if (g_graphs.size() > 64) {
-        for (auto& kv : g_graphs) {
-          (void)hipGraphExecDestroy(kv.second.exec);
-          (void)hipGraphDestroy(kv.second.graph);
-          (void)hipFree(kv.second.jobs_dev);
-          (void)hipFree(kv.second.work);
-        }
-        g_graphs.clear();
+      rc = launch_batch_raw(plan, t, ge.jobs_dev, ge.work, cs);
+      e = hipStreamEndCapture(cs, &ge.graph);
+      if (rc == XYZ_OK && e == hipSuccess) e = hipGraphInstantiate(&ge.exec, ge.graph, nullptr, nullptr, 0);
+      if (rc != XYZ_OK || e != hipSuccess) {
+        free_graph_entry(ge);
+        return rc != XYZ_OK ? rc : hip_fail(e, "render_batch capture");
       }
       it = g_graphs.emplace(key, ge).first;
     }
     if (stats) (void)hipEventRecord(e0, st);
-    hipError_t const e = hipGraphLaunch(it->second.exec, st);
-    if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+    rc = replay_graph(it->second, st);
+    if (rc != XYZ_OK) return rc;
   } else {
     void* jobs_dev = nullptr;
     uint32_t* work = nullptr;
