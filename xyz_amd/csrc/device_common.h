@@ -45,7 +45,7 @@ static_assert(kClustersPerStep >= 1 && kClustersPerStep <= 16, "survivor list ho
 // Interior nodes a lane may visit per cooperative step (0: no limit); a lane
 // that has not reached a leaf continues in the next step.
 #ifndef XYZ_DESCENT_BUDGET
-#define XYZ_DESCENT_BUDGET 0
+#define XYZ_DESCENT_BUDGET 8
 #endif
 
 // Everything a kernel needs to know about one scene; passed by value.
@@ -120,6 +120,14 @@ __device__ __forceinline__ float split_distance(uint2 w, float3 o, float3 d, uin
   *nearc = i ? left : right;
   *farc = i ? right : left;
   return (split - oa) / da;
+}
+
+// The far child of interior node `node` (word w) for a ray from o: the child
+// split_distance() returns as the far one.
+__device__ __forceinline__ uint32_t far_child(uint2 w, float3 o, uint32_t node) {
+  uint32_t const axis = w.x & 3u;
+  float const split = __uint_as_float(w.y);
+  return (sel3(o, axis) < split) ? (w.x >> 2) : node + 1u;
 }
 
 // Closest-hit query, the reference's recursive traversal (xyz/geom/kdtree.cpp:82-150,
@@ -403,17 +411,22 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Exclusive prefix sum of c over the wave and its total.
+// Exclusive prefix sum of c over the wave and its total (all 64 lanes call
+// it).  Inclusive scan by data-parallel primitives: within each row of 16
+// lanes, the values 1, 2, 4 and 8 lanes below are added (a lane with no such
+// lane in its row adds 0); then row_bcast:15 adds row r's last lane to row
+// r + 1 (rows 1 and 3), and row_bcast:31 adds lane 31 to rows 2 and 3.
 __device__ __forceinline__ void wave_prefix(uint32_t c, uint32_t* excl, uint32_t* total) {
-  uint32_t const lane = __lane_id();
-  uint32_t incl = c;
-#pragma unroll
-  for (uint32_t off = 1; off < 64; off <<= 1) {
-    uint32_t const v = __shfl_up(incl, off);
-    if (lane >= off) incl += v;
-  }
+  int v = int(c);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  uint32_t const incl = uint32_t(v);
   *excl = incl - c;
-  *total = __shfl(incl, 63);
+  *total = uint32_t(__builtin_amdgcn_readlane(v, 63));
 }
 
 // The lane that owns item base + lane of a wave_prefix distribution (excl, c),
@@ -471,7 +484,9 @@ __device__ __forceinline__ float interval_t_max(const DevScene& s, const TraceSt
 // Returns true when this lane's (active) query finished in this step.
 // `keys` is this wave's 64-entry LDS array (index = lane), `own` its table for
 // batch_owner (64 entries kOwnStride bytes apart).
-template <bool kCount, uint32_t kOwnStride = 8>
+// kFG: the caller needs f, g and the side of the hit (otherwise only the
+// primitive and t are filled in).
+template <bool kCount, bool kFG = true, uint32_t kOwnStride = 8>
 __device__ __forceinline__ bool coop_step(const DevScene& s, TraceState& st, bool active,
                                           uint32_t* stk, uint32_t stride,
                                           unsigned long long* keys, uint8_t* own) {
@@ -626,13 +641,15 @@ __device__ __forceinline__ bool coop_step(const DevScene& s, TraceState& st, boo
         if (st.lj == st.lend) {
           uint32_t const rank = 0xFFFFFFFFu - uint32_t(key & 0xFFFFFFFFull);
           uint32_t const j = s.rank_pos[(size_t(st.lc0) << kClusterShift) + rank];
-          float t, f, g, det;
-          (void)tri_test(st.o, st.d, s.triA[j], s.triB[j], s.triC[j], st.t_min, st.t_max, &t, &f,
-                         &g, &det);
           st.hit = int32_t(j);
-          st.f = f;
-          st.g = g;
-          st.back = det > 0.0f;
+          if constexpr (kFG) {
+            float t, f, g, det;
+            (void)tri_test(st.o, st.d, s.triA[j], s.triB[j], s.triC[j], st.t_min, st.t_max, &t,
+                           &f, &g, &det);
+            st.f = f;
+            st.g = g;
+            st.back = det > 0.0f;
+          }
         }
       }
     }
@@ -644,12 +661,13 @@ __device__ __forceinline__ bool coop_step(const DevScene& s, TraceState& st, boo
   st.in_leaf = false;
   if (st.hit >= 0) return true;
   if (st.sp == 0) return true;
+  // pop: the far child of the top entry, with t_min = that entry's split
+  // distance, which t_max holds (no hit changed it: a hit ends the query), and
+  // t_max = the split distance of the entry below
   st.sp--;
   uint32_t const p = stk[st.sp * stride];
-  uint32_t farc, nearc;
-  float const e = split_distance(s.nodes[p], st.o, st.d, p, &farc, &nearc);
-  st.node = farc;
-  st.t_min = e;
+  st.node = far_child(s.nodes[p], st.o, p);
+  st.t_min = st.t_max;
   st.t_max = interval_t_max(s, st, stk, stride);
   return false;
 }

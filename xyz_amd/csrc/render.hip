@@ -733,7 +733,7 @@ k_shadow(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBu
     }
     if (__ballot(rq != 0xFFFFFFFFu) == 0) break;
     bool const act = rq != 0xFFFFFFFFu;
-    bool const fin = coop_step<false>(s, st, act, stk, kBlock, keys + (threadIdx.x & ~63u),
+    bool const fin = coop_step<false, false>(s, st, act, stk, kBlock, keys + (threadIdx.x & ~63u),
                                             reinterpret_cast<uint8_t*>(keys + kBlock + (threadIdx.x & ~63u)));
     if (!act || !fin) continue;
     HitRec const h = trace_result(s, st);
@@ -1512,7 +1512,7 @@ __global__ void k_batch(const BatchJob* __restrict__ jobs, uint32_t n_jobs, uint
       tile_next += want < avail ? want : avail;
     }
     bool const act = phase != kIdle;
-    bool const fin = coop_step<false, 1>(jb.s, st, act, stk, blockDim.x, keys, owners);
+    bool const fin = coop_step<false, true, 1>(jb.s, st, act, stk, blockDim.x, keys, owners);
     if (!(act && fin)) continue;
     HitRec const h = trace_result(jb.s, st);
     bool path_done = false;

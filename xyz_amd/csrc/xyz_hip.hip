@@ -64,8 +64,10 @@ hipError_t pool_alloc(void** p, size_t bytes, hipStream_t stream) {
 
 // ------------------------------------------------------------ closest hit
 constexpr uint32_t kHitBlock = 256;
+// Persistent workgroups per CU: 7 is what fits at once (67 VGPRs give 7 waves
+// per SIMD; the LDS stacks of 7 workgroups take 150 of the CU's 160 KiB).
 #ifndef XYZ_HIT_BLOCKS_PER_CU
-#define XYZ_HIT_BLOCKS_PER_CU 8
+#define XYZ_HIT_BLOCKS_PER_CU 7
 #endif
 constexpr uint32_t kHitBlocksPerCu = XYZ_HIT_BLOCKS_PER_CU;
 
@@ -79,7 +81,7 @@ constexpr uint32_t kHitBlocksPerCu = XYZ_HIT_BLOCKS_PER_CU;
 #ifndef XYZ_HIT_WAVES_PER_EU
 #define XYZ_HIT_WAVES_PER_EU 0
 #endif
-template <bool kCount>
+template <bool kCount, bool kFG>
 __global__ void __launch_bounds__(kHitBlock)
 #if XYZ_HIT_WAVES_PER_EU
 __attribute__((amdgpu_waves_per_eu(XYZ_HIT_WAVES_PER_EU, XYZ_HIT_WAVES_PER_EU)))
@@ -111,7 +113,7 @@ closest_hit_kernel(DevScene s, uint32_t n, const float* __restrict__ org,
     if (__ballot(r != 0xFFFFFFFFu) == 0) break;
     bool const act = r != 0xFFFFFFFFu;
 #if XYZ_COOP
-    bool const fin = coop_step<kCount>(s, st, act, stk, kHitBlock, keys + (threadIdx.x & ~63u),
+    bool const fin = coop_step<kCount, kFG>(s, st, act, stk, kHitBlock, keys + (threadIdx.x & ~63u),
                                              reinterpret_cast<uint8_t*>(keys + kHitBlock + (threadIdx.x & ~63u)));
 #else
     bool const fin = act && trace_step<kCount>(s, st, stk, kHitBlock);
@@ -147,6 +149,20 @@ closest_hit_kernel(DevScene s, uint32_t n, const float* __restrict__ org,
   }
 }
 
+// The kernel instantiation for what the caller asked: counting (per-ray
+// counters or launch totals) and barycentrics / side (f, g, back).
+void launch_hit_kernel(const DevScene& s, uint32_t grid, size_t lds, hipStream_t stream, uint32_t n,
+                       const float* org, const float* dir, int32_t* prim, float* t, float* f,
+                       float* g, uint8_t* back, uint32_t* counters, unsigned long long* totals,
+                       uint32_t* w) {
+  bool const cnt = counters || totals;
+  bool const fg = f || g || back;
+  auto k = cnt ? (fg ? closest_hit_kernel<true, true> : closest_hit_kernel<true, false>)
+               : (fg ? closest_hit_kernel<false, true> : closest_hit_kernel<false, false>);
+  hipLaunchKernelGGL(k, dim3(grid), dim3(kHitBlock), lds, stream, s, n, org, dir, prim, t, f, g,
+                     back, counters, totals, w);
+}
+
 // Clears the closest hit's work counters (one per part of the rays).
 __global__ void k_zero_u32(uint32_t* p) {
   if (blockIdx.x == 0 && threadIdx.x < WaveFeeder::kFeedParts) p[threadIdx.x] = 0;
@@ -173,13 +189,7 @@ int launch_closest_hit(const DevScene& s, uint32_t n, const float* org, const fl
   if (e != hipSuccess) return hip_fail(e, "pool_alloc(work counters)");
   uint32_t* w = static_cast<uint32_t*>(work);
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, w);
-  if (counters || totals) {
-    hipLaunchKernelGGL(closest_hit_kernel<true>, dim3(grid), dim3(kHitBlock), lds, stream, s, n,
-                       org, dir, prim, t, f, g, back, counters, totals, w);
-  } else {
-    hipLaunchKernelGGL(closest_hit_kernel<false>, dim3(grid), dim3(kHitBlock), lds, stream, s, n,
-                       org, dir, prim, t, f, g, back, counters, totals, w);
-  }
+  launch_hit_kernel(s, grid, lds, stream, n, org, dir, prim, t, f, g, back, counters, totals, w);
   e = hipGetLastError();
   (void)hipFreeAsync(work, stream);
   if (e != hipSuccess) return hip_fail(e, "closest_hit launch");
@@ -244,13 +254,7 @@ int launch_closest_hit_noalloc(const DevScene& s, uint32_t n, const float* org, 
   uint32_t const need_blocks = (n + kHitBlock - 1) / kHitBlock;
   uint32_t const grid = std::min<uint32_t>(need_blocks, uint32_t(g_hit_cu) * kHitBlocksPerCu);
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, w);  // a kernel, not a memset
-  if (counters) {
-    hipLaunchKernelGGL(closest_hit_kernel<true>, dim3(grid), dim3(kHitBlock), lds, stream, s, n,
-                       org, dir, prim, t, f, g, back, counters, nullptr, w);
-  } else {
-    hipLaunchKernelGGL(closest_hit_kernel<false>, dim3(grid), dim3(kHitBlock), lds, stream, s, n,
-                       org, dir, prim, t, f, g, back, counters, nullptr, w);
-  }
+  launch_hit_kernel(s, grid, lds, stream, n, org, dir, prim, t, f, g, back, counters, nullptr, w);
   hipError_t const e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "closest_hit_noalloc launch");
   return XYZ_OK;
