@@ -1123,13 +1123,19 @@ struct LaneJoin {
   }
 };
 
+// Persistent workgroups per CU of the extension and shadow kernels.
+#ifndef XYZ_TRACE_BLOCKS_PER_CU
+#define XYZ_TRACE_BLOCKS_PER_CU 8
+#endif
+
 // Runs the bounce loop of one wave.  kReplay: adjoint replay pass.
 template <bool kReplay>
 int run_bounces(const RJob* jobs, const WaveJobs* wj, WaveBufs wb, uint32_t n, uint32_t depth,
                 uint32_t stack_cap, GradArgs ga, hipStream_t st, bool guided = false) {
   int const ncu = num_cu();
   size_t const lds = size_t(stack_cap) * kBlock * sizeof(uint32_t);
-  uint32_t const trace_grid = std::max(1u, std::min((n + kBlock - 1) / kBlock, uint32_t(ncu) * 8));
+  uint32_t const trace_grid =
+      std::max(1u, std::min((n + kBlock - 1) / kBlock, uint32_t(ncu) * XYZ_TRACE_BLOCKS_PER_CU));
   uint32_t const shade_grid = std::max(1u, std::min((n + kBlock - 1) / kBlock, uint32_t(ncu) * 8));
   uint32_t* qc = wb.qa;
   uint32_t* qn = wb.qb;
