@@ -578,7 +578,11 @@ def bench_batch(args, hip, capi, scenes):
     engine, replayed from a captured hipGraph; the first call captures, the
     timed call is one graph launch) against the same renders one after another
     through xyz_render.  `ideal_batched_over_sequential` is the ratio if every
-    job ran at the rate of the job with the most samples."""
+    job ran at the rate of the job with the most samples.
+    `bound_smaller_jobs_free` is the ratio if the jobs of at least 2^26 samples
+    kept their sequential time and every other job cost nothing: each of those
+    jobs fills the GPU on its own (batched together they gain 1 %), so no
+    batching of this mix can exceed it."""
     D = 8
     jobs = [scenes.batch_scene(s) for s in range(64)]
     gs = [capi.Scene.from_dict(j) for j in jobs]
@@ -606,6 +610,8 @@ def bench_batch(args, hip, capi, scenes):
     job_samples = [j["width"] * j["height"] * j["spp"] for j in jobs]
     big = int(np.argmax(job_samples))
     big_rate = job_samples[big] / (job_ms[big] * 1e-3)
+    large = [k for k in range(len(gs)) if job_samples[k] >= (1 << 26)]
+    large_ms = sum(job_ms[k] for k in large)
     seq_films = [f.numpy() for f in films]
     # batched: one graph launch (first call captures; the timed call replays)
     for f in films:
@@ -629,6 +635,10 @@ def bench_batch(args, hip, capi, scenes):
                                 msamples_per_s=round(big_rate / 1e6, 1)),
                smallest_job_msamples_per_s=round(
                    min(a / (b * 1e-3) for a, b in zip(job_samples, job_ms)) / 1e6, 1),
+               bound_smaller_jobs_free=round(seq_ms / large_ms, 3),
+               large_jobs=dict(count=len(large),
+                               sample_share=round(sum(job_samples[k] for k in large) / samples, 3),
+                               sequential_ms=round(large_ms, 1)),
                tuning=dict(engine=["persistent megakernel", "shared wavefront"][best.engine],
                            tile_pixels=best.tile_pixels, blocks_per_cu=best.blocks_per_cu,
                            block_threads=best.block_threads, use_graph=best.use_graph),
