@@ -63,16 +63,17 @@ void make_record(float3 v0, float3 v1, float3 v2, float4* A, float4* B, float4* 
   *C = make_float4(c[0], c[1], c[2], nz);
 }
 
-// The margin cluster_may_hit uses (for the report only).
+// The margin cluster_may_hit adds to the stored box (for the report only).
 double margin(float4 L, float4 H, float3 o, float3 d) {
   double const u = 5.9604644775390625e-08;
   double const dl = std::sqrt(double(d.x) * d.x + double(d.y) * d.y + double(d.z) * d.z);
-  double const lam = u * dl * L.w * 1e6;
-  V const c{0.5 * (L.x + H.x), 0.5 * (L.y + H.y), 0.5 * (L.z + H.z)};
-  V const h{0.5 * (H.x - L.x), 0.5 * (H.y - L.y), 0.5 * (H.z - L.z)};
-  V const oc = sub(V{o.x, o.y, o.z}, c);
-  double const R = std::sqrt(dot(oc, oc)) + std::sqrt(dot(h, h));
-  return 2 * lam * (32 * R + 14 * H.w) + 8 * u * (R + H.w);
+  double R2 = 0.0;
+  double const oo[3] = {o.x, o.y, o.z}, lo[3] = {L.x, L.y, L.z}, hi[3] = {H.x, H.y, H.z};
+  for (int k = 0; k < 3; ++k) {
+    double const r = std::max(std::fabs(oo[k] - lo[k]), std::fabs(hi[k] - oo[k]));
+    R2 += r * r;
+  }
+  return std::sqrt(R2) * (dl * L.w + 8 * u) + dl * H.w;
 }
 
 double box_distance(float4 L, float4 H, V p) {
@@ -88,9 +89,10 @@ struct Stats {
 };
 
 bool may_hit(float4 L, float4 H, float3 o, float3 d, float tmin, float tmax) {
-  float const dl = approx_sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
-  float3 const inv = make_float3(approx_rcp(d.x), approx_rcp(d.y), approx_rcp(d.z));
-  return cluster_may_hit(L, H, o, d, dl, inv, tmin, tmax);
+  float dl;
+  float3 inv;
+  cull_ray(d, &dl, &inv);
+  return cluster_may_hit(L, H, o, dl, inv, tmin, tmax);
 }
 
 // Tests every triangle of the cluster against the ray over [tmin, tmax].
@@ -117,8 +119,9 @@ void check(const std::vector<float4>& A, const std::vector<float4>& B, const std
     double const mg = margin(L, H, o, d);
     V const p = add(V{o.x, o.y, o.z}, mul(V{d.x, d.y, d.z}, double(t)));
     double const dist = box_distance(L, H, p);
+    // where culling is on: lam = |d| P / 64 <= 0.01
     if (mg > 0 && std::isfinite(mg) && L.w * std::sqrt(double(d.x) * d.x + double(d.y) * d.y +
-                                                       double(d.z) * d.z) * 5.96e-2 <= 0.01)
+                                                       double(d.z) * d.z) / 64.0 <= 0.01)
       st->worst = std::max(st->worst, dist / mg);
   }
 }

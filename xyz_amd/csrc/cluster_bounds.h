@@ -1,5 +1,5 @@
-// Host computation of a triangle cluster's culling record (box, N, E), shared
-// by the scene builder and the exactness test of cluster_may_hit.
+// Host computation of a triangle cluster's culling record, shared by the scene
+// builder and the exactness test of cluster_may_hit.
 #ifndef XYZ_CLUSTER_BOUNDS_H_
 #define XYZ_CLUSTER_BOUNDS_H_
 
@@ -24,10 +24,19 @@ inline float float_down(double x) {
   return f;
 }
 
-// Records A = (v0, n.x), B = (e1, n.y), C = (e2, n.z) of n triangles ->
-// lo = (box lo, N), hi = (box hi, E): the box of v0, v0 + e1, v0 + e2 (exact
-// sums, rounded outward), N >= max |e1||e2|, E >= max(|e1|, |e2|).  A
-// non-finite input makes N NaN, which disables culling of the cluster.
+// Records A = (v0, n.x), B = (e1, n.y), C = (e2, n.z) of n triangles -> the
+// cluster's culling record lo = (box lo - K, P), hi = (box hi + K, Q).
+//
+// The box is that of v0, v0 + e1, v0 + e2 (exact sums, rounded outward).  With
+// N >= max |e1||e2| and E >= max(|e1|, |e2|) over the triangles, u = 2^-24 and
+// c = u / 1e-6, the record holds the parts of cluster_may_hit's margin that do
+// not depend on the ray:
+//   P >= 64 c N                 (64 lam / |d|)
+//   Q >= 28 c N E               (28 lam E / |d|)
+//   K >= 8 u E + 4 u (largest |box coordinate|) + 1e-30,
+// so that the margin of a ray with |d| = dl and distance bound R is
+// R (dl P + 8 u) + dl Q on top of the box already widened by K.  A non-finite
+// input makes P NaN, which disables culling of the cluster.
 inline void cluster_bounds(const float4* A, const float4* B, const float4* C, uint32_t n,
                            float4* lo_out, float4* hi_out) {
   double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -50,11 +59,16 @@ inline void cluster_bounds(const float4* A, const float4* B, const float4* C, ui
     N = std::max(N, l1 * l2);
     E = std::max(E, std::max(l1, l2));
   }
-  // 1e-12 relative slack over the double arithmetic above
-  float const fN = bad ? NAN : float_up(N * (1.0 + 1e-12));
-  float const fE = bad ? NAN : float_up(E * (1.0 + 1e-12));
-  *lo_out = make_float4(float_down(lo[0]), float_down(lo[1]), float_down(lo[2]), fN);
-  *hi_out = make_float4(float_up(hi[0]), float_up(hi[1]), float_up(hi[2]), fE);
+  double const u = 0x1p-24, c = u / 1e-6;
+  // 1e-12 relative slack over the double arithmetic below
+  double const up = 1.0 + 1e-12;
+  double big = 0.0;
+  for (int k = 0; k < 3; ++k) big = std::max(big, std::max(std::fabs(lo[k]), std::fabs(hi[k])));
+  double const K = (8.0 * u * E + 4.0 * u * big + 1e-30) * up;
+  float const P = bad ? NAN : float_up(64.0 * c * N * up);
+  float const Q = bad ? NAN : float_up(28.0 * c * N * E * up);
+  *lo_out = make_float4(float_down(lo[0] - K), float_down(lo[1] - K), float_down(lo[2] - K), P);
+  *hi_out = make_float4(float_up(hi[0] + K), float_up(hi[1] + K), float_up(hi[2] + K), Q);
 }
 
 }  // namespace xyzhip

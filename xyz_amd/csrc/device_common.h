@@ -53,9 +53,10 @@ static_assert(kClustersPerStep >= 1 && kClustersPerStep <= 16, "survivor list ho
 // HBM layout (all arrays 16-byte aligned):
 //   nodes     uint2[n_nodes]   pre-order kd nodes, 8 bytes each; a leaf is
 //                              (count << 2 | 3, first cluster)
-//   clo/chi   float4[n_cl]     cluster boxes: (lo.xyz, N), (hi.xyz, E) with
-//                              N >= max |e1||e2| and E >= max(|e1|, |e2|)
-//                              over the cluster's triangles
+//   clo/chi   float4[n_cl]     cluster culling records: (lo.xyz, P), (hi.xyz, Q),
+//                              the triangles' box widened by the ray-independent
+//                              part of the culling margin, and the margin's
+//                              coefficients (cluster_bounds.h)
 //   triA/B/C  float4[n_cl*16]  triangle records in cluster order:
 //             A = (v0.xyz, n.x), B = (e1.xyz, n.y), C = (e2.xyz, n.z),
 //             n = cross(e2, e1) exactly as the reference recomputes it per test
@@ -305,20 +306,30 @@ __device__ __forceinline__ bool trace_step(const DevScene& s, TraceState& st, ui
 
 // ---------------------------------------------------------------------------
 // Cluster culling.  cluster_may_hit() returns false only when no triangle of
-// cluster `ci` can be accepted by tri_test with t in [t_min, t_max].
+// the cluster can be accepted by tri_test with t in [t_min, t_max].
 //
 // Error bound (first order in u = 2^-24, every operation of tri_test rounded
-// once): for an accepted triangle the computed point o + t d lies within
+// once): for an accepted triangle the point o + t d (t as computed) lies within
 //   Delta = lam (32 R + 14 E) + 2.3 u R + 5.4 u E
 // of the triangle, where lam = u |d| N / 1e-6 (an accepted |det| is at least
 // 1e-6, and N = |e1||e2| bounds |n| and the rounding of n against
-// cross(e2, e1)), R bounds both |v0 - o| and |t d| (R = |o - box centre| +
-// half diagonal, padded), E = max(|e1|, |e2|), provided lam <= 0.01.  The
-// box is expanded by twice the lam term plus 8 u (R + E) + 4 u (largest box
-// coordinate) + 1e-30, and the slab interval is widened by 8 ulp, which covers
-// the box test's own rounding.  Rays or clusters outside the range where no
+// cross(e2, e1)), R bounds both |v0 - o| and |t d| (R = the largest distance
+// from o to the cluster's box, padded), E = max(|e1|, |e2|), provided
+// lam <= 0.01.  The box is expanded by twice the lam term plus 8 u (R + E) +
+// 4 u (largest box coordinate) + 1e-30; the ray-independent part of that
+// (the last three terms but 8 u R) is built into the stored box, the rest is
+//   D = R (|d| P + 8 u) + |d| Q,   P = 64 u N / 1e-6,  Q = 28 u N E / 1e-6
+// (cluster_bounds.h), and the slab interval is widened by 6e-7 of its ends (at
+// least 5 ulp), which covers the box test's own rounding.  Rays or clusters outside the range where no
 // operation of tri_test can overflow (R < 1e12, 1e-12 < |d| < 1e12) or with
 // lam > 0.01 are never culled.
+//
+// A zero direction component has an infinite reciprocal: the slab of that axis
+// then gives (-inf, +inf) when o lies strictly inside it and an empty interval
+// (culled) otherwise.  Where o lies exactly on a face of the widened box the
+// product 0 * inf is NaN and fminf / fmaxf take the other end, which culls:
+// the ray's points keep that coordinate, which lies D + K beyond the
+// triangles' box, farther than Delta.
 // Scale of the culling margin: 1 in the library; the exactness test builds a
 // copy with 0 to show that it detects a missing margin.
 #ifndef XYZ_CULL_MARGIN
@@ -342,41 +353,41 @@ __host__ __device__ __forceinline__ float approx_rcp(float x) {
 #endif
 }
 
-__host__ __device__ __forceinline__ bool cluster_may_hit(float4 L, float4 H, float3 o, float3 d,
-                                                         float dlen, float3 inv, float t_min,
-                                                         float t_max) {
+// The ray's part of the box test: |d| (within an ulp) and 1/d per axis (within
+// an ulp; a zero component gives an infinity of its sign).  |d| is returned as
+// +inf, which makes cluster_may_hit keep every cluster, for rays outside the
+// range the margin covers: |d| not in (1e-12, 1e12) (NaN included), or a
+// non-zero component whose reciprocal is not below 3e38.
+__host__ __device__ __forceinline__ void cull_ray(float3 d, float* dlen, float3* inv) {
+  float const dl = approx_sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
+  float3 const iv = make_float3(approx_rcp(d.x), approx_rcp(d.y), approx_rcp(d.z));
+  bool const ok = (dl > 1e-12f) && (dl < 1e12f) && (d.x == 0.0f || fabsf(iv.x) < 3e38f) &&
+                  (d.y == 0.0f || fabsf(iv.y) < 3e38f) && (d.z == 0.0f || fabsf(iv.z) < 3e38f);
+  *dlen = ok ? dl : __builtin_inff();
+  *inv = iv;
+}
+
+// L, H: the cluster's record; o, dlen, inv: the ray (cull_ray).
+__host__ __device__ __forceinline__ bool cluster_may_hit(float4 L, float4 H, float3 o, float dlen,
+                                                         float3 inv, float t_min, float t_max) {
   float const u = 5.9604645e-08f;
-  float const lam = 0.059604645f * dlen * L.w;  // u |d| N / 1e-6
-  float const cx = 0.5f * (L.x + H.x), cy = 0.5f * (L.y + H.y), cz = 0.5f * (L.z + H.z);
-  float const hx = 0.5f * (H.x - L.x), hy = 0.5f * (H.y - L.y), hz = 0.5f * (H.z - L.z);
-  float const ox = o.x - cx, oy = o.y - cy, oz = o.z - cz;
-  float const R = (approx_sqrt(ox * ox + oy * oy + oz * oz) + approx_sqrt(hx * hx + hy * hy + hz * hz)) * 1.000002f;
-  if (!(lam <= 0.01f) || !(R < 1e12f) || !(dlen < 1e12f) || !(dlen > 1e-12f)) return true;
-  float const big = fmaxf(fmaxf(fmaxf(fabsf(L.x), fabsf(H.x)), fmaxf(fabsf(L.y), fabsf(H.y))),
-                          fmaxf(fabsf(L.z), fabsf(H.z)));
-  float const D = XYZ_CULL_MARGIN *
-                      (2.0f * lam * (32.0f * R + 14.0f * H.w) + 8.0f * u * (R + H.w) + 4.0f * u * big) +
-                  1e-30f;
+  float const a = dlen * L.w;  // 64 lam
+  // the largest distance from o to a point of the box
+  float const rx = fmaxf(fabsf(o.x - L.x), fabsf(H.x - o.x));
+  float const ry = fmaxf(fabsf(o.y - L.y), fabsf(H.y - o.y));
+  float const rz = fmaxf(fabsf(o.z - L.z), fabsf(H.z - o.z));
+  float const R = approx_sqrt(rx * rx + ry * ry + rz * rz) * 1.000002f;
+  if (!(a <= 0.64f) || !(R < 1e12f)) return true;
+  float const D = XYZ_CULL_MARGIN * (R * (a + 8.0f * u) + dlen * H.w) * 1.00001f;
   if (!(D < 1e30f)) return true;
-  float te = -__builtin_inff(), tx = __builtin_inff();
-  float const lo[3] = {L.x - D, L.y - D, L.z - D};
-  float const hi[3] = {H.x + D, H.y + D, H.z + D};
-  float const oa[3] = {o.x, o.y, o.z};
-  float const da[3] = {d.x, d.y, d.z};
-  float const ia[3] = {inv.x, inv.y, inv.z};
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    if (da[a] == 0.0f) {
-      if ((oa[a] < lo[a]) || (oa[a] > hi[a])) return false;
-    } else {
-      if (!(fabsf(ia[a]) < 3e38f)) return true;
-      float const p = (lo[a] - oa[a]) * ia[a], q = (hi[a] - oa[a]) * ia[a];
-      te = fmaxf(te, fminf(p, q));
-      tx = fminf(tx, fmaxf(p, q));
-    }
-  }
-  te = te - fabsf(te) * 5e-7f;
-  tx = tx + fabsf(tx) * 5e-7f;
+  float const px = ((L.x - D) - o.x) * inv.x, qx = ((H.x + D) - o.x) * inv.x;
+  float const py = ((L.y - D) - o.y) * inv.y, qy = ((H.y + D) - o.y) * inv.y;
+  float const pz = ((L.z - D) - o.z) * inv.z, qz = ((H.z + D) - o.z) * inv.z;
+  float te = fmaxf(fmaxf(fminf(px, qx), fminf(py, qy)), fminf(pz, qz));
+  float tx = fminf(fminf(fmaxf(px, qx), fmaxf(py, qy)), fmaxf(pz, qz));
+  // widen by 6e-7 relative (at least 5 ulp); multiplications keep infinities
+  te = te * ((te > 0.0f) ? 0.9999994f : 1.0000006f);
+  tx = tx * ((tx > 0.0f) ? 1.0000006f : 0.9999994f);
   return !((te > t_max) || (tx < t_min) || (te > tx));
 }
 
@@ -539,6 +550,9 @@ __device__ __forceinline__ bool coop_step(const DevScene& s, TraceState& st, boo
     // LDS word (the first half of the owner's key slot)
     uint32_t bexcl, btotal;
     wave_prefix(ncl, &bexcl, &btotal);
+    float dlen;
+    float3 inv;
+    cull_ray(st.d, &dlen, &inv);
     uint32_t* const masks = reinterpret_cast<uint32_t*>(keys);
     masks[2 * lane] = 0u;
     own[kOwnStride * lane] = 0xFFu;
@@ -547,15 +561,14 @@ __device__ __forceinline__ bool coop_step(const DevScene& s, TraceState& st, boo
       uint32_t const k = base + lane;
       uint32_t const owner = batch_owner<kOwnStride>(own, bexcl, ncl, base);
       float3 const o = make_float3(__shfl(st.o.x, owner), __shfl(st.o.y, owner), __shfl(st.o.z, owner));
-      float3 const d = make_float3(__shfl(st.d.x, owner), __shfl(st.d.y, owner), __shfl(st.d.z, owner));
+      float3 const iv = make_float3(__shfl(inv.x, owner), __shfl(inv.y, owner), __shfl(inv.z, owner));
+      float const dl = __shfl(dlen, owner);
       float const tmin = __shfl(st.t_min, owner);
       float const tmax = __shfl(st.t_max, owner);
       uint32_t const i = k - __shfl(bexcl, owner);
       uint32_t const ci = __shfl(st.lj, owner) + i;
       if (k < btotal) {
-        float const dlen = approx_sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
-        float3 const inv = make_float3(approx_rcp(d.x), approx_rcp(d.y), approx_rcp(d.z));
-        if (cluster_may_hit(s.clo[ci], s.chi[ci], o, d, dlen, inv, tmin, tmax))
+        if (cluster_may_hit(s.clo[ci], s.chi[ci], o, dl, iv, tmin, tmax))
           atomicOr(&masks[2 * owner], 1u << i);
       }
     }
@@ -578,11 +591,12 @@ __device__ __forceinline__ bool coop_step(const DevScene& s, TraceState& st, boo
   }
 #else
   if (ncl > 0) {
-    float const dlen = approx_sqrt(st.d.x * st.d.x + st.d.y * st.d.y + st.d.z * st.d.z);
-    float3 const inv = make_float3(approx_rcp(st.d.x), approx_rcp(st.d.y), approx_rcp(st.d.z));
+    float dlen;
+    float3 inv;
+    cull_ray(st.d, &dlen, &inv);
     for (uint32_t i = 0; i < ncl; ++i) {
       uint32_t const ci = st.lj + i;
-      if (cluster_may_hit(s.clo[ci], s.chi[ci], st.o, st.d, dlen, inv, st.t_min, st.t_max)) {
+      if (cluster_may_hit(s.clo[ci], s.chi[ci], st.o, dlen, inv, st.t_min, st.t_max)) {
         list |= static_cast<unsigned long long>(i) << (4u * nsurv);
         ++nsurv;
         c += (ci + 1 == st.lend) ? st.ln - ((ci - st.lc0) << kClusterShift) : kClusterSize;
