@@ -104,6 +104,14 @@ struct HitRec {
   bool back;
 };
 
+// Element i of one of the scene's record arrays, addressed by a 32-bit byte
+// offset (scene creation keeps every record array below 4 GiB), so that the
+// load takes the array's address as a uniform base plus a per-lane offset.
+template <typename T>
+__device__ __forceinline__ T rec_at(const T* base, uint32_t i) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * uint32_t(sizeof(T)));
+}
+
 __device__ __forceinline__ float sel3(float3 v, uint32_t a) {
   return a == 0 ? v.x : (a == 1 ? v.y : v.z);
 }
@@ -377,9 +385,10 @@ __host__ __device__ __forceinline__ bool cluster_may_hit(float4 L, float4 H, flo
   float const ry = fmaxf(fabsf(o.y - L.y), fabsf(H.y - o.y));
   float const rz = fmaxf(fabsf(o.z - L.z), fabsf(H.z - o.z));
   float const R = approx_sqrt(rx * rx + ry * ry + rz * rz) * 1.000002f;
-  if (!(a <= 0.64f) || !(R < 1e12f)) return true;
   float const D = XYZ_CULL_MARGIN * (R * (a + 8.0f * u) + dlen * H.w) * 1.00001f;
-  if (!(D < 1e30f)) return true;
+  // outside the covered range every cluster is kept (computed without a
+  // branch, so that the record's loads are not split around one)
+  bool const covered = (a <= 0.64f) && (R < 1e12f) && (D < 1e30f);
   float const px = ((L.x - D) - o.x) * inv.x, qx = ((H.x + D) - o.x) * inv.x;
   float const py = ((L.y - D) - o.y) * inv.y, qy = ((H.y + D) - o.y) * inv.y;
   float const pz = ((L.z - D) - o.z) * inv.z, qz = ((H.z + D) - o.z) * inv.z;
@@ -388,7 +397,7 @@ __host__ __device__ __forceinline__ bool cluster_may_hit(float4 L, float4 H, flo
   // widen by 6e-7 relative (at least 5 ulp); multiplications keep infinities
   te = te * ((te > 0.0f) ? 0.9999994f : 1.0000006f);
   tx = tx * ((tx > 0.0f) ? 1.0000006f : 0.9999994f);
-  return !((te > t_max) || (tx < t_min) || (te > tx));
+  return !covered || !((te > t_max) || (tx < t_min) || (te > tx));
 }
 
 // ---------------------------------------------------------------------------
@@ -568,7 +577,7 @@ __device__ __forceinline__ bool coop_step(const DevScene& s, TraceState& st, boo
       uint32_t const i = k - __shfl(bexcl, owner);
       uint32_t const ci = __shfl(st.lj, owner) + i;
       if (k < btotal) {
-        if (cluster_may_hit(s.clo[ci], s.chi[ci], o, dl, iv, tmin, tmax))
+        if (cluster_may_hit(rec_at(s.clo, ci), rec_at(s.chi, ci), o, dl, iv, tmin, tmax))
           atomicOr(&masks[2 * owner], 1u << i);
       }
     }
@@ -627,12 +636,15 @@ __device__ __forceinline__ bool coop_step(const DevScene& s, TraceState& st, boo
     uint32_t const j = (ci << kClusterShift) + (r & (kClusterSize - 1u));
     if (k < total) {
       float t, f, g, det;
-      bool const ok = tri_test(o, d, s.triA[j], s.triB[j], s.triC[j], tmin, tmax, &t, &f, &g, &det);
+      float4 const A = rec_at(s.triA, j), B = rec_at(s.triB, j), C = rec_at(s.triC, j);
+      // the tie-break rank, loaded with the record
+      uint32_t const rank = rec_at(s.rec_rank, j);
+      bool const ok = tri_test(o, d, A, B, C, tmin, tmax, &t, &f, &g, &det);
       if (t != t && !(fabsf(det) < 1e-6f)) {
         atomicMin(&keys[owner], 0ull);
       } else if (ok) {
         unsigned long long const key =
-            (static_cast<unsigned long long>(__float_as_uint(t)) << 32) | (0xFFFFFFFFu - s.rec_rank[j]);
+            (static_cast<unsigned long long>(__float_as_uint(t)) << 32) | (0xFFFFFFFFu - rank);
         atomicMin(&keys[owner], key);
       }
     }

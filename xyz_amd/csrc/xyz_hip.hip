@@ -461,6 +461,11 @@ int scene_create(const xyz_scene_desc* desc, xyz_scene** out) {
   for (size_t i = 0; i < nodes.size(); ++i) nodes[i] = make_uint2(T.nodes[2 * i], T.nodes[2 * i + 1]);
   Clustered CL;
   cluster_leaves(T, desc->v0, nrm, e1, e2, &nodes, &CL);
+  // the traversal addresses the record arrays with 32-bit byte offsets
+  if (CL.A.size() * sizeof(float4) > size_t(0xFFFFFFFFu)) {
+    set_error("xyz_scene_create: the kd leaves hold more than 2^28 triangle references");
+    return XYZ_E_UNSUPPORTED;
+  }
 
   // per-primitive shading records
   std::vector<float4> png(n), pv0(n), pe1(n), pe2(n);
