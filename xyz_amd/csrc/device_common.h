@@ -498,7 +498,7 @@ __device__ __forceinline__ float interval_t_max(const DevScene& s, const TraceSt
   if (st.sp == 0) return __builtin_inff();
   uint32_t const pb = stk[(st.sp - 1) * stride];
   uint32_t f2, n2;
-  return split_distance(s.nodes[pb], st.o, st.d, pb, &f2, &n2);
+  return split_distance(rec_at(s.nodes, pb), st.o, st.d, pb, &f2, &n2);
 }
 
 // Returns true when this lane's (active) query finished in this step.
@@ -512,7 +512,7 @@ __device__ __forceinline__ bool coop_step(const DevScene& s, TraceState& st, boo
                                           unsigned long long* keys, uint8_t* own) {
   uint32_t const lane = __lane_id();
   if (active && !st.in_leaf) {
-    uint2 w = s.nodes[st.node];
+    uint2 w = rec_at(s.nodes, st.node);
     if (kCount) st.nn++;
     uint32_t budget = XYZ_DESCENT_BUDGET;
     bool leaf = (w.x & 3u) == 3u;
@@ -530,7 +530,7 @@ __device__ __forceinline__ bool coop_step(const DevScene& s, TraceState& st, boo
         st.node = nearc;
       }
       if (XYZ_DESCENT_BUDGET != 0 && --budget == 0) break;  // st.node is loaded next step
-      w = s.nodes[st.node];
+      w = rec_at(s.nodes, st.node);
       if (kCount) st.nn++;
       leaf = (w.x & 3u) == 3u;
     }
@@ -692,7 +692,7 @@ __device__ __forceinline__ bool coop_step(const DevScene& s, TraceState& st, boo
   // t_max = the split distance of the entry below
   st.sp--;
   uint32_t const p = stk[st.sp * stride];
-  st.node = far_child(s.nodes[p], st.o, p);
+  st.node = far_child(rec_at(s.nodes, p), st.o, p);
   st.t_min = st.t_max;
   st.t_max = interval_t_max(s, st, stk, stride);
   return false;

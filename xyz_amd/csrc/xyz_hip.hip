@@ -461,9 +461,10 @@ int scene_create(const xyz_scene_desc* desc, xyz_scene** out) {
   for (size_t i = 0; i < nodes.size(); ++i) nodes[i] = make_uint2(T.nodes[2 * i], T.nodes[2 * i + 1]);
   Clustered CL;
   cluster_leaves(T, desc->v0, nrm, e1, e2, &nodes, &CL);
-  // the traversal addresses the record arrays with 32-bit byte offsets
-  if (CL.A.size() * sizeof(float4) > size_t(0xFFFFFFFFu)) {
-    set_error("xyz_scene_create: the kd leaves hold more than 2^28 triangle references");
+  // the traversal addresses the node and record arrays with 32-bit byte offsets
+  if (CL.A.size() * sizeof(float4) > size_t(0xFFFFFFFFu) ||
+      nodes.size() * sizeof(uint2) > size_t(0xFFFFFFFFu)) {
+    set_error("xyz_scene_create: the kd tree holds more than 2^28 triangle references or 2^29 nodes");
     return XYZ_E_UNSUPPORTED;
   }
 
