@@ -7,7 +7,8 @@
 // The rays aim at the hardest cases for the error bound: grazing rays whose
 // |det| is just above the 1e-6 threshold, on triangles up to the size where
 // culling switches itself off, intervals shrunk to the computed t itself,
-// axis-parallel directions, origins on or inside the cluster box.  Also
+// axis-parallel directions, origins on or inside the cluster box, smooth
+// patches of a curved surface met at every tilt.  Also
 // reports the largest observed (distance of o + t d from the box) / margin.
 //
 // usage: cull_fuzz <cases> <seed>; exit status 1 on a violation.
@@ -63,7 +64,8 @@ void make_record(float3 v0, float3 v1, float3 v2, float4* A, float4* B, float4* 
   *C = make_float4(c[0], c[1], c[2], nz);
 }
 
-// The margin cluster_may_hit adds to the stored box (for the report only).
+// What cluster_may_hit allows a computed hit point to lie outside the stored
+// box (for the report only): the box's widening plus the t widening's length.
 double margin(float4 L, float4 H, float3 o, float3 d) {
   double const u = 5.9604644775390625e-08;
   double const dl = std::sqrt(double(d.x) * d.x + double(d.y) * d.y + double(d.z) * d.z);
@@ -73,7 +75,8 @@ double margin(float4 L, float4 H, float3 o, float3 d) {
     double const r = std::max(std::fabs(oo[k] - lo[k]), std::fabs(hi[k] - oo[k]));
     R2 += r * r;
   }
-  return std::sqrt(R2) * (dl * L.w + 8 * u) + dl * H.w;
+  double const tr = std::sqrt(R2) * (dl * L.w + 8 * u);
+  return 2.0 * tr + dl * H.w;
 }
 
 double box_distance(float4 L, float4 H, V p) {
@@ -119,9 +122,9 @@ void check(const std::vector<float4>& A, const std::vector<float4>& B, const std
     double const mg = margin(L, H, o, d);
     V const p = add(V{o.x, o.y, o.z}, mul(V{d.x, d.y, d.z}, double(t)));
     double const dist = box_distance(L, H, p);
-    // where culling is on: lam = |d| P / 64 <= 0.01
+    // where culling is on: lam = |d| P / 30 <= 0.01
     if (mg > 0 && std::isfinite(mg) && L.w * std::sqrt(double(d.x) * d.x + double(d.y) * d.y +
-                                                       double(d.z) * d.z) / 64.0 <= 0.01)
+                                                       double(d.z) * d.z) / 30.0 <= 0.01)
       st->worst = std::max(st->worst, dist / mg);
   }
 }
@@ -200,6 +203,52 @@ void cluster_case(Stats* st) {
   }
 }
 
+// A smooth patch: up to 16 triangles of a gently curved height field (normals
+// within a few degrees of each other, the case where the normal cone bounds
+// |det| far above 1e-6), rays at tilts from grazing to head-on, aimed at
+// points of the patch, with intervals around the hit.
+void patch_case(Stats* st) {
+  double const S = loguni(0.01, 50.0);
+  V const ctr{S * sym(), S * sym(), S * sym()};
+  V const nz = rand_dir();
+  V tx = unit(cross(nz, rand_dir()));
+  V const ty = cross(nz, tx);
+  double const h = loguni(1e-3, 0.5);           // cell size
+  double const curv = loguni(1e-4, 1.0) / h;    // curvature of the field
+  auto at = [&](double x, double y) {
+    double const z = 0.5 * curv * (x * x + 0.7 * y * y);
+    return add(ctr, add(add(mul(tx, x), mul(ty, y)), mul(nz, z)));
+  };
+  int const cells = 1 + int(next_u64() % 8);     // 2 triangles each
+  std::vector<float4> A, B, C;
+  for (int k = 0; k < cells; ++k) {
+    double const x0 = h * (k % 4 - 2), y0 = h * (k / 4 - 1);
+    V const p00 = at(x0, y0), p10 = at(x0 + h, y0), p01 = at(x0, y0 + h), p11 = at(x0 + h, y0 + h);
+    A.resize(A.size() + 2); B.resize(B.size() + 2); C.resize(C.size() + 2);
+    size_t const m = A.size() - 2;
+    make_record(f3(p00), f3(p10), f3(p11), &A[m], &B[m], &C[m]);
+    make_record(f3(p00), f3(p11), f3(p01), &A[m + 1], &B[m + 1], &C[m + 1]);
+  }
+  for (int r = 0; r < 6; ++r) {
+    size_t const m = next_u64() % A.size();
+    double f = uni(), g = uni();
+    if (f + g > 1) { f = 1 - f; g = 1 - g; }
+    V const P = add(V{A[m].x, A[m].y, A[m].z}, add(mul(V{B[m].x, B[m].y, B[m].z}, f),
+                                                  mul(V{C[m].x, C[m].y, C[m].z}, g)));
+    // tilt from the surface: log-uniform from grazing (1e-6) to head-on
+    double const tilt = loguni(1e-6, 1.0);
+    V const inplane = unit(add(mul(tx, sym()), mul(ty, sym())));
+    V dir = unit(add(mul(inplane, std::sqrt(std::max(0.0, 1.0 - tilt * tilt))), mul(nz, tilt * (uni() < 0.5 ? -1 : 1))));
+    dir = mul(dir, loguni(1e-2, 1e2));
+    double const dist = loguni(1e-4, 300.0);
+    V const o = sub(P, mul(dir, dist / std::sqrt(dot(dir, dir))));
+    float3 const of = f3(o), df = f3(dir);
+    check(A, B, C, of, df, 1e-5f, INFINITY, st);
+    float const a = float(loguni(1e-5, 300.0)), b = float(loguni(1e-5, 300.0));
+    check(A, B, C, of, df, std::min(a, b), std::max(a, b), st);
+  }
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -209,6 +258,7 @@ int main(int argc, char** argv) {
   for (uint64_t i = 0; i < cases; ++i) {
     grazing_case(&st);
     cluster_case(&st);
+    patch_case(&st);
   }
   std::printf("checks %llu accepted %llu violations %llu culled %.4f worst distance/margin %.3e\n",
               (unsigned long long)st.checks, (unsigned long long)st.accepted,

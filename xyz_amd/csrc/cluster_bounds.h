@@ -31,8 +31,8 @@ inline float float_down(double x) {
 // N >= max |e1||e2| and E >= max(|e1|, |e2|) over the triangles, u = 2^-24 and
 // c = u / 1e-6, the record holds the parts of cluster_may_hit's margin that do
 // not depend on the ray:
-//   P >= 64 c N                 (64 lam / |d|)
-//   Q >= 28 c N E               (28 lam E / |d|)
+//   P >= 30 c N                 (30 lam / |d| at |det| = 1e-6)
+//   Q >= 13 c N E               (13 lam E / |d| at |det| = 1e-6)
 //   K >= 8 u E + 4 u (largest |box coordinate|) + 1e-30,
 // so that the margin of a ray with |d| = dl and distance bound R is
 // R (dl P + 8 u) + dl Q on top of the box already widened by K.  A non-finite
@@ -65,8 +65,8 @@ inline void cluster_bounds(const float4* A, const float4* B, const float4* C, ui
   double big = 0.0;
   for (int k = 0; k < 3; ++k) big = std::max(big, std::max(std::fabs(lo[k]), std::fabs(hi[k])));
   double const K = (8.0 * u * E + 4.0 * u * big + 1e-30) * up;
-  float const P = bad ? NAN : float_up(64.0 * c * N * up);
-  float const Q = bad ? NAN : float_up(28.0 * c * N * E * up);
+  float const P = bad ? NAN : float_up(30.0 * c * N * up);
+  float const Q = bad ? NAN : float_up(13.0 * c * N * E * up);
   *lo_out = make_float4(float_down(lo[0] - K), float_down(lo[1] - K), float_down(lo[2] - K), P);
   *hi_out = make_float4(float_up(hi[0] + K), float_up(hi[1] + K), float_up(hi[2] + K), Q);
 }

@@ -316,28 +316,43 @@ __device__ __forceinline__ bool trace_step(const DevScene& s, TraceState& st, ui
 // Cluster culling.  cluster_may_hit() returns false only when no triangle of
 // the cluster can be accepted by tri_test with t in [t_min, t_max].
 //
-// Error bound (first order in u = 2^-24, every operation of tri_test rounded
-// once): for an accepted triangle the point o + t d (t as computed) lies within
-//   Delta = lam (32 R + 14 E) + 2.3 u R + 5.4 u E
-// of the triangle, where lam = u |d| N / 1e-6 (an accepted |det| is at least
-// 1e-6, and N = |e1||e2| bounds |n| and the rounding of n against
-// cross(e2, e1)), R bounds both |v0 - o| and |t d| (R = the largest distance
-// from o to the cluster's box, padded), E = max(|e1|, |e2|), provided
-// lam <= 0.01.  The box is expanded by twice the lam term plus 8 u (R + E) +
-// 4 u (largest box coordinate) + 1e-30; the ray-independent part of that
-// (the last three terms but 8 u R) is built into the stored box, the rest is
-//   D = R (|d| P + 8 u) + |d| Q,   P = 64 u N / 1e-6,  Q = 28 u N E / 1e-6
-// (cluster_bounds.h), and the slab interval is widened by 6e-7 of its ends (at
-// least 5 ulp), which covers the box test's own rounding.  Rays or clusters outside the range where no
-// operation of tri_test can overflow (R < 1e12, 1e-12 < |d| < 1e12) or with
-// lam > 0.01 are never culled.
+// Error bound.  Let n* = e2 x e1 (exact), D* = d . n*, m* = v0 - o and (T, F, G)
+// the exact solution of o + T d = v0 + F e1 + G e2 (T = m* . n* / D*,
+// F = e2 . (d x m*) / D*, G = -e1 . (d x m*) / D*, the reference's formulas
+// without rounding).  With u = 2^-24, every operation of tri_test rounded once,
+// the stored n within 2.9 u |e1||e2| of n*, det, f and g as computed:
+//   det = D* + eD,  |eD| <= 5.91 u |d| N
+//   fl(e2 . q) = e2 . (d x m*) + eF,  |eF| <= 6.9 u |e2||d||m*|  (q = fl(d x m)
+//   within 3.9 u |d||m*| of d x m*), and alike for e1 and for fl(m . n)
+//   (|eT| <= 6.91 u |m*| N),
+// and, exactly, f - F = F p + (1 + p)(eF - F eD) / det with |p| <= 2.01 u (the
+// reciprocal and the product), alike for g and t.  Here N >= |e1||e2|,
+// E >= max(|e1|, |e2|), lam = u |d| N / |det| (an accepted |det| is at least
+// 1e-6), R >= |m*| and R >= |T d| - (the excess below).  For an accepted
+// triangle (f, g >= 0, fl(f + g) <= 1) the point v0 + f e1 + g e2 lies within
+// 2 u E of the triangle, so the point of the ray's line at T lies within
+//   |f - F||e1| + |g - G||e2| + 2 u E <= 15 lam R + 6.5 lam E + 4.2 u E
+// of the triangle, and the computed t is within
+//   |t - T| <= (14.6 lam R + 2.7 u R) / |d|
+// of T; both include the second-order terms while lam <= 0.01 (|F|, |G| <= 1
+// + their errors, |T d| <= 1.3 R).  So the ray's line must meet the box
+// widened by the first bound at a parameter within the second bound of
+// [t_min, t_max].  The test widens the box by twice the lam terms plus 8 u R
+// and 8 u E + 4 u (largest box coordinate) + 1e-30 (that part, which does not
+// depend on the ray, is built into the stored box), i.e. by
+//   D = R (|d| P + 8 u) + |d| Q,   P = 30 u N / 1e-6,  Q = 13 u N E / 1e-6
+// (cluster_bounds.h), widens the slab interval by 6e-7 of its ends (at least
+// 5 ulp, the box test's own rounding), and widens [t_min, t_max] by
+// R (|d| P + 8 u) / |d|, with 1 / |d| bounded above by min |1 / d_i|.  Rays or
+// clusters outside the range where no operation of tri_test can overflow
+// (R < 1e12, 1e-12 < |d| < 1e12) or with lam > 0.01 are never culled.
 //
 // A zero direction component has an infinite reciprocal: the slab of that axis
 // then gives (-inf, +inf) when o lies strictly inside it and an empty interval
 // (culled) otherwise.  Where o lies exactly on a face of the widened box the
 // product 0 * inf is NaN and fminf / fmaxf take the other end, which culls:
 // the ray's points keep that coordinate, which lies D + K beyond the
-// triangles' box, farther than Delta.
+// triangles' box, farther than the first bound.
 // Scale of the culling margin: 1 in the library; the exactness test builds a
 // copy with 0 to show that it detects a missing margin.
 #ifndef XYZ_CULL_MARGIN
@@ -379,16 +394,17 @@ __host__ __device__ __forceinline__ void cull_ray(float3 d, float* dlen, float3*
 __host__ __device__ __forceinline__ bool cluster_may_hit(float4 L, float4 H, float3 o, float dlen,
                                                          float3 inv, float t_min, float t_max) {
   float const u = 5.9604645e-08f;
-  float const a = dlen * L.w;  // 64 lam
+  float const a = dlen * L.w;  // 30 lam
   // the largest distance from o to a point of the box
   float const rx = fmaxf(fabsf(o.x - L.x), fabsf(H.x - o.x));
   float const ry = fmaxf(fabsf(o.y - L.y), fabsf(H.y - o.y));
   float const rz = fmaxf(fabsf(o.z - L.z), fabsf(H.z - o.z));
   float const R = approx_sqrt(rx * rx + ry * ry + rz * rz) * 1.000002f;
-  float const D = XYZ_CULL_MARGIN * (R * (a + 8.0f * u) + dlen * H.w) * 1.00001f;
+  float const tr = R * (a + 8.0f * u);  // also the t widening, times |d|
+  float const D = XYZ_CULL_MARGIN * (tr + dlen * H.w) * 1.00001f;
   // outside the covered range every cluster is kept (computed without a
   // branch, so that the record's loads are not split around one)
-  bool const covered = (a <= 0.64f) && (R < 1e12f) && (D < 1e30f);
+  bool const covered = (a <= 0.3f) && (R < 1e12f) && (D < 1e30f);
   float const px = ((L.x - D) - o.x) * inv.x, qx = ((H.x + D) - o.x) * inv.x;
   float const py = ((L.y - D) - o.y) * inv.y, qy = ((H.y + D) - o.y) * inv.y;
   float const pz = ((L.z - D) - o.z) * inv.z, qz = ((H.z + D) - o.z) * inv.z;
@@ -397,7 +413,10 @@ __host__ __device__ __forceinline__ bool cluster_may_hit(float4 L, float4 H, flo
   // widen by 6e-7 relative (at least 5 ulp); multiplications keep infinities
   te = te * ((te > 0.0f) ? 0.9999994f : 1.0000006f);
   tx = tx * ((tx > 0.0f) ? 1.0000006f : 0.9999994f);
-  return !covered || !((te > t_max) || (tx < t_min) || (te > tx));
+  // the computed t's distance from the line's parameter, 1 / |d| <= min |1 / d_i|
+  float const dt = XYZ_CULL_MARGIN * tr *
+                   fminf(fminf(fabsf(inv.x), fabsf(inv.y)), fabsf(inv.z)) * 1.00001f;
+  return !covered || !((te - dt > t_max) || (tx + dt < t_min) || (te > tx));
 }
 
 // ---------------------------------------------------------------------------
