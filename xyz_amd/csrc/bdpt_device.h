@@ -399,10 +399,10 @@ __device__ __forceinline__ float3 bdpt_sample(const DevScene& s, const DevCamera
       if (h.prim < 0) break;
       uint32_t bsdf;
       {
-        float4 const ngb = s.prim_ng[h.prim];
+        float4 const ngb = gp(s.prim_ng)[h.prim];
         bsdf = __float_as_uint(ngb.w);
       }
-      DevBsdf const bs = s.bsdfs[bsdf];
+      DevBsdf const bs = gp(s.bsdfs)[bsdf];
       if (bs.type == XYZ_LIGHT) {
         nw.tri = -1;
         break;
@@ -435,7 +435,7 @@ __device__ __forceinline__ float3 bdpt_sample(const DevScene& s, const DevCamera
   for (uint32_t k = 2; k < kmax; ++k) {
     BVert const& lv = L[k];
     if (lv.tri < 0) break;
-    uint32_t const ty = s.bsdfs[__float_as_uint(s.prim_ng[lv.tri].w)].type;
+    uint32_t const ty = gp(s.bsdfs)[__float_as_uint(gp(s.prim_ng)[lv.tri].w)].type;
     if (ty == XYZ_MIRROR || ty == XYZ_GLASS) continue;
     float3 const d = normalize3(sub3(lv.pos, E[1].pos));
     float const cog = -dot3(d, lv.ng);
@@ -495,7 +495,7 @@ __device__ __forceinline__ float3 bdpt_sample(const DevScene& s, const DevCamera
     wn.cos_out_geo = dot3(wn.wout, nw.ng);
     nw.g = wn.cos_out_geo * we.cos_in_sh / (h.t * h.t);
     we = wn;
-    DevBsdf const bs = s.bsdfs[bsdf];
+    DevBsdf const bs = gp(s.bsdfs)[bsdf];
     if (++k >= kmax) {
       E[k + 1].tri = -1;
       break;
@@ -516,7 +516,7 @@ __device__ __forceinline__ float3 bdpt_sample(const DevScene& s, const DevCamera
   for (uint32_t t = 2; t < kmax; ++t) {
     BVert& ev = E[t];
     if (ev.tri < 0) break;
-    DevBsdf const eb = s.bsdfs[__float_as_uint(s.prim_ng[ev.tri].w)];
+    DevBsdf const eb = gp(s.bsdfs)[__float_as_uint(gp(s.prim_ng)[ev.tri].w)];
     if (eb.type == XYZ_LIGHT) {
       float wst = 1.0f;
       if (t > 2) {
@@ -533,7 +533,7 @@ __device__ __forceinline__ float3 bdpt_sample(const DevScene& s, const DevCamera
     for (uint32_t k = 1; (k + t - 1) < kmax; ++k) {
       BVert const& lv = L[k];
       if (lv.tri < 0) break;
-      uint32_t const lt = s.bsdfs[__float_as_uint(s.prim_ng[lv.tri].w)].type;
+      uint32_t const lt = gp(s.bsdfs)[__float_as_uint(gp(s.prim_ng)[lv.tri].w)].type;
       if (lt == XYZ_MIRROR || lt == XYZ_GLASS) continue;
       float3 const d = normalize3(sub3(ev.pos, lv.pos));
       float const cog = dot3(d, lv.ng);

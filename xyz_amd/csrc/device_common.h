@@ -104,12 +104,28 @@ struct HitRec {
   bool back;
 };
 
+// A scene array as a pointer the compiler knows addresses global memory.  A
+// kernel that finds its scene in a job table in memory (not in a kernel
+// argument) cannot tell which address space the table's pointers address and
+// would read through flat loads, which make a wave wait for its LDS and
+// memory operations together; every array of a scene is device memory.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define XYZ_GLOBAL_AS __attribute__((address_space(1)))
+#else
+#define XYZ_GLOBAL_AS
+#endif
+template <typename T>
+__device__ __forceinline__ const XYZ_GLOBAL_AS T* gp(const T* p) {
+  return (const XYZ_GLOBAL_AS T*)p;
+}
+
 // Element i of one of the scene's record arrays, addressed by a 32-bit byte
 // offset (scene creation keeps every record array below 4 GiB), so that the
 // load takes the array's address as a uniform base plus a per-lane offset.
 template <typename T>
 __device__ __forceinline__ T rec_at(const T* base, uint32_t i) {
-  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * uint32_t(sizeof(T)));
+  auto const b = (const XYZ_GLOBAL_AS char*)base;
+  return *(const XYZ_GLOBAL_AS T*)(b + i * uint32_t(sizeof(T)));
 }
 
 __device__ __forceinline__ float sel3(float3 v, uint32_t a) {
@@ -251,7 +267,7 @@ __device__ __forceinline__ bool trace_step(const DevScene& s, TraceState& st, ui
                                            uint32_t stride) {
   if (!st.in_leaf) {
     // descend to a leaf: xyz/geom/kdtree.cpp:82-108
-    uint2 w = s.nodes[st.node];
+    uint2 w = gp(s.nodes)[st.node];
     if (kCount) st.nn++;
     while ((w.x & 3u) != 3u) {
       uint32_t farc, nearc;
@@ -266,7 +282,7 @@ __device__ __forceinline__ bool trace_step(const DevScene& s, TraceState& st, ui
         st.t_max = e;
         st.node = nearc;
       }
-      w = s.nodes[st.node];
+      w = gp(s.nodes)[st.node];
       if (kCount) st.nn++;
     }
     st.lc0 = w.y;
@@ -277,12 +293,12 @@ __device__ __forceinline__ bool trace_step(const DevScene& s, TraceState& st, ui
   }
   // up to kLeafChunk triangles of the leaf, in the reference's order (kdtree.cpp:140-150)
   uint32_t const stop = (st.lend - st.lj > kLeafChunk) ? st.lj + kLeafChunk : st.lend;
-  uint32_t const* const pos = s.rank_pos + (size_t(st.lc0) << kClusterShift);
+  auto const pos = gp(s.rank_pos) + (size_t(st.lc0) << kClusterShift);
   for (uint32_t r = st.lj; r < stop; ++r) {
     uint32_t const j = pos[r];
     if (kCount) st.nt++;
     float t, f, g, det;
-    if (tri_test(st.o, st.d, s.triA[j], s.triB[j], s.triC[j], st.t_min, st.t_max, &t, &f, &g,
+    if (tri_test(st.o, st.d, gp(s.triA)[j], gp(s.triB)[j], gp(s.triC)[j], st.t_min, st.t_max, &t, &f, &g,
                  &det)) {
       st.t_max = t;
       st.f = f;
@@ -299,13 +315,13 @@ __device__ __forceinline__ bool trace_step(const DevScene& s, TraceState& st, ui
   st.sp--;
   uint32_t const p = stk[st.sp * stride];
   uint32_t farc, nearc;
-  float const e = split_distance(s.nodes[p], st.o, st.d, p, &farc, &nearc);
+  float const e = split_distance(gp(s.nodes)[p], st.o, st.d, p, &farc, &nearc);
   st.node = farc;
   st.t_min = e;
   if (st.sp > 0) {
     uint32_t const pb = stk[(st.sp - 1) * stride];
     uint32_t f2, n2;
-    st.t_max = split_distance(s.nodes[pb], st.o, st.d, pb, &f2, &n2);
+    st.t_max = split_distance(gp(s.nodes)[pb], st.o, st.d, pb, &f2, &n2);
   } else {
     st.t_max = __builtin_inff();
   }
@@ -495,11 +511,11 @@ __device__ __forceinline__ uint32_t batch_owner(uint8_t* own, uint32_t excl, uin
 
 // The current leaf in the reference's order, with the sequential semantics.
 __device__ __forceinline__ void leaf_sequential(const DevScene& s, TraceState& st) {
-  uint32_t const* const pos = s.rank_pos + (size_t(st.lc0) << kClusterShift);
+  auto const pos = gp(s.rank_pos) + (size_t(st.lc0) << kClusterShift);
   for (uint32_t r = 0; r < st.ln; ++r) {
     uint32_t const j = pos[r];
     float t, f, g, det;
-    if (tri_test(st.o, st.d, s.triA[j], s.triB[j], s.triC[j], st.t_min, st.t_max, &t, &f, &g,
+    if (tri_test(st.o, st.d, gp(s.triA)[j], gp(s.triB)[j], gp(s.triC)[j], st.t_min, st.t_max, &t, &f, &g,
                  &det)) {
       st.t_max = t;
       st.f = f;
@@ -624,7 +640,7 @@ __device__ __forceinline__ bool coop_step(const DevScene& s, TraceState& st, boo
     cull_ray(st.d, &dlen, &inv);
     for (uint32_t i = 0; i < ncl; ++i) {
       uint32_t const ci = st.lj + i;
-      if (cluster_may_hit(s.clo[ci], s.chi[ci], st.o, dlen, inv, st.t_min, st.t_max)) {
+      if (cluster_may_hit(gp(s.clo)[ci], gp(s.chi)[ci], st.o, dlen, inv, st.t_min, st.t_max)) {
         list |= static_cast<unsigned long long>(i) << (4u * nsurv);
         ++nsurv;
         c += (ci + 1 == st.lend) ? st.ln - ((ci - st.lc0) << kClusterShift) : kClusterSize;
@@ -685,11 +701,11 @@ __device__ __forceinline__ bool coop_step(const DevScene& s, TraceState& st, boo
         st.t_max = __uint_as_float(uint32_t(key >> 32));
         if (st.lj == st.lend) {
           uint32_t const rank = 0xFFFFFFFFu - uint32_t(key & 0xFFFFFFFFull);
-          uint32_t const j = s.rank_pos[(size_t(st.lc0) << kClusterShift) + rank];
+          uint32_t const j = gp(s.rank_pos)[(size_t(st.lc0) << kClusterShift) + rank];
           st.hit = int32_t(j);
           if constexpr (kFG) {
             float t, f, g, det;
-            (void)tri_test(st.o, st.d, s.triA[j], s.triB[j], s.triC[j], st.t_min, st.t_max, &t,
+            (void)tri_test(st.o, st.d, gp(s.triA)[j], gp(s.triB)[j], gp(s.triC)[j], st.t_min, st.t_max, &t,
                            &f, &g, &det);
             st.f = f;
             st.g = g;
@@ -720,7 +736,7 @@ __device__ __forceinline__ bool coop_step(const DevScene& s, TraceState& st, boo
 __device__ __forceinline__ HitRec trace_result(const DevScene& s, const TraceState& st) {
   HitRec h;
   if (st.hit >= 0) {
-    h.prim = s.ref_prim[st.hit];
+    h.prim = gp(s.ref_prim)[st.hit];
     h.t = st.t_max;
     h.f = st.f;
     h.g = st.g;

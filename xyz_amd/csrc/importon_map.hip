@@ -137,7 +137,7 @@ __global__ void __launch_bounds__(kEmitBlock)
         if (h.prim < 0) break;
         uint32_t bsdf_id;
         Basis B = shading_basis(s, h.prim, h.f, h.g, h.back, &bsdf_id);
-        DevBsdf const& bs = s.bsdfs[bsdf_id];
+        auto const& bs = gp(s.bsdfs)[bsdf_id];
         pos = add3(pos, scl3(dir, h.t));
         if (bs.type == XYZ_LAMBERT) {
           basis_about(B.ng, &B.t, &B.b);  // Triangle::GeometricBasis

@@ -292,36 +292,36 @@ __device__ __forceinline__ LightVertex sample_light(const DevScene& s, float u0,
     lv.bsdf = 0;
     return lv;
   }
-  float const total = s.light_cdf[s.n_lights];
+  float const total = gp(s.light_cdf)[s.n_lights];
   float const u = u0 * total;
   // std::upper_bound over cdf[0..n]: first index with cdf[i] > u
   uint32_t lo = 0, hi = s.n_lights + 1;
   while (lo < hi) {
     uint32_t const mid = (lo + hi) >> 1;
-    if (u < s.light_cdf[mid]) hi = mid; else lo = mid + 1;
+    if (u < gp(s.light_cdf)[mid]) hi = mid; else lo = mid + 1;
   }
   uint32_t n = lo - 1;
   if (n >= s.n_lights) n = s.n_lights - 1;
-  float const v = (u - s.light_cdf[n]) / (s.light_cdf[n + 1] - s.light_cdf[n]);
-  uint32_t const p = s.light_prim[n];
-  float3 const v0 = ld3(s.prim_v0[p]), e1 = ld3(s.prim_e1[p]), e2 = ld3(s.prim_e2[p]);
+  float const v = (u - gp(s.light_cdf)[n]) / (gp(s.light_cdf)[n + 1] - gp(s.light_cdf)[n]);
+  uint32_t const p = gp(s.light_prim)[n];
+  float3 const v0 = ld3(gp(s.prim_v0)[p]), e1 = ld3(gp(s.prim_e1)[p]), e2 = ld3(gp(s.prim_e2)[p]);
   float const ss = sqrtf(v);
   float const tt = u1;
   float const uu = ss * (1.0f - tt);
   float const vv = ss * tt;
   lv.prim = int32_t(p);
   lv.pos = add3(add3(v0, scl3l(uu, e1)), scl3l(vv, e2));
-  float4 const ngb = s.prim_ng[p];
+  float4 const ngb = gp(s.prim_ng)[p];
   lv.ng = ld3(ngb);
   if (s.prim_n0) {
-    float3 const n0 = ld3(s.prim_n0[p]), s1 = ld3(s.prim_s1[p]), s2 = ld3(s.prim_s2[p]);
+    float3 const n0 = ld3(gp(s.prim_n0)[p]), s1 = ld3(gp(s.prim_s1)[p]), s2 = ld3(gp(s.prim_s2)[p]);
     lv.ns = normalize3(add3(add3(n0, scl3l(uu, s1)), scl3l(vv, s2)));
   } else {
     float3 const z = f3(0.f, 0.f, 0.f);
     lv.ns = normalize3(add3(add3(lv.ng, scl3l(uu, z)), scl3l(vv, z)));
   }
   lv.bsdf = __float_as_uint(ngb.w);
-  DevBsdf const& b = s.bsdfs[lv.bsdf];
+  auto const& b = gp(s.bsdfs)[lv.bsdf];
   lv.power = f3(b.phi[0], b.phi[1], b.phi[2]);
   return lv;
 }
@@ -334,13 +334,13 @@ struct Basis {
 // xyz/geom/triangle.cpp:194-219
 __device__ __forceinline__ Basis shading_basis(const DevScene& s, int32_t prim, float f, float g,
                                                bool back, uint32_t* bsdf) {
-  float4 const ngb = s.prim_ng[prim];
+  float4 const ngb = gp(s.prim_ng)[prim];
   *bsdf = __float_as_uint(ngb.w);
   float3 n0, s1, s2;
   if (s.prim_n0) {
-    n0 = ld3(s.prim_n0[prim]);
-    s1 = ld3(s.prim_s1[prim]);
-    s2 = ld3(s.prim_s2[prim]);
+    n0 = ld3(gp(s.prim_n0)[prim]);
+    s1 = ld3(gp(s.prim_s1)[prim]);
+    s2 = ld3(gp(s.prim_s2)[prim]);
   } else {
     n0 = ld3(ngb);
     s1 = s2 = f3(0.f, 0.f, 0.f);
@@ -370,8 +370,8 @@ __device__ __forceinline__ float3 albedo_at(const DevScene& s, const DevBsdf& b,
                                             float f, float g, int32_t* texel) {
   *texel = -1;
   if (b.texture < 0) return f3(b.rgb[0], b.rgb[1], b.rgb[2]);
-  DevTex const tx = s.tex[b.texture];
-  float2 const a = s.prim_uv[3 * prim], c1 = s.prim_uv[3 * prim + 1], c2 = s.prim_uv[3 * prim + 2];
+  DevTex const tx = gp(s.tex)[b.texture];
+  float2 const a = gp(s.prim_uv)[3 * prim], c1 = gp(s.prim_uv)[3 * prim + 1], c2 = gp(s.prim_uv)[3 * prim + 2];
   float const u = (a.x + f * (c1.x - a.x)) + g * (c2.x - a.x);
   float const v = (a.y + f * (c1.y - a.y)) + g * (c2.y - a.y);
   int ix = int(u * float(tx.w));
@@ -380,7 +380,7 @@ __device__ __forceinline__ float3 albedo_at(const DevScene& s, const DevBsdf& b,
   iy = iy < 0 ? 0 : (iy >= int(tx.h) ? int(tx.h) - 1 : iy);
   int const idx = iy * int(tx.w) + ix;
   *texel = idx;
-  const float* p = s.texels + tx.offset + 3 * size_t(idx);
+  auto const p = gp(s.texels) + tx.offset + 3 * size_t(idx);
   return f3(p[0], p[1], p[2]);
 }
 
@@ -515,13 +515,13 @@ __device__ __forceinline__ void importance_counts(const DevImportons& m, float3 
   for (int z = z0; z <= z1; ++z) {
     for (int y = y0; y <= y1; ++y) {
       uint32_t const row = (uint32_t(z) * m.dim[1] + uint32_t(y)) * m.dim[0];
-      uint32_t const b0 = m.cell_start[row + uint32_t(x0)];
-      uint32_t const b1 = m.cell_start[row + uint32_t(x1) + 1u];
+      uint32_t const b0 = gp(m.cell_start)[row + uint32_t(x0)];
+      uint32_t const b1 = gp(m.cell_start)[row + uint32_t(x1) + 1u];
       for (uint32_t i = b0; i < b1; ++i) {
-        float4 const P = m.pos[i];
+        float4 const P = gp(m.pos)[i];
         float3 const d = f3(P.x - p.x, P.y - p.y, P.z - p.z);
         if (!(dot3(d, d) < m.r2)) continue;
-        float3 const w = ld3(m.dir[i]);
+        float3 const w = ld3(gp(m.dir)[i]);
         float const ct = -dot3(w, B.ns);
         if (ct <= 0.0f) continue;
         float3 const vp = add3(d, scl3(w, dot3(d, B.ns)));
@@ -625,8 +625,8 @@ __device__ __forceinline__ void wave_importance_counts(const DevImportons& m, bo
       if (lane < nrows) {
         uint32_t const row = ((uint32_t(z0) + lane / ny) * m.dim[1] + uint32_t(y0) + lane % ny) *
                              m.dim[0];
-        rb0 = m.cell_start[row + uint32_t(x0)];
-        rlen = m.cell_start[row + uint32_t(x1) + 1u] - rb0;
+        rb0 = gp(m.cell_start)[row + uint32_t(x0)];
+        rlen = gp(m.cell_start)[row + uint32_t(x1) + 1u] - rb0;
       }
       uint32_t b0s[kImpRows], pre[kImpRows + 1];
       pre[0] = 0u;
@@ -645,8 +645,8 @@ __device__ __forceinline__ void wave_importance_counts(const DevImportons& m, bo
 #pragma unroll
           for (uint32_t r = 0; r < kImpRows; ++r)
             i = (k >= pre[r] && k < pre[r + 1]) ? b0s[r] + (k - pre[r]) : i;
-          P = m.pos[i];
-          W = m.dir[i];
+          P = gp(m.pos)[i];
+          W = gp(m.dir)[i];
         }
         add_member_bins(members, P, W, have, p, B, m.r2, lane, c);
       }
@@ -654,15 +654,15 @@ __device__ __forceinline__ void wave_importance_counts(const DevImportons& m, bo
       for (int z = z0; z <= z1; ++z) {
         for (int y = y0; y <= y1; ++y) {
           uint32_t const row = (uint32_t(z) * m.dim[1] + uint32_t(y)) * m.dim[0];
-          uint32_t const b0 = m.cell_start[row + uint32_t(x0)];
-          uint32_t const b1 = m.cell_start[row + uint32_t(x1) + 1u];
+          uint32_t const b0 = gp(m.cell_start)[row + uint32_t(x0)];
+          uint32_t const b1 = gp(m.cell_start)[row + uint32_t(x1) + 1u];
           for (uint32_t base = b0; base < b1; base += 64u) {
             uint32_t const i = base + lane;
             bool const have = i < b1;
             float4 P = make_float4(0.f, 0.f, 0.f, 0.f), W = P;
             if (have) {
-              P = m.pos[i];
-              W = m.dir[i];
+              P = gp(m.pos)[i];
+              W = gp(m.dir)[i];
             }
             add_member_bins(members, P, W, have, p, B, m.r2, lane, c);
           }
@@ -767,7 +767,7 @@ __device__ __forceinline__ bool shade_begin(const DevScene& s, const PTConst& c,
   uint32_t const bsdf_id = x.bsdf_id;
   if (-dot3(ps.dir, B.ns) <= 0.0f) return false;
   ps.pos = add3(ps.pos, scl3(ps.dir, h.t));
-  DevBsdf const& b = s.bsdfs[bsdf_id];
+  auto const& b = gp(s.bsdfs)[bsdf_id];
   float const A = s.light_area;
   if (b.type == XYZ_LIGHT && !h.back) {
     float3 const pe = mul3(f3(b.phi[0], b.phi[1], b.phi[2]), ps.power);
@@ -808,7 +808,7 @@ __device__ __forceinline__ bool shade_end(const DevScene& s, const PTConst& c, P
                                           Grad& grad, const uint32_t* hc) {
   *has_sr = false;
   Basis const& B = x.B;
-  DevBsdf const& b = s.bsdfs[x.bsdf_id];
+  auto const& b = gp(s.bsdfs)[x.bsdf_id];
   uint32_t const j = ps.k - 1;
   float4 const u = x.u;
   bool const guided = kGuide && x.guided;

@@ -766,7 +766,7 @@ k_shadow(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBu
         size_t const ti = size_t(vj) * wb.cap + slot;
         float4 const ta = wb.tape_a[ti], tb = wb.tape_b[ti];
         float3 const pw = mul3(ld3(ta), ld3(tb));  // the throughput after the albedo
-        DevBsdf const& lb = s.bsdfs[meta.z];
+        auto const& lb = gp(s.bsdfs)[meta.z];
         wb.tape_c[ti] = make_float4(float(double(lb.phi[0]) * wfac), float(double(lb.phi[1]) * wfac),
                                     float(double(lb.phi[2]) * wfac), 0.0f);
         double const u = kPi * double(s.light_area) * wfac;
