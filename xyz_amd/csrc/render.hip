@@ -59,6 +59,11 @@ constexpr uint32_t kSegPad = 256;      // queue segments start at multiples of t
 #ifndef XYZ_GUIDED_SHADE_BLOCKS
 #define XYZ_GUIDED_SHADE_BLOCKS 4
 #endif
+// Waves per SIMD the extension and shadow kernels are compiled for (0: the
+// compiler's choice).
+#ifndef XYZ_TRACE_WAVES_PER_EU
+#define XYZ_TRACE_WAVES_PER_EU 0
+#endif
 // Workgroups per CU the plain and replay shading kernels are compiled for
 // (1: the compiler's register choice).
 #ifndef XYZ_SHADE_BLOCKS
@@ -489,6 +494,9 @@ k_camera(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBu
 // Persistent: lanes take active paths from the queues (SegFeeder) and trace
 // them one step at a time, so a wave never waits for its slowest ray.
 __global__ void __launch_bounds__(kBlock)
+#if XYZ_TRACE_WAVES_PER_EU
+__attribute__((amdgpu_waves_per_eu(XYZ_TRACE_WAVES_PER_EU, XYZ_TRACE_WAVES_PER_EU)))
+#endif
 k_extend(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBufs wb,
          const uint32_t* __restrict__ qcur) {
   extern __shared__ uint32_t lds_stack[];
@@ -684,6 +692,9 @@ k_shade(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBuf
 
 template <bool kReplay>
 __global__ void __launch_bounds__(kBlock)
+#if XYZ_TRACE_WAVES_PER_EU
+__attribute__((amdgpu_waves_per_eu(XYZ_TRACE_WAVES_PER_EU, XYZ_TRACE_WAVES_PER_EU)))
+#endif
 k_shadow(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBufs wb, GradArgs ga) {
   extern __shared__ uint32_t lds_stack[];
   __shared__ uint32_t vst[kMaxWaveJobs + 1];
