@@ -68,7 +68,7 @@ class Camera(ctypes.Structure):
 class RenderParams(ctypes.Structure):
     _fields_ = [("spp", ctypes.c_uint32), ("sample_offset", ctypes.c_uint32),
                 ("max_depth", ctypes.c_uint32), ("rr_start", ctypes.c_uint32),
-                ("rr_threshold", ctypes.c_float), ("reserved", ctypes.c_uint32),
+                ("rr_threshold", ctypes.c_float), ("engine", ctypes.c_uint32),
                 ("seed", ctypes.c_uint64)]
 
 
@@ -376,12 +376,14 @@ class Scene:
         return dict(paths=st.paths, queries=st.queries, seconds=st.seconds)
 
     def render_bdpt(self, camera, film, spp, max_depth, seed=0, sample_offset=0, splat=None,
-                    stream=None):
+                    stream=None, engine=0):
         """Bidirectional path tracing: adds the per-pixel SUM of `spp` samples' own
         colours to `film` and their light-tracing contributions to `splat` (or to
-        `film` when splat is None); device float32 (H, W, 3), row 0 = bottom."""
+        `film` when splat is None); device float32 (H, W, 3), row 0 = bottom.
+        engine: 0 by the scene's size, 1 per-thread queries, 2 queued queries."""
         cam = make_camera(camera, film.shape[1], film.shape[0])
         p = make_params(spp, max_depth, 0, 0.2, seed, sample_offset)
+        p.engine = engine
         st = RenderStats()
         check(lib().xyz_render_bdpt(self.h, ctypes.byref(cam), ctypes.byref(p), _dptr(film),
                                     _dptr(splat), ctypes.byref(st), _stream(stream)))

@@ -337,17 +337,18 @@ __device__ __forceinline__ float mis_eye_side(const BVert* E, uint32_t t, float 
   return wst;
 }
 
-// One BDPT sample of pixel (px, py): returns the pixel's own colour and adds
-// the light-tracing (t = 1) contributions to `splat` (atomically).
+// The two subpaths of one BDPT sample of pixel (px, py): the set-up of
+// Scene::EvaluateBidirectionalPathTracing, BuildLightsPath and BuildTheEyePath
+// (scene_bidirectionalpathtracing.cpp:8-126, 251-347) into L and E (kmax + 2
+// entries each), with per-thread closest-hit queries.
 template <bool kCount>
-__device__ __forceinline__ float3 bdpt_sample(const DevScene& s, const DevCamera& cam,
-                                              const BdptConst& bc, uint32_t px, uint32_t py,
-                                              uint32_t smp, uint32_t* stk, uint32_t stride,
-                                              float* splat, uint32_t* queries) {
-  uint32_t const W = cam.width, H = cam.height;
+__device__ __forceinline__ void bdpt_build(const DevScene& s, const DevCamera& cam,
+                                           const BdptConst& bc, uint32_t px, uint32_t py,
+                                           uint32_t smp, uint32_t* stk, uint32_t stride,
+                                           BVert* L, BVert* E, uint32_t* queries) {
+  uint32_t const W = cam.width;
   uint32_t const pixel = py * W + px;
   uint32_t const kmax = bc.c.kmax;
-  BVert L[kBdptMaxDepth + 3], E[kBdptMaxDepth + 3];
   for (uint32_t i = 0; i < kmax + 2; ++i) {
     bvert_clear(L[i]);
     bvert_clear(E[i]);
@@ -431,38 +432,6 @@ __device__ __forceinline__ float3 bdpt_sample(const DevScene& s, const DevCamera
     }
   }
 
-  // EvaluatePathS1 (t = 1)
-  for (uint32_t k = 2; k < kmax; ++k) {
-    BVert const& lv = L[k];
-    if (lv.tri < 0) break;
-    uint32_t const ty = gp(s.bsdfs)[__float_as_uint(gp(s.prim_ng)[lv.tri].w)].type;
-    if (ty == XYZ_MIRROR || ty == XYZ_GLASS) continue;
-    float3 const d = normalize3(sub3(lv.pos, E[1].pos));
-    float const cog = -dot3(d, lv.ng);
-    if (cog <= 0.0f) continue;
-    if (-dot3(d, lv.ns) <= 0.0f) continue;
-    float const cis = dot3(d, E[1].ns);
-    if (cis <= 0.0f) continue;
-    float fx, fy;
-    if (!film_position(cam, sub3(lv.pos, E[1].pos), &fx, &fy)) continue;
-    HitRec const h = find(E[1].pos, d);
-    if (h.prim != lv.tri) continue;
-    // fx < 1 can still round fx * W up to W: keep the last column / row
-    uint32_t const jx = min(uint32_t(fx * float(int(W))), W - 1u);
-    uint32_t const jy = min(uint32_t(fy * float(int(H))), H - 1u);
-    float const flux = bc.C / fourth(cis);
-    float const G = cis * cog / (h.t * h.t);
-    L[k + 1].s_prev = flux;
-    float const wst = rcpf(mis_light_side(L, k, G, 1.0f));
-    double const sc1 = double(flux * G) * kInvPi * double(wst);
-    float3 const pw = mul3(lv.power, E[1].power);
-    float const w = float(sc1);
-    float* dst = splat + 3 * size_t(jy * W + jx);
-    atomicAdd(dst + 0, pw.x * w);
-    atomicAdd(dst + 1, pw.y * w);
-    atomicAdd(dst + 2, pw.z * w);
-  }
-
   // BuildTheEyePath.  we: the working part of the eye path's last vertex.
   BWork we;
   we.win = primary_dir(cam, px, py, de.x, de.y);
@@ -510,6 +479,54 @@ __device__ __forceinline__ float3 bdpt_sample(const DevScene& s, const DevCamera
       break;
     }
   }
+}
+
+// The connections of one sample over its built subpaths: EvaluatePathS1 (light
+// tracing, t = 1) and EvaluatePathST (scene_bidirectionalpathtracing.cpp:127-250,
+// 348-522)
+// in the reference's order.  Every visibility query goes through q.find(o, d),
+// in that order; which queries are made depends on the vertices alone (the
+// strategies' densities the connections overwrite are not among their
+// conditions), so a pass that only records them makes the same ones.  The
+// light-tracing contributions go to `splat` (atomically), the own colour is
+// returned.
+template <class Q>
+__device__ __forceinline__ float3 bdpt_connect(const DevScene& s, const DevCamera& cam,
+                                               const BdptConst& bc, BVert* L, BVert* E,
+                                               float* splat, Q& q) {
+  uint32_t const W = cam.width, H = cam.height;
+  uint32_t const kmax = bc.c.kmax;
+  // EvaluatePathS1 (t = 1)
+  for (uint32_t k = 2; k < kmax; ++k) {
+    BVert const& lv = L[k];
+    if (lv.tri < 0) break;
+    uint32_t const ty = gp(s.bsdfs)[__float_as_uint(gp(s.prim_ng)[lv.tri].w)].type;
+    if (ty == XYZ_MIRROR || ty == XYZ_GLASS) continue;
+    float3 const d = normalize3(sub3(lv.pos, E[1].pos));
+    float const cog = -dot3(d, lv.ng);
+    if (cog <= 0.0f) continue;
+    if (-dot3(d, lv.ns) <= 0.0f) continue;
+    float const cis = dot3(d, E[1].ns);
+    if (cis <= 0.0f) continue;
+    float fx, fy;
+    if (!film_position(cam, sub3(lv.pos, E[1].pos), &fx, &fy)) continue;
+    HitRec const h = q.find(E[1].pos, d);
+    if (h.prim != lv.tri) continue;
+    // fx < 1 can still round fx * W up to W: keep the last column / row
+    uint32_t const jx = min(uint32_t(fx * float(int(W))), W - 1u);
+    uint32_t const jy = min(uint32_t(fy * float(int(H))), H - 1u);
+    float const flux = bc.C / fourth(cis);
+    float const G = cis * cog / (h.t * h.t);
+    L[k + 1].s_prev = flux;
+    float const wst = rcpf(mis_light_side(L, k, G, 1.0f));
+    double const sc1 = double(flux * G) * kInvPi * double(wst);
+    float3 const pw = mul3(lv.power, E[1].power);
+    float const w = float(sc1);
+    float* dst = splat + 3 * size_t(jy * W + jx);
+    atomicAdd(dst + 0, pw.x * w);
+    atomicAdd(dst + 1, pw.y * w);
+    atomicAdd(dst + 2, pw.z * w);
+  }
 
   // EvaluatePathST
   float3 color = f3(0.f, 0.f, 0.f);
@@ -542,7 +559,7 @@ __device__ __forceinline__ float3 bdpt_sample(const DevScene& s, const DevCamera
       if (-dot3(d, ev.ng) <= 0.0f) continue;
       float const cis = -dot3(d, ev.ns);
       if (cis <= 0.0f) continue;
-      HitRec const h = find(lv.pos, d);
+      HitRec const h = q.find(lv.pos, d);
       if (h.prim != ev.tri) continue;
       float const G = cis * cog / (h.t * h.t);
       L[k + 1].s_prev = float(kInvPi);
@@ -555,6 +572,118 @@ __device__ __forceinline__ float3 bdpt_sample(const DevScene& s, const DevCamera
     }
   }
   return color;
+}
+
+// Visibility queries traced at once by the calling thread.
+template <bool kCount>
+struct BdptTrace {
+  const DevScene* s;
+  uint32_t* stk;
+  uint32_t stride;
+  uint32_t* queries;
+  __device__ __forceinline__ HitRec find(float3 o, float3 d) {
+    if (kCount) ++*queries;
+    return trace_closest<false>(*s, o, d, stk, stride, nullptr, nullptr);
+  }
+};
+
+__device__ __forceinline__ HitRec bdpt_no_hit() {
+  HitRec h;
+  h.prim = -1;
+  h.t = __builtin_inff();
+  h.f = h.g = 0.0f;
+  h.back = false;
+  return h;
+}
+
+// Counts a sample's visibility queries (every answer a miss).
+struct BdptCountQueries {
+  uint32_t n;
+  __device__ __forceinline__ HitRec find(float3, float3) {
+    ++n;
+    return bdpt_no_hit();
+  }
+};
+
+// Writes a sample's visibility rays to consecutive queue entries from `pos`
+// (origins and directions as 3 floats each; every answer a miss).
+struct BdptRecordQueries {
+  float* org;
+  float* dir;
+  uint32_t pos;
+  __device__ __forceinline__ HitRec find(float3 o, float3 d) {
+    size_t const k = 3 * size_t(pos++);
+    org[k] = o.x;
+    org[k + 1] = o.y;
+    org[k + 2] = o.z;
+    dir[k] = d.x;
+    dir[k + 1] = d.y;
+    dir[k + 2] = d.z;
+    return bdpt_no_hit();
+  }
+};
+
+// Answers a sample's visibility queries, in order, from the closest hits of
+// its recorded rays.
+struct BdptReplayQueries {
+  const int32_t* prim;
+  const float* t;
+  uint32_t pos;
+  __device__ __forceinline__ HitRec find(float3, float3) {
+    HitRec h = bdpt_no_hit();
+    h.prim = prim[pos];
+    h.t = t[pos];
+    ++pos;
+    return h;
+  }
+};
+
+// One BDPT sample of pixel (px, py) traced by one thread: returns the pixel's
+// own colour and adds the light-tracing (t = 1) contributions to `splat`.
+template <bool kCount>
+__device__ __forceinline__ float3 bdpt_sample(const DevScene& s, const DevCamera& cam,
+                                              const BdptConst& bc, uint32_t px, uint32_t py,
+                                              uint32_t smp, uint32_t* stk, uint32_t stride,
+                                              float* splat, uint32_t* queries) {
+  BVert L[kBdptMaxDepth + 3], E[kBdptMaxDepth + 3];
+  bdpt_build<kCount>(s, cam, bc, px, py, smp, stk, stride, L, E, queries);
+  BdptTrace<kCount> q{&s, stk, stride, queries};
+  return bdpt_connect(s, cam, bc, L, E, splat, q);
+}
+
+// Stored subpath vertices: entry k of path p (0 light, 1 eye) of item i at
+// v[((p * entries + k) * 4 + f) * items + i] for its four float4 fields f, so
+// that neighbouring items' fields are neighbours.  (pos, tri + 1 | specular
+// << 31), (ng, g), (ns, s_prev), (power, s_next).
+__device__ __forceinline__ void bvert_store(float4* v, size_t fstride, const BVert& b) {
+  uint32_t const w = uint32_t(b.tri + 1) | (b.specular ? 0x80000000u : 0u);
+  v[0] = make_float4(b.pos.x, b.pos.y, b.pos.z, __uint_as_float(w));
+  v[fstride] = make_float4(b.ng.x, b.ng.y, b.ng.z, b.g);
+  v[2 * fstride] = make_float4(b.ns.x, b.ns.y, b.ns.z, b.s_prev);
+  v[3 * fstride] = make_float4(b.power.x, b.power.y, b.power.z, b.s_next);
+}
+__device__ __forceinline__ void bvert_load(const float4* v, size_t fstride, BVert& b) {
+  float4 const a = v[0], g = v[fstride], n = v[2 * fstride], p = v[3 * fstride];
+  uint32_t const w = __float_as_uint(a.w);
+  b.tri = int32_t(w & 0x7FFFFFFFu) - 1;
+  b.specular = (w >> 31) != 0;
+  b.pos = f3(a.x, a.y, a.z);
+  b.ng = f3(g.x, g.y, g.z);
+  b.g = g.w;
+  b.ns = f3(n.x, n.y, n.z);
+  b.s_prev = n.w;
+  b.power = f3(p.x, p.y, p.z);
+  b.s_next = p.w;
+}
+
+// The entries of a built subpath that a connection pass can read: up to and
+// including the first entry from `first` on past the path's end (tri < 0), at
+// most kmax + 1.  The light path's vertices start at entry 1, the eye path's at
+// entry 2 (entry 1 is the camera, which has no triangle).
+__device__ __forceinline__ uint32_t bdpt_path_len(const BVert* P, uint32_t kmax, uint32_t first) {
+  uint32_t k = first;
+  while (k < kmax + 1 && P[k].tri >= 0) ++k;
+  return k;
 }
 
 }  // namespace xyzhip

@@ -2051,6 +2051,109 @@ __global__ void __launch_bounds__(kBlock)
   if (kCount && q) atomicAdd(queries, static_cast<unsigned long long>(q));
 }
 
+// Queued form of the same estimator, in three passes per chunk of items:
+//   1. k_bdpt_paths: a thread per item builds the two subpaths (per-thread
+//      queries), stores their vertices, and writes the visibility rays of all
+//      its connections to a queue (a count pass over the connections first,
+//      then one atomic per wave for the wave's entries);
+//   2. the cooperative closest-hit kernel over the queue;
+//   3. k_bdpt_connect: a thread per item reloads its subpaths and evaluates
+//      its connections in the reference's order, reading the queue's hits in
+//      the order the rays were written.
+// The connections' queries are the bulk of the estimator's work; traced by one
+// thread each they diverge with the scene's size.  xyz_render_bdpt takes the
+// queued form for scenes of at least XYZ_BDPT_QUEUED_MIN_REFS kd leaf
+// references (measured crossover between 1.4e5 and 2.2e5 on the batch
+// workload's scenes at 256x256, 16 spp; on the 36-triangle Cornell box the
+// per-thread form is 1.4x faster, on the 2M-triangle production scene the
+// queued form 1.7x), unless the caller names the form (params->engine).
+#ifndef XYZ_BDPT_QUEUED_MIN_REFS
+#define XYZ_BDPT_QUEUED_MIN_REFS 200000
+#endif
+
+// Most connection queries of one sample: kmax - 2 for light tracing and
+// sum over t = 2 .. kmax - 1 of kmax - t for the connections.
+__host__ __device__ __forceinline__ uint32_t bdpt_max_queries(uint32_t kmax) {
+  return kmax < 2 ? 0u : (kmax - 2) * (kmax + 1) / 2;
+}
+
+template <bool kCount>
+__global__ void __launch_bounds__(kBlock)
+#if XYZ_BDPT_WAVES_PER_EU > 0
+    __attribute__((amdgpu_waves_per_eu(XYZ_BDPT_WAVES_PER_EU)))
+#endif
+    k_bdpt_paths(DevScene s, DevCamera cam, BdptConst bc, float4* __restrict__ verts,
+                 uint32_t* __restrict__ lens, uint32_t* __restrict__ qoff, float* __restrict__ qorg,
+                 float* __restrict__ qdir, uint32_t* __restrict__ qcount, uint32_t n,
+                 uint32_t sample0, uint32_t items, unsigned long long* queries) {
+  extern __shared__ uint32_t lds_stack[];
+  uint32_t* const stk = lds_stack + threadIdx.x;
+  uint32_t const i = blockIdx.x * kBlock + threadIdx.x;
+  bool const act = i < items;
+  uint32_t const kmax = bc.c.kmax;
+  uint32_t nq = 0, q = 0;
+  BVert L[kBdptMaxDepth + 3], E[kBdptMaxDepth + 3];
+  if (act) {
+    uint32_t const smp = sample0 + i / n;
+    uint32_t const p = i % n;
+    bdpt_build<kCount>(s, cam, bc, p % cam.width, p / cam.width, smp, stk, kBlock, L, E, &q);
+    BdptCountQueries c{0};
+    (void)bdpt_connect(s, cam, bc, L, E, nullptr, c);
+    nq = c.n;
+  }
+  // this item's queue entries: consecutive, after the wave's earlier items
+  uint32_t excl, total;
+  wave_prefix(nq, &excl, &total);
+  uint32_t base = 0;
+  if (__lane_id() == 0 && total > 0) base = atomicAdd(qcount, total);
+  base = uint32_t(__builtin_amdgcn_readfirstlane(base));
+  if (act) {
+    uint32_t const off = base + excl;
+    qoff[i] = off;
+    size_t const fs = items;  // distance between an entry's fields
+    uint32_t const entries = kmax + 2;
+    uint32_t const nl = bdpt_path_len(L, kmax, 1), ne = bdpt_path_len(E, kmax, 2);
+    lens[i] = nl | (ne << 16);
+    for (uint32_t k = 0; k <= nl; ++k) bvert_store(verts + size_t(k) * 4 * fs + i, fs, L[k]);
+    for (uint32_t k = 0; k <= ne; ++k)
+      bvert_store(verts + size_t(entries + k) * 4 * fs + i, fs, E[k]);
+    BdptRecordQueries r{qorg, qdir, off};
+    (void)bdpt_connect(s, cam, bc, L, E, nullptr, r);
+  }
+  if (kCount) {
+    unsigned long long v = q + nq;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+    if (__lane_id() == 0 && v) atomicAdd(queries, v);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+#if XYZ_BDPT_WAVES_PER_EU > 0
+    __attribute__((amdgpu_waves_per_eu(XYZ_BDPT_WAVES_PER_EU)))
+#endif
+    k_bdpt_connect(DevScene s, DevCamera cam, BdptConst bc, const float4* __restrict__ verts,
+                   const uint32_t* __restrict__ lens, const uint32_t* __restrict__ qoff,
+                   const int32_t* __restrict__ qprim, const float* __restrict__ qt,
+                   float4* __restrict__ colours, float* splat, uint32_t items) {
+  uint32_t const i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= items) return;
+  uint32_t const kmax = bc.c.kmax;
+  BVert L[kBdptMaxDepth + 3], E[kBdptMaxDepth + 3];
+  for (uint32_t k = 0; k < kmax + 2; ++k) {
+    bvert_clear(L[k]);
+    bvert_clear(E[k]);
+  }
+  size_t const fs = items;
+  uint32_t const entries = kmax + 2;
+  uint32_t const ln = lens[i];
+  uint32_t const nl = ln & 0xFFFFu, ne = ln >> 16;
+  for (uint32_t k = 0; k <= nl; ++k) bvert_load(verts + size_t(k) * 4 * fs + i, fs, L[k]);
+  for (uint32_t k = 0; k <= ne; ++k) bvert_load(verts + size_t(entries + k) * 4 * fs + i, fs, E[k]);
+  BdptReplayQueries r{qprim, qt, qoff[i]};
+  float3 const c = bdpt_connect(s, cam, bc, L, E, splat, r);
+  colours[i] = make_float4(c.x, c.y, c.z, 0.0f);
+}
+
 __global__ void __launch_bounds__(kBlock) k_bdpt_sum(float* film, const float4* colours,
                                                      uint32_t n, uint32_t count) {
   uint32_t const p = blockIdx.x * kBlock + threadIdx.x;
@@ -2076,6 +2179,9 @@ __global__ void __launch_bounds__(kBlock) k_add_into(float* dst, const float* sr
 __global__ void k_zero_u64(unsigned long long* p) {
   if (threadIdx.x == 0) *p = 0ull;
 }
+__global__ void k_zero_u32x(uint32_t* p) {
+  if (threadIdx.x == 0) *p = 0u;
+}
 
 int render_bdpt_impl(xyz_scene* sc, const xyz_camera* cam, const xyz_render_params* prm,
                      float* film, float* splat, xyz_render_stats* stats, hipStream_t st) {
@@ -2093,6 +2199,12 @@ int render_bdpt_impl(xyz_scene* sc, const xyz_camera* cam, const xyz_render_para
     set_error("render_bdpt: max_depth must be <= 16");
     return XYZ_E_UNSUPPORTED;
   }
+  if (prm->engine > 2) {
+    set_error("render_bdpt: engine must be 0 (by scene size), 1 (per-thread) or 2 (queued)");
+    return XYZ_E_INVALID;
+  }
+  bool const queued =
+      prm->engine == 2 || (prm->engine == 0 && sc->info.n_leaf_refs >= XYZ_BDPT_QUEUED_MIN_REFS);
   uint64_t const wh = uint64_t(cam->width) * cam->height;
   if (wh >= (1ull << 31)) {
     set_error("render_bdpt: image too large");
@@ -2133,25 +2245,83 @@ int render_bdpt_impl(xyz_scene* sc, const xyz_camera* cam, const xyz_render_para
     (void)hipEventCreate(&e1);
     (void)hipEventRecord(e0, st);
   }
-  // samples in chunks of at most 2^24 (sample, pixel) items (256 MiB of colours)
+  // samples in chunks of at most 2^24 (sample, pixel) items (256 MiB of colours);
+  // the queued form also keeps each item's subpaths and its worst-case queue
+  // entries, within about 6 GiB
   uint32_t const n = uint32_t(wh);
-  uint32_t const per_chunk = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(prm->spp, (1ull << 24) / wh)));
+  uint64_t chunk_items = 1ull << 24;
+  uint32_t const kmax = bc.c.kmax;
+  uint32_t const wmax = bdpt_max_queries(kmax);
+  size_t const item_bytes = size_t(kmax + 2) * 2 * 4 * sizeof(float4) + 2 * sizeof(uint32_t) +
+                            size_t(wmax) * (6 * sizeof(float) + sizeof(int32_t) + sizeof(float));
+  if (queued) chunk_items = std::min<uint64_t>(chunk_items, (6ull << 30) / item_bytes);
+  uint32_t const per_chunk = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(prm->spp, chunk_items / wh)));
   DevMem colbuf;
   colbuf.st = st;
   e = pool_alloc(&colbuf.p, size_t(per_chunk) * wh * sizeof(float4), st);
   if (e != hipSuccess) return hip_fail(e, "render_bdpt: colour buffer");
   float4* const cols = static_cast<float4*>(colbuf.p);
   size_t const lds = size_t(sc->dev.stack_cap) * kBlock * sizeof(uint32_t);
+  size_t const cap = queued ? size_t(per_chunk) * wh : 0;  // items per chunk
+  size_t const qcap = cap * wmax;                           // queue entries per chunk
+  DevMem vbuf, lbuf, obuf, qobuf, qdbuf, qpbuf, qtbuf, qcbuf;
+  vbuf.st = lbuf.st = obuf.st = qobuf.st = qdbuf.st = qpbuf.st = qtbuf.st = qcbuf.st = st;
+  if (queued) {
+    e = pool_alloc(&vbuf.p, cap * size_t(kmax + 2) * 2 * 4 * sizeof(float4), st);
+    if (e == hipSuccess) e = pool_alloc(&lbuf.p, cap * sizeof(uint32_t), st);
+    if (e == hipSuccess) e = pool_alloc(&obuf.p, cap * sizeof(uint32_t), st);
+    if (e == hipSuccess) e = pool_alloc(&qobuf.p, std::max<size_t>(1, qcap) * 3 * sizeof(float), st);
+    if (e == hipSuccess) e = pool_alloc(&qdbuf.p, std::max<size_t>(1, qcap) * 3 * sizeof(float), st);
+    if (e == hipSuccess) e = pool_alloc(&qpbuf.p, std::max<size_t>(1, qcap) * sizeof(int32_t), st);
+    if (e == hipSuccess) e = pool_alloc(&qtbuf.p, std::max<size_t>(1, qcap) * sizeof(float), st);
+    if (e == hipSuccess) e = pool_alloc(&qcbuf.p, sizeof(uint32_t), st);
+    if (e != hipSuccess) return hip_fail(e, "render_bdpt: path and query buffers");
+  }
+  float4* const verts = static_cast<float4*>(vbuf.p);
+  uint32_t* const lens = static_cast<uint32_t*>(lbuf.p);
+  uint32_t* const qoff = static_cast<uint32_t*>(obuf.p);
+  float* const qorg = static_cast<float*>(qobuf.p);
+  float* const qdir = static_cast<float*>(qdbuf.p);
+  int32_t* const qprim = static_cast<int32_t*>(qpbuf.p);
+  float* const qt = static_cast<float*>(qtbuf.p);
+  uint32_t* const qcount = static_cast<uint32_t*>(qcbuf.p);
   for (uint32_t done = 0; done < prm->spp; done += per_chunk) {
     uint32_t const count = std::min(per_chunk, prm->spp - done);
     uint64_t const items = uint64_t(count) * wh;
-    uint32_t const grid = uint32_t(std::min<uint64_t>((items + kBlock - 1) / kBlock, 1u << 20));
-    if (stats) {
-      hipLaunchKernelGGL(k_bdpt<true>, dim3(grid), dim3(kBlock), lds, st, sc->dev, dc, bc, cols, sp,
-                         n, prm->sample_offset + done, items, qd);
+    if (queued) {
+      uint32_t const grid = uint32_t((items + kBlock - 1) / kBlock);
+      hipLaunchKernelGGL(k_zero_u32x, dim3(1), dim3(64), 0, st, qcount);
+      if (stats) {
+        hipLaunchKernelGGL(k_bdpt_paths<true>, dim3(grid), dim3(kBlock), lds, st, sc->dev, dc, bc,
+                           verts, lens, qoff, qorg, qdir, qcount, n, prm->sample_offset + done,
+                           uint32_t(items), qd);
+      } else {
+        hipLaunchKernelGGL(k_bdpt_paths<false>, dim3(grid), dim3(kBlock), lds, st, sc->dev, dc, bc,
+                           verts, lens, qoff, qorg, qdir, qcount, n, prm->sample_offset + done,
+                           uint32_t(items), qd);
+      }
+      uint32_t nq = 0;
+      e = hipMemcpyAsync(&nq, qcount, sizeof(nq), hipMemcpyDeviceToHost, st);
+      if (e == hipSuccess) e = hipStreamSynchronize(st);
+      if (e != hipSuccess) return hip_fail(e, "render_bdpt: query count");
+      if (nq > qcap) {
+        set_error("render_bdpt: connection queue overflow");
+        return XYZ_E_DEVICE;
+      }
+      rc = launch_closest_hit(sc->dev, nq, qorg, qdir, qprim, qt, nullptr, nullptr, nullptr, nullptr,
+                              nullptr, st);
+      if (rc != XYZ_OK) return rc;
+      hipLaunchKernelGGL(k_bdpt_connect, dim3(grid), dim3(kBlock), 0, st, sc->dev, dc, bc, verts, lens,
+                         qoff, qprim, qt, cols, sp, uint32_t(items));
     } else {
-      hipLaunchKernelGGL(k_bdpt<false>, dim3(grid), dim3(kBlock), lds, st, sc->dev, dc, bc, cols,
-                         sp, n, prm->sample_offset + done, items, qd);
+      uint32_t const grid = uint32_t(std::min<uint64_t>((items + kBlock - 1) / kBlock, 1u << 20));
+      if (stats) {
+        hipLaunchKernelGGL(k_bdpt<true>, dim3(grid), dim3(kBlock), lds, st, sc->dev, dc, bc, cols, sp,
+                           n, prm->sample_offset + done, items, qd);
+      } else {
+        hipLaunchKernelGGL(k_bdpt<false>, dim3(grid), dim3(kBlock), lds, st, sc->dev, dc, bc, cols,
+                           sp, n, prm->sample_offset + done, items, qd);
+      }
     }
     hipLaunchKernelGGL(k_bdpt_sum, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, film, cols,
                        n, count);
