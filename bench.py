@@ -339,7 +339,34 @@ def bench_bdpt(args, hip, capi, scenes):
                                    single_thread=dict(value=round(W * H * 2 / c1["seconds"] / 1e6, 4),
                                                       cores=1, sample="2 spp"))
         out["speedup_vs_cpu"] = round(gpu / cpu, 1)
+    out["production"] = bench_bdpt_production(hip, capi, scenes)
     log(f"[bench] bdpt: {out}")
+    return out
+
+
+def bench_bdpt_production(hip, capi, scenes):
+    """BDPT on the ~2M-triangle production scene at 480x270, depth 8, 8 spp:
+    the queued form of the connections (which xyz_render_bdpt takes for a scene
+    this size) against each thread tracing its own queries; the own films of
+    the two forms are compared bit for bit."""
+    sc = scenes.production_scene()
+    g = capi.Scene.from_dict(sc)
+    W, H, D, spp = 480, 270, 8, 8
+    res = {}
+    films = {}
+    for name, engine in (("queued", 2), ("per_thread", 1)):
+        warm = hip.DeviceArray.zeros((H, W, 3), np.float32)
+        g.render_bdpt(sc["camera"], warm, 1, D, seed=1, engine=engine)
+        film = hip.DeviceArray.zeros((H, W, 3), np.float32)
+        splat = hip.DeviceArray.zeros((H, W, 3), np.float32)
+        st = g.render_bdpt(sc["camera"], film, spp, D, seed=1, splat=splat, engine=engine)
+        res[name] = round(W * H * spp / st["seconds"] / 1e6, 2)
+        films[name] = film.numpy()
+    out = dict(msamples_per_s=res["queued"], per_thread_msamples_per_s=res["per_thread"],
+               n_tris=int(g.info()["n_tris"]), width=W, height=H, spp=spp, max_depth=D,
+               default_engine="queued" if g.info()["n_leaf_refs"] >= 200000 else "per_thread",
+               own_films_identical=bool(np.array_equal(films["queued"].view(np.uint32),
+                                                       films["per_thread"].view(np.uint32))))
     return out
 
 
