@@ -606,12 +606,16 @@ struct BdptCountQueries {
 };
 
 // Writes a sample's visibility rays to consecutive queue entries from `pos`
-// (origins and directions as 3 floats each; every answer a miss).
+// (origins and directions as 3 floats each; every answer a miss); entries at
+// or past `cap` are not written (the caller sizes the queue for the most
+// queries a sample can make and checks the total).
 struct BdptRecordQueries {
   float* org;
   float* dir;
   uint32_t pos;
+  uint32_t cap;
   __device__ __forceinline__ HitRec find(float3 o, float3 d) {
+    if (pos >= cap) return bdpt_no_hit();
     size_t const k = 3 * size_t(pos++);
     org[k] = o.x;
     org[k + 1] = o.y;

@@ -2084,8 +2084,8 @@ __global__ void __launch_bounds__(kBlock)
 #endif
     k_bdpt_paths(DevScene s, DevCamera cam, BdptConst bc, float4* __restrict__ verts,
                  uint32_t* __restrict__ lens, uint32_t* __restrict__ qoff, float* __restrict__ qorg,
-                 float* __restrict__ qdir, uint32_t* __restrict__ qcount, uint32_t n,
-                 uint32_t sample0, uint32_t items, unsigned long long* queries) {
+                 float* __restrict__ qdir, uint32_t* __restrict__ qcount, uint32_t qcap,
+                 uint32_t n, uint32_t sample0, uint32_t items, unsigned long long* queries) {
   extern __shared__ uint32_t lds_stack[];
   uint32_t* const stk = lds_stack + threadIdx.x;
   uint32_t const i = blockIdx.x * kBlock + threadIdx.x;
@@ -2117,7 +2117,7 @@ __global__ void __launch_bounds__(kBlock)
     for (uint32_t k = 0; k <= nl; ++k) bvert_store(verts + size_t(k) * 4 * fs + i, fs, L[k]);
     for (uint32_t k = 0; k <= ne; ++k)
       bvert_store(verts + size_t(entries + k) * 4 * fs + i, fs, E[k]);
-    BdptRecordQueries r{qorg, qdir, off};
+    BdptRecordQueries r{qorg, qdir, off, qcap};
     (void)bdpt_connect(s, cam, bc, L, E, nullptr, r);
   }
   if (kCount) {
@@ -2256,6 +2256,10 @@ int render_bdpt_impl(xyz_scene* sc, const xyz_camera* cam, const xyz_render_para
                             size_t(wmax) * (6 * sizeof(float) + sizeof(int32_t) + sizeof(float));
   if (queued) chunk_items = std::min<uint64_t>(chunk_items, (6ull << 30) / item_bytes);
   uint32_t const per_chunk = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(prm->spp, chunk_items / wh)));
+  if (queued && uint64_t(per_chunk) * wh * wmax >= (1ull << 32)) {
+    set_error("render_bdpt: image too large for the queued connections");
+    return XYZ_E_UNSUPPORTED;
+  }
   DevMem colbuf;
   colbuf.st = st;
   e = pool_alloc(&colbuf.p, size_t(per_chunk) * wh * sizeof(float4), st);
@@ -2293,12 +2297,12 @@ int render_bdpt_impl(xyz_scene* sc, const xyz_camera* cam, const xyz_render_para
       hipLaunchKernelGGL(k_zero_u32x, dim3(1), dim3(64), 0, st, qcount);
       if (stats) {
         hipLaunchKernelGGL(k_bdpt_paths<true>, dim3(grid), dim3(kBlock), lds, st, sc->dev, dc, bc,
-                           verts, lens, qoff, qorg, qdir, qcount, n, prm->sample_offset + done,
-                           uint32_t(items), qd);
+                           verts, lens, qoff, qorg, qdir, qcount, uint32_t(qcap), n,
+                           prm->sample_offset + done, uint32_t(items), qd);
       } else {
         hipLaunchKernelGGL(k_bdpt_paths<false>, dim3(grid), dim3(kBlock), lds, st, sc->dev, dc, bc,
-                           verts, lens, qoff, qorg, qdir, qcount, n, prm->sample_offset + done,
-                           uint32_t(items), qd);
+                           verts, lens, qoff, qorg, qdir, qcount, uint32_t(qcap), n,
+                           prm->sample_offset + done, uint32_t(items), qd);
       }
       uint32_t nq = 0;
       e = hipMemcpyAsync(&nq, qcount, sizeof(nq), hipMemcpyDeviceToHost, st);
