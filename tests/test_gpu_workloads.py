@@ -5,7 +5,8 @@
   mirror, 10 % glass; four lights) at depth 16 with Russian roulette after
   depth 4 (xyz/core/scene_pathtracing.cpp:11-144 with Glass, glass.cpp:104-165,
   and Mirror, mirror.cpp:56-105): a band of 64 rows at 1 spp bit-identical to
-  the oracle; at the full frame, passes of 1 spp add up to one 4-spp render.
+  the oracle; at the full frame, passes of 1 spp add up to one 4-spp render,
+  and to one 33-spp render that spans two waves of path slots.
 * Jobs of the 64-scene batched workload (the smallest, a mid-size and the
   largest by triangle count), at their own resolution with 4 spp: both batched
   engines give the films of one xyz_render per job bit for bit
@@ -63,6 +64,25 @@ def test_production_frame_passes_add_up(capi_mod, device, production):
     acc = DeviceArray.zeros((H, W, 3), np.float32)
     for s in range(4):
         g.render(sc["camera"], acc, 1, D, rr_start=R, seed=7, sample_offset=s)
+    assert _same(one.numpy(), acc.numpy())
+
+
+def test_production_frame_across_two_waves(capi_mod, device, production):
+    """33 spp of the 1920x1080 frame are 68.4M work items, more than one wave of
+    2^26 path slots holds: the wave boundary falls inside sample 32, so some
+    pixels get that sample from the first wave and the rest from the second.
+    The film still equals 33 one-spp passes, bit for bit."""
+    from xyz_amd.hip import DeviceArray
+    sc = production
+    W, H, D, R, spp = 1920, 1080, 16, 4, 33
+    assert W * H * spp > 1 << 26
+    g = capi_mod.Scene.from_dict(sc)
+    one = DeviceArray.zeros((H, W, 3), np.float32)
+    st = g.render(sc["camera"], one, spp, D, rr_start=R, seed=3)
+    assert st["paths"] == W * H * spp
+    acc = DeviceArray.zeros((H, W, 3), np.float32)
+    for s in range(spp):
+        g.render(sc["camera"], acc, 1, D, rr_start=R, seed=3, sample_offset=s)
     assert _same(one.numpy(), acc.numpy())
 
 

@@ -32,8 +32,11 @@ namespace xyzhip {
 namespace {
 
 constexpr uint32_t kBlock = 256;
+// Path slots per wave: 2^26 (about 17 GB of wave buffers per internal stream);
+// the 1920x1080 256-spp frame measured 498-505 Msamples/s at 2^25, 516-532 at
+// 2^26 and 526-529 at 2^27.
 #ifndef XYZ_WAVE_CAP_LOG2
-#define XYZ_WAVE_CAP_LOG2 25
+#define XYZ_WAVE_CAP_LOG2 26
 #endif
 constexpr uint64_t kWaveCap = 1ull << XYZ_WAVE_CAP_LOG2;
 // Bytes of adjoint vertex tape per internal stream (the adjoint's waves are
@@ -1664,8 +1667,8 @@ std::map<std::string, GraphEntry> g_graphs;
 uint64_t g_graph_clock = 0;
 hipStream_t g_capture_stream = nullptr;
 // Cached graphs at most (a shared-wave entry holds its wave buffers, up to
-// ~17 GB for 2^25-slot waves on two lanes).
-constexpr size_t kMaxCachedGraphs = 4;
+// ~34 GB for 2^26-slot waves on two lanes).
+constexpr size_t kMaxCachedGraphs = 2;
 
 void free_graph_entry(GraphEntry& ge) {
   if (ge.done) (void)hipEventSynchronize(ge.done);  // its last replay has finished
