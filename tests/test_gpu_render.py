@@ -70,6 +70,44 @@ def test_progressive_passes_equal_one_pass(capi_mod, device):
     assert np.array_equal(a.numpy().view(np.uint32), b.numpy().view(np.uint32))
 
 
+def test_workspace_reused_across_streams(capi_mod, device):
+    """Wave buffers of 32 MiB and more (about 220 bytes per path slot) come from
+    blocks the library keeps for later calls.  Renders issued on two streams
+    with no synchronisation between them -- each reusing the block the one
+    before released on the other stream, one needing a larger block -- give
+    the films of the same renders made one at a time."""
+    import ctypes
+    from xyz_amd import hip
+    from xyz_amd.hip import DeviceArray
+    sc = S.cornell_box()
+    g = capi_mod.Scene.from_dict(sc)
+    cam = sc["camera"]
+    # (W, H, spp, seed): 0.5M, 0.4M, 1.3M and 33k path slots
+    jobs = [(256, 256, 8, 1), (192, 160, 12, 2), (320, 256, 16, 3), (128, 128, 2, 4)]
+    refs = [gpu_render(capi_mod, g, cam, W, H, spp, 8, rr_start=3, seed=sd)[0]
+            for W, H, spp, sd in jobs]
+    R = hip.rt()
+    streams = [ctypes.c_void_p(), ctypes.c_void_p()]
+    for s in streams:
+        assert R.hipStreamCreate(ctypes.byref(s)) == 0
+    try:
+        films = [DeviceArray.zeros((H, W, 3), np.float32) for W, H, _, _ in jobs]
+        hip.synchronize()
+        for k, ((W, H, spp, sd), film) in enumerate(zip(jobs, films)):
+            c = capi_mod.make_camera(cam, W, H)
+            p = capi_mod.make_params(spp, 8, 3, 0.2, sd, 0)
+            capi_mod.check(capi_mod.lib().xyz_render(g.h, ctypes.byref(c), ctypes.byref(p),
+                                                     ctypes.c_void_p(film.data_ptr()), None,
+                                                     streams[k % 2]))
+        for s in streams:
+            assert R.hipStreamSynchronize(s) == 0
+        for f, r in zip(films, refs):
+            _assert_same(f.numpy(), r)
+    finally:
+        for s in streams:
+            R.hipStreamDestroy(s)
+
+
 def test_textured_cornell_box_bit_exact(oracle_mod, capi_mod, device):
     sc = S.cornell_box(textured=True, tex_res=16)
     o, g = both(oracle_mod, capi_mod, sc)

@@ -1001,12 +1001,11 @@ int make_pt_const(const xyz_render_params* p, PTConst* c) {
   return XYZ_OK;
 }
 
+// Workspace of one call, from pool_alloc or big_alloc, released on `st`.
 struct DevMem {
   void* p = nullptr;
   hipStream_t st = nullptr;
-  ~DevMem() {
-    if (p) (void)hipFreeAsync(p, st);
-  }
+  ~DevMem() { big_free(p, st); }
 };
 
 size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
@@ -1056,10 +1055,10 @@ void carve_wave(void* mem, uint32_t cap, uint32_t tape_depth, WaveBufs* wb) {
   wb->tape_depth = tape_depth;
 }
 
-// One lane's wave buffers from the stream-ordered pool.
+// One lane's wave buffers (big_alloc).
 int alloc_wave(uint32_t cap, uint32_t tape_depth, hipStream_t st, DevMem* mem, WaveBufs* wb) {
-  hipError_t const e = pool_alloc(&mem->p, wave_bytes(cap, tape_depth), st);
-  if (e != hipSuccess) return hip_fail(e, "pool_alloc(wave)");
+  hipError_t const e = big_alloc(&mem->p, wave_bytes(cap, tape_depth), st);
+  if (e != hipSuccess) return hip_fail(e, "big_alloc(wave)");
   mem->st = st;
   carve_wave(mem->p, cap, tape_depth, wb);
   return XYZ_OK;
@@ -2265,33 +2264,41 @@ int render_bdpt_impl(xyz_scene* sc, const xyz_camera* cam, const xyz_render_para
   }
   DevMem colbuf;
   colbuf.st = st;
-  e = pool_alloc(&colbuf.p, size_t(per_chunk) * wh * sizeof(float4), st);
+  e = big_alloc(&colbuf.p, size_t(per_chunk) * wh * sizeof(float4), st);
   if (e != hipSuccess) return hip_fail(e, "render_bdpt: colour buffer");
   float4* const cols = static_cast<float4*>(colbuf.p);
   size_t const lds = size_t(sc->dev.stack_cap) * kBlock * sizeof(uint32_t);
   size_t const cap = queued ? size_t(per_chunk) * wh : 0;  // items per chunk
   size_t const qcap = cap * wmax;                           // queue entries per chunk
-  DevMem vbuf, lbuf, obuf, qobuf, qdbuf, qpbuf, qtbuf, qcbuf;
-  vbuf.st = lbuf.st = obuf.st = qobuf.st = qdbuf.st = qpbuf.st = qtbuf.st = qcbuf.st = st;
+  // the queued form's buffers, carved from one block: subpath vertices, path
+  // lengths, queue offsets, query origins, directions, hit ids and distances,
+  // and the queue counter
+  size_t const qn = std::max<size_t>(1, qcap);
+  size_t const qsizes[8] = {cap * size_t(kmax + 2) * 2 * 4 * sizeof(float4), cap * sizeof(uint32_t),
+                            cap * sizeof(uint32_t), qn * 3 * sizeof(float), qn * 3 * sizeof(float),
+                            qn * sizeof(int32_t), qn * sizeof(float), sizeof(uint32_t)};
+  char* qparts[8] = {};
+  DevMem qmem;
+  qmem.st = st;
   if (queued) {
-    e = pool_alloc(&vbuf.p, cap * size_t(kmax + 2) * 2 * 4 * sizeof(float4), st);
-    if (e == hipSuccess) e = pool_alloc(&lbuf.p, cap * sizeof(uint32_t), st);
-    if (e == hipSuccess) e = pool_alloc(&obuf.p, cap * sizeof(uint32_t), st);
-    if (e == hipSuccess) e = pool_alloc(&qobuf.p, std::max<size_t>(1, qcap) * 3 * sizeof(float), st);
-    if (e == hipSuccess) e = pool_alloc(&qdbuf.p, std::max<size_t>(1, qcap) * 3 * sizeof(float), st);
-    if (e == hipSuccess) e = pool_alloc(&qpbuf.p, std::max<size_t>(1, qcap) * sizeof(int32_t), st);
-    if (e == hipSuccess) e = pool_alloc(&qtbuf.p, std::max<size_t>(1, qcap) * sizeof(float), st);
-    if (e == hipSuccess) e = pool_alloc(&qcbuf.p, sizeof(uint32_t), st);
+    size_t total = 0;
+    for (size_t b : qsizes) total += align_up(b);
+    e = big_alloc(&qmem.p, total, st);
     if (e != hipSuccess) return hip_fail(e, "render_bdpt: path and query buffers");
+    char* at = static_cast<char*>(qmem.p);
+    for (int i = 0; i < 8; ++i) {
+      qparts[i] = at;
+      at += align_up(qsizes[i]);
+    }
   }
-  float4* const verts = static_cast<float4*>(vbuf.p);
-  uint32_t* const lens = static_cast<uint32_t*>(lbuf.p);
-  uint32_t* const qoff = static_cast<uint32_t*>(obuf.p);
-  float* const qorg = static_cast<float*>(qobuf.p);
-  float* const qdir = static_cast<float*>(qdbuf.p);
-  int32_t* const qprim = static_cast<int32_t*>(qpbuf.p);
-  float* const qt = static_cast<float*>(qtbuf.p);
-  uint32_t* const qcount = static_cast<uint32_t*>(qcbuf.p);
+  float4* const verts = reinterpret_cast<float4*>(qparts[0]);
+  uint32_t* const lens = reinterpret_cast<uint32_t*>(qparts[1]);
+  uint32_t* const qoff = reinterpret_cast<uint32_t*>(qparts[2]);
+  float* const qorg = reinterpret_cast<float*>(qparts[3]);
+  float* const qdir = reinterpret_cast<float*>(qparts[4]);
+  int32_t* const qprim = reinterpret_cast<int32_t*>(qparts[5]);
+  float* const qt = reinterpret_cast<float*>(qparts[6]);
+  uint32_t* const qcount = reinterpret_cast<uint32_t*>(qparts[7]);
   for (uint32_t done = 0; done < prm->spp; done += per_chunk) {
     uint32_t const count = std::min(per_chunk, prm->spp - done);
     uint64_t const items = uint64_t(count) * wh;

@@ -42,6 +42,14 @@ int hip_fail(hipError_t e, const char* what);
 // hipFreeAsync.
 hipError_t pool_alloc(void** p, size_t bytes, hipStream_t stream);
 
+// Large workspace: requests of 32 MiB and more outside stream capture come
+// from blocks the library keeps per device for reuse by later calls (grown on
+// demand, held until process exit); smaller ones, and any under capture, from
+// the pool.  Release with big_free on the stream that last used the memory
+// (also correct for pool memory).
+hipError_t big_alloc(void** p, size_t bytes, hipStream_t stream);
+void big_free(void* p, hipStream_t stream);
+
 int launch_closest_hit(const DevScene& s, uint32_t n, const float* org, const float* dir,
                        int32_t* prim, float* t, float* f, float* g, uint8_t* back,
                        uint32_t* counters, unsigned long long* totals, hipStream_t stream);

@@ -62,6 +62,115 @@ hipError_t pool_alloc(void** p, size_t bytes, hipStream_t stream) {
   return hipMallocFromPoolAsync(p, bytes, pool, stream);
 }
 
+namespace {
+// Blocks of large workspace (wave buffers, BDPT chunks) cached per device.
+// The stream-ordered pool hands a freed block out whole to any later request
+// it can hold, so renders whose workspace differs in size from call to call
+// (tens of GB at the largest) kept growing it: a second pass over the 64-job
+// render sequence left the GPU idle for two thirds of its time.  A block here
+// is held by one call at a time; its release is recorded as an event on the
+// caller's stream, which a later caller on another stream waits for.
+constexpr size_t kBigMin = size_t(32) << 20;
+struct BigBlock {
+  void* p = nullptr;
+  size_t bytes = 0;
+  hipEvent_t done = nullptr;  // recorded at the last release
+  hipStream_t last = nullptr;
+  bool held = false;
+};
+std::mutex g_big_mu;
+std::map<int, std::vector<BigBlock>> g_big;
+
+// bytes rounded up to a multiple of an eighth of the next power of two
+size_t big_round(size_t bytes) {
+  size_t pw = size_t(1) << 21;
+  while (pw < bytes) pw <<= 1;
+  size_t const step = pw >> 3;
+  return (bytes + step - 1) / step * step;
+}
+
+// Frees the idle blocks of `v` (waits for their last use).
+void big_trim(std::vector<BigBlock>& v) {
+  for (auto it = v.begin(); it != v.end();) {
+    if (it->held) {
+      ++it;
+      continue;
+    }
+    (void)hipEventSynchronize(it->done);
+    (void)hipFree(it->p);
+    (void)hipEventDestroy(it->done);
+    it = v.erase(it);
+  }
+}
+}  // namespace
+
+hipError_t big_alloc(void** p, size_t bytes, hipStream_t stream) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  hipError_t e = hipStreamIsCapturing(stream, &cs);
+  if (e != hipSuccess) return e;
+  if (bytes < kBigMin || cs != hipStreamCaptureStatusNone) return pool_alloc(p, bytes, stream);
+  int dev = 0;
+  e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> lock(g_big_mu);
+  std::vector<BigBlock>& v = g_big[dev];
+  BigBlock* best = nullptr;
+  for (BigBlock& b : v)
+    if (!b.held && b.bytes >= bytes && (!best || b.bytes < best->bytes)) best = &b;
+  if (best) {
+    if (best->last != stream) {
+      e = hipStreamWaitEvent(stream, best->done, 0);
+      if (e != hipSuccess) return e;
+    }
+    best->held = true;
+    *p = best->p;
+    return hipSuccess;
+  }
+  // every idle block is too small: free them, then take a new one
+  big_trim(v);
+  BigBlock b;
+  b.bytes = big_round(bytes);
+  e = hipMalloc(&b.p, b.bytes);
+  if (e == hipErrorOutOfMemory) {
+    // the pool's idle memory goes back to the driver, then one more try
+    (void)hipGetLastError();
+    (void)hipDeviceSynchronize();
+    {
+      std::lock_guard<std::mutex> pl(g_pool_mu);
+      auto it = g_pools.find(dev);
+      if (it != g_pools.end()) (void)hipMemPoolTrimTo(it->second, 0);
+    }
+    e = hipMalloc(&b.p, b.bytes);
+  }
+  if (e != hipSuccess) return e;
+  e = hipEventCreateWithFlags(&b.done, hipEventDisableTiming);
+  if (e != hipSuccess) {
+    (void)hipFree(b.p);
+    return e;
+  }
+  b.held = true;
+  b.last = stream;
+  v.push_back(b);
+  *p = b.p;
+  return hipSuccess;
+}
+
+void big_free(void* p, hipStream_t stream) {
+  if (!p) return;
+  {
+    std::lock_guard<std::mutex> lock(g_big_mu);
+    for (auto& kv : g_big)
+      for (BigBlock& b : kv.second)
+        if (b.p == p && b.held) {
+          (void)hipEventRecord(b.done, stream);
+          b.last = stream;
+          b.held = false;
+          return;
+        }
+  }
+  (void)hipFreeAsync(p, stream);
+}
+
 // ------------------------------------------------------------ closest hit
 constexpr uint32_t kHitBlock = 256;
 // Persistent workgroups per CU: 7 is what fits at once (67 VGPRs give 7 waves
