@@ -640,6 +640,13 @@ def bench_batch(args, hip, capi, scenes):
     large = [k for k in range(len(gs)) if job_samples[k] >= (1 << 26)]
     large_ms = sum(job_ms[k] for k in large)
     seq_films = [f.numpy() for f in films]
+    # the same sequence again: later passes must not be slower than the first
+    # (workspace reuse across calls)
+    e0.record(None)
+    for s, c, f, sp, sd in zip(gs, cams, films, spps, seeds):
+        s.render(c, f, sp, D, rr_start=3, seed=sd)
+    e1.record(None)
+    seq2_ms = e0.elapsed_ms(e1)
     # batched: one graph launch (first call captures; the timed call replays)
     for f in films:
         f.zero()
@@ -653,6 +660,7 @@ def bench_batch(args, hip, capi, scenes):
     e1.record(None)
     bat_ms = e0.elapsed_ms(e1)
     res = dict(batched_ms=round(bat_ms, 1), sequential_ms=round(seq_ms, 1),
+               sequential_second_pass_ms=round(seq2_ms, 1),
                batched_over_sequential=round(seq_ms / bat_ms, 3),
                batched_msamples_per_s=round(samples / (bat_ms * 1e-3) / 1e6, 1),
                sequential_msamples_per_s=round(samples / (seq_ms * 1e-3) / 1e6, 1),

@@ -70,13 +70,18 @@ def test_progressive_passes_equal_one_pass(capi_mod, device):
     assert np.array_equal(a.numpy().view(np.uint32), b.numpy().view(np.uint32))
 
 
-def test_workspace_reused_across_streams(capi_mod, device):
+@pytest.mark.parametrize("budget", [None, "0"])
+def test_workspace_reused_across_streams(capi_mod, device, monkeypatch, budget):
     """Wave buffers of 32 MiB and more (about 220 bytes per path slot) come from
     blocks the library keeps for later calls.  Renders issued on two streams
-    with no synchronisation between them -- each reusing the block the one
-    before released on the other stream, one needing a larger block -- give
-    the films of the same renders made one at a time."""
+    with no synchronisation between them give the films of the same renders
+    made one at a time: within the cache's budget each stream takes blocks of
+    its own; with a budget of 0 every render reuses the block the one before
+    released on the other stream (after an event wait), and the third, which
+    needs a larger block, frees the idle smaller ones first."""
     import ctypes
+    if budget is not None:
+        monkeypatch.setenv("XYZ_WORKSPACE_CACHE_BYTES", budget)
     from xyz_amd import hip
     from xyz_amd.hip import DeviceArray
     sc = S.cornell_box()

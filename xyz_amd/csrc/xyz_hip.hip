@@ -4,6 +4,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <map>
@@ -69,7 +71,11 @@ namespace {
 // (tens of GB at the largest) kept growing it: a second pass over the 64-job
 // render sequence left the GPU idle for two thirds of its time.  A block here
 // is held by one call at a time; its release is recorded as an event on the
-// caller's stream, which a later caller on another stream waits for.
+// caller's stream.  A request takes the smallest idle block that holds it and
+// was last used on the same stream (stream order makes it free); else a new
+// block while the cache stays within its budget, so calls on different
+// streams can run at once; else another stream's idle block, after waiting
+// for that block's event.
 constexpr size_t kBigMin = size_t(32) << 20;
 struct BigBlock {
   void* p = nullptr;
@@ -78,8 +84,13 @@ struct BigBlock {
   hipStream_t last = nullptr;
   bool held = false;
 };
+struct BigCache {
+  std::vector<BigBlock> blocks;
+  size_t bytes = 0;     // held and idle
+  size_t quarter = 0;   // a quarter of the device's memory
+};
 std::mutex g_big_mu;
-std::map<int, std::vector<BigBlock>> g_big;
+std::map<int, BigCache> g_big;
 
 // bytes rounded up to a multiple of an eighth of the next power of two
 size_t big_round(size_t bytes) {
@@ -89,17 +100,29 @@ size_t big_round(size_t bytes) {
   return (bytes + step - 1) / step * step;
 }
 
-// Frees the idle blocks of `v` (waits for their last use).
-void big_trim(std::vector<BigBlock>& v) {
-  for (auto it = v.begin(); it != v.end();) {
-    if (it->held) {
+// The cache's budget: XYZ_WORKSPACE_CACHE_BYTES when set, else a quarter of
+// the device's memory.
+size_t big_budget(BigCache& c) {
+  if (char const* v = std::getenv("XYZ_WORKSPACE_CACHE_BYTES")) return size_t(std::strtoull(v, nullptr, 10));
+  if (!c.quarter) {
+    size_t fr = 0, tot = 0;
+    c.quarter = hipMemGetInfo(&fr, &tot) == hipSuccess ? tot / 4 : size_t(16) << 30;
+  }
+  return c.quarter;
+}
+
+// Frees the idle blocks of `c` smaller than `below` (waits for their last use).
+void big_trim(BigCache& c, size_t below) {
+  for (auto it = c.blocks.begin(); it != c.blocks.end();) {
+    if (it->held || it->bytes >= below) {
       ++it;
       continue;
     }
     (void)hipEventSynchronize(it->done);
     (void)hipFree(it->p);
     (void)hipEventDestroy(it->done);
-    it = v.erase(it);
+    c.bytes -= it->bytes;
+    it = c.blocks.erase(it);
   }
 }
 }  // namespace
@@ -113,27 +136,34 @@ hipError_t big_alloc(void** p, size_t bytes, hipStream_t stream) {
   e = hipGetDevice(&dev);
   if (e != hipSuccess) return e;
   std::lock_guard<std::mutex> lock(g_big_mu);
-  std::vector<BigBlock>& v = g_big[dev];
-  BigBlock* best = nullptr;
-  for (BigBlock& b : v)
-    if (!b.held && b.bytes >= bytes && (!best || b.bytes < best->bytes)) best = &b;
-  if (best) {
-    if (best->last != stream) {
-      e = hipStreamWaitEvent(stream, best->done, 0);
+  BigCache& c = g_big[dev];
+  BigBlock *same = nullptr, *other = nullptr;
+  for (BigBlock& b : c.blocks) {
+    if (b.held || b.bytes < bytes) continue;
+    BigBlock*& slot = b.last == stream ? same : other;
+    if (!slot || b.bytes < slot->bytes) slot = &b;
+  }
+  size_t const want = big_round(bytes);
+  bool const within = c.bytes + want <= big_budget(c);
+  BigBlock* const take = same ? same : (!within ? other : nullptr);
+  if (take) {
+    if (take != same) {
+      e = hipStreamWaitEvent(stream, take->done, 0);
       if (e != hipSuccess) return e;
     }
-    best->held = true;
-    *p = best->p;
+    take->held = true;
+    *p = take->p;
     return hipSuccess;
   }
-  // every idle block is too small: free them, then take a new one
-  big_trim(v);
+  if (!within) big_trim(c, bytes);
   BigBlock b;
-  b.bytes = big_round(bytes);
+  b.bytes = want;
   e = hipMalloc(&b.p, b.bytes);
   if (e == hipErrorOutOfMemory) {
-    // the pool's idle memory goes back to the driver, then one more try
+    // every idle block and the pool's idle memory go back to the driver, then
+    // one more try
     (void)hipGetLastError();
+    big_trim(c, SIZE_MAX);
     (void)hipDeviceSynchronize();
     {
       std::lock_guard<std::mutex> pl(g_pool_mu);
@@ -150,7 +180,8 @@ hipError_t big_alloc(void** p, size_t bytes, hipStream_t stream) {
   }
   b.held = true;
   b.last = stream;
-  v.push_back(b);
+  c.blocks.push_back(b);
+  c.bytes += b.bytes;
   *p = b.p;
   return hipSuccess;
 }
@@ -160,7 +191,7 @@ void big_free(void* p, hipStream_t stream) {
   {
     std::lock_guard<std::mutex> lock(g_big_mu);
     for (auto& kv : g_big)
-      for (BigBlock& b : kv.second)
+      for (BigBlock& b : kv.second.blocks)
         if (b.p == p && b.held) {
           (void)hipEventRecord(b.done, stream);
           b.last = stream;
