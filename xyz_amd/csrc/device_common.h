@@ -82,9 +82,9 @@ struct DevScene {
   const float4* prim_s1;
   const float4* prim_s2;
   const float2* prim_uv;    // NULL or 3 per primitive
-  const float4* prim_v0;    // v0, e1, e2 per primitive (for light sampling)
-  const float4* prim_e1;
-  const float4* prim_e2;
+  const float4* prim_v0;    // per primitive: the triangle record's A, B, C
+  const float4* prim_e1;    // ((v0, n.x), (e1, n.y), (e2, n.z)), for light
+  const float4* prim_e2;    // sampling and shadow queries
   // lights (caller order of LIGHT triangles)
   uint32_t n_lights;
   const float* light_cdf;   // n_lights + 1
@@ -161,8 +161,8 @@ __device__ __forceinline__ uint32_t far_child(uint2 w, float3 o, uint32_t node) 
 //
 // Stack: one 4-byte entry per pending far child, holding the interior node whose
 // far child is pending.  Only pushes change t_max before a hit ends the query,
-// so t_max always equals the split distance e of the top entry (+inf when the
-// stack is empty).  A pop therefore sets t_min = t_max (the popped entry's e)
+// so t_max always equals the split distance e of the top entry (t_cap, the
+// root interval's t_max, when the stack is empty: +inf for a plain query).  A pop therefore sets t_min = t_max (the popped entry's e)
 // and recomputes t_max from the new top; both recomputations repeat the same
 // IEEE division and give the same bits.  The stack lives in LDS, interleaved by
 // lane (`stride` words apart).
@@ -179,6 +179,7 @@ constexpr uint32_t kLeafChunk = XYZ_LEAF_CHUNK;
 struct TraceState {
   float3 o, d;
   float t_min, t_max;
+  float t_cap;              // t_max of the root interval (+inf but for capped shadow queries)
   uint32_t sp, node;
   uint32_t lj, lend;        // cooperative: next / end cluster; sequential: next rank / count
   uint32_t lc0, ln;         // current leaf: first cluster, triangle count
@@ -190,11 +191,14 @@ struct TraceState {
   uint32_t xt, xb;  // counting mode: triangle tests and cluster box tests executed
 };
 
+constexpr float kTMin = 1e-5f;  // Intersection::Init, xyz/geom/intersection.hpp:11-14
+
 __device__ __forceinline__ void trace_init(TraceState& st, float3 o, float3 d) {
   st.o = o;
   st.d = d;
-  st.t_min = 1e-5f;  // Intersection::Init, xyz/geom/intersection.hpp:11-14
+  st.t_min = kTMin;
   st.t_max = __builtin_inff();
+  st.t_cap = __builtin_inff();
   st.sp = 0;
   st.node = 0;
   st.lj = st.lend = 0;
@@ -323,7 +327,7 @@ __device__ __forceinline__ bool trace_step(const DevScene& s, TraceState& st, ui
     uint32_t f2, n2;
     st.t_max = split_distance(gp(s.nodes)[pb], st.o, st.d, pb, &f2, &n2);
   } else {
-    st.t_max = __builtin_inff();
+    st.t_max = st.t_cap;
   }
   return false;
 }
@@ -530,7 +534,7 @@ __device__ __forceinline__ void leaf_sequential(const DevScene& s, TraceState& s
 // t_max of the current interval: the split distance of the top stack entry.
 __device__ __forceinline__ float interval_t_max(const DevScene& s, const TraceState& st,
                                                 const uint32_t* stk, uint32_t stride) {
-  if (st.sp == 0) return __builtin_inff();
+  if (st.sp == 0) return st.t_cap;
   uint32_t const pb = stk[(st.sp - 1) * stride];
   uint32_t f2, n2;
   return split_distance(rec_at(s.nodes, pb), st.o, st.d, pb, &f2, &n2);

@@ -411,6 +411,7 @@ struct ShadowReq {
   float fsn, fres, mx;  // METAL only
   float gd;          // LAMBERT: the importance histogram's density of w (1 when unguided)
   uint32_t vj;       // index of the shading vertex on its path (0 = first hit)
+  float t_target;    // distance of the shading point's triangle along w (+inf if NaN)
 };
 
 // Contribution of a visible NEE sample (xyz/bsdf/lambert.cpp:157-172,
@@ -828,6 +829,16 @@ __device__ __forceinline__ bool shade_end(const DevScene& s, const PTConst& c, P
       if (ok) {
         nk2 = -dot3(w, B.ns);
         ok = !((nk2 <= 0.0f) || (-dot3(w, B.ng) <= 0.0f));
+      }
+      if (ok) {
+        // The shadow query (a closest hit from lv.pos along w) can find the
+        // shading point's triangle only at that triangle's own tri_test
+        // distance T: where tri_test rejects it over [kTMin, +inf] no query
+        // finds it and the sample adds nothing, so no query is made.
+        float tf, tg, tdet;
+        ok = tri_test(lv.pos, w, gp(s.prim_v0)[h.prim], gp(s.prim_e1)[h.prim], gp(s.prim_e2)[h.prim], kTMin,
+                      __builtin_inff(), &sr->t_target, &tf, &tg, &tdet);
+        if (!(sr->t_target == sr->t_target)) sr->t_target = __builtin_inff();
       }
       if (ok) {
         sr->o = lv.pos;

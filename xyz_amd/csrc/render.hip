@@ -118,7 +118,7 @@ struct WaveBufs {
   float4* so;     // shadow: origin.xyz, nk2
   float4* sw;     // shadow: w.xyz, nk3
   float4* spe;    // shadow: pe.xyz, bits(slot)
-  uint4* smeta;   // shadow: expected prim, type, light bsdf, 0
+  uint4* smeta;   // shadow: expected prim, type | vertex << 8, light bsdf, t_target bits
   float4* smet;   // shadow: fsn, fres, mx, histogram density
   uint32_t* qa;   // 8 * cap
   uint32_t* qb;   // 8 * cap
@@ -686,7 +686,8 @@ k_shade(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBuf
       wb.so[rq] = make_float4(sr.o.x, sr.o.y, sr.o.z, sr.nk2);
       wb.sw[rq] = make_float4(sr.w.x, sr.w.y, sr.w.z, sr.nk3);
       wb.spe[rq] = make_float4(sr.pe.x, sr.pe.y, sr.pe.z, __uint_as_float(slot));
-      wb.smeta[rq] = make_uint4(uint32_t(sr.expected), sr.type, sr.light_bsdf, sr.vj);
+      wb.smeta[rq] = make_uint4(uint32_t(sr.expected), sr.type | (sr.vj << 8), sr.light_bsdf,
+                                __float_as_uint(sr.t_target));
       wb.smet[rq] = make_float4(sr.fsn, sr.fres, sr.mx, sr.gd);
     }
   }
@@ -744,6 +745,17 @@ k_shadow(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBu
     if (r != 0xFFFFFFFFu) {
       rq = wj.start[lseg] + r;
       trace_init(st, ld3(wb.so[rq]), ld3(wb.sw[rq]));
+      // The query only asks whether its closest hit is the shading point's
+      // triangle, which it can find only at that triangle's distance T (the
+      // request's t_target; tri_test on the same record values gives the same
+      // bits).  With the root interval [t_min, T] in place of [t_min, +inf]
+      // the traversal visits the same leaves up to the first whose interval
+      // reaches past T, each interval cut at T.  A result of the full query at
+      // or before T is the least key of a subset that still holds it, so it is
+      // kept; a result past T comes from a leaf whose interval reaches past T,
+      // and no later leaf can accept T.  So the capped query finds the
+      // triangle exactly when the full query does, at t = T.
+      st.t_cap = st.t_max = __uint_as_float(wb.smeta[rq].w);
     }
     if (__ballot(rq != 0xFFFFFFFFu) == 0) break;
     bool const act = rq != 0xFFFFFFFFu;
@@ -758,7 +770,7 @@ k_shadow(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBu
     float4 const o = wb.so[rr], w = wb.sw[rr];
     float4 const pe = wb.spe[rr], met = wb.smet[rr];
     ShadowReq sr;
-    sr.type = meta.y;
+    sr.type = meta.y & 0xFFu;
     sr.nk2 = o.w;
     sr.nk3 = w.w;
     sr.pe = ld3(pe);
@@ -775,7 +787,7 @@ k_shadow(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBu
       RJob const& J = jobs[wj.jlo + job];
       unsigned long long const item = wj.q0 + slot - J.base;
       uint32_t const pixel = uint32_t(item % J.wh);
-      uint32_t const vj = meta.w;
+      uint32_t const vj = meta.y >> 8;
       if (vj < wb.tape_depth) {
         size_t const ti = size_t(vj) * wb.cap + slot;
         float4 const ta = wb.tape_a[ti], tb = wb.tape_b[ti];

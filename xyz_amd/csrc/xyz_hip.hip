@@ -615,9 +615,10 @@ int scene_create(const xyz_scene_desc* desc, xyz_scene** out) {
     float bsf;
     std::memcpy(&bsf, &bs, 4);
     png[i] = make_float4(ng[3 * i], ng[3 * i + 1], ng[3 * i + 2], bsf);
-    pv0[i] = make_float4(f3x(desc->v0, i, 0), f3x(desc->v0, i, 1), f3x(desc->v0, i, 2), 0.0f);
-    pe1[i] = make_float4(e1[3 * i], e1[3 * i + 1], e1[3 * i + 2], 0.0f);
-    pe2[i] = make_float4(e2[3 * i], e2[3 * i + 1], e2[3 * i + 2], 0.0f);
+    // the triangle record's A, B, C (n = cross(e2, e1) in w)
+    pv0[i] = make_float4(f3x(desc->v0, i, 0), f3x(desc->v0, i, 1), f3x(desc->v0, i, 2), nrm[3 * i]);
+    pe1[i] = make_float4(e1[3 * i], e1[3 * i + 1], e1[3 * i + 2], nrm[3 * i + 1]);
+    pe2[i] = make_float4(e2[3 * i], e2[3 * i + 1], e2[3 * i + 2], nrm[3 * i + 2]);
   }
   std::vector<float4> pn0, ps1, ps2;
   if (desc->n0 && desc->n1 && desc->n2) {
