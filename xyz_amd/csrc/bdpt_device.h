@@ -355,7 +355,7 @@ __device__ __forceinline__ void bdpt_build(const DevScene& s, const DevCamera& c
   }
   auto find = [&](float3 o, float3 d) {
     if (kCount) ++*queries;
-    return trace_closest<false>(s, o, d, stk, stride, nullptr, nullptr);
+    return trace_lane(s, o, d, stk, stride);
   };
   float4 const de = draw4(bc.c, pixel, smp, 0u, kBdptStreamEye);
   float4 const dl = draw4(bc.c, pixel, smp, 0u, kBdptStreamLight);
@@ -583,7 +583,7 @@ struct BdptTrace {
   uint32_t* queries;
   __device__ __forceinline__ HitRec find(float3 o, float3 d) {
     if (kCount) ++*queries;
-    return trace_closest<false>(*s, o, d, stk, stride, nullptr, nullptr);
+    return trace_lane(*s, o, d, stk, stride);
   }
 };
 

@@ -769,6 +769,100 @@ __device__ __forceinline__ HitRec trace_closest(const DevScene& s, float3 o, flo
   return trace_result(s, st);
 }
 
+// Closest hit of one lane's ray, for kernels whose lanes trace on their own
+// (divergent callers such as subpath construction).  Descent and pops are
+// trace_step's; a leaf is evaluated like coop_step's, by one lane: the least
+// key (bits(t) << 32) | ~rank over the triangles of the clusters that
+// cluster_may_hit keeps, which is the reference's sequential result, with the
+// bound tightened to the best t found so far (a later triangle at the same t
+// still wins on its rank).  A leaf with NaN bounds or a NaN t is run
+// sequentially in the reference's order.
+__device__ __forceinline__ HitRec trace_closest_culled(const DevScene& s, float3 o, float3 d,
+                                                       uint32_t* stk, uint32_t stride) {
+  TraceState st;
+  trace_init(st, o, d);
+  float dlen;
+  float3 inv;
+  cull_ray(d, &dlen, &inv);
+  for (;;) {
+    // descend to a leaf: xyz/geom/kdtree.cpp:82-108
+    uint2 w = rec_at(s.nodes, st.node);
+    while ((w.x & 3u) != 3u) {
+      uint32_t farc, nearc;
+      float const e = split_distance(w, st.o, st.d, st.node, &farc, &nearc);
+      if ((e > st.t_max) || (e < 0.0f)) {
+        st.node = nearc;
+      } else if (e < st.t_min) {
+        st.node = farc;
+      } else {
+        stk[st.sp * stride] = st.node;
+        st.sp++;
+        st.t_max = e;
+        st.node = nearc;
+      }
+      w = rec_at(s.nodes, st.node);
+    }
+    st.lc0 = w.y;
+    st.ln = w.x >> 2;
+    st.lend = st.lc0 + ((st.ln + kClusterSize - 1) >> kClusterShift);
+    st.hit = -1;
+    bool seq = (st.t_min != st.t_min) || (st.t_max != st.t_max);
+    if (!seq) {
+      unsigned long long best = ~0ull;
+      float tmax = st.t_max;
+      for (uint32_t ci = st.lc0; ci < st.lend && !seq; ++ci) {
+        if (!cluster_may_hit(rec_at(s.clo, ci), rec_at(s.chi, ci), st.o, dlen, inv, st.t_min, tmax)) continue;
+        uint32_t const n = (ci + 1u == st.lend) ? st.ln - ((ci - st.lc0) << kClusterShift) : kClusterSize;
+        for (uint32_t m = 0; m < n; ++m) {
+          uint32_t const j = (ci << kClusterShift) + m;
+          float t, f, g, det;
+          bool const ok = tri_test(st.o, st.d, rec_at(s.triA, j), rec_at(s.triB, j), rec_at(s.triC, j), st.t_min,
+                                   tmax, &t, &f, &g, &det);
+          if (t != t && !(fabsf(det) < 1e-6f)) {
+            seq = true;
+            break;
+          }
+          if (ok) {
+            unsigned long long const key = (static_cast<unsigned long long>(__float_as_uint(t)) << 32) |
+                                           (0xFFFFFFFFu - rec_at(s.rec_rank, j));
+            if (key < best) {
+              best = key;
+              tmax = t;
+              st.hit = int32_t(j);
+              st.f = f;
+              st.g = g;
+              st.back = det > 0.0f;
+            }
+          }
+        }
+      }
+      if (!seq && st.hit >= 0) st.t_max = tmax;
+    }
+    if (seq) {
+      st.hit = -1;
+      leaf_sequential(s, st);
+    }
+    if (st.hit >= 0 || st.sp == 0) break;
+    // pop: the far child of the top entry, with [e, e_below]
+    st.sp--;
+    uint32_t const p = stk[st.sp * stride];
+    st.node = far_child(rec_at(s.nodes, p), st.o, p);
+    st.t_min = st.t_max;
+    st.t_max = interval_t_max(s, st, stk, stride);
+  }
+  return trace_result(s, st);
+}
+
+// The closest hit a lane traces on its own.
+#ifndef XYZ_LANE_CULL
+#define XYZ_LANE_CULL 1
+#endif
+__device__ __forceinline__ HitRec trace_lane(const DevScene& s, float3 o, float3 d, uint32_t* stk,
+                                             uint32_t stride) {
+  if (XYZ_LANE_CULL) return trace_closest_culled(s, o, d, stk, stride);
+  return trace_closest<false>(s, o, d, stk, stride, nullptr, nullptr);
+}
+
 #ifndef XYZ_COOP
 #define XYZ_COOP 1
 #endif

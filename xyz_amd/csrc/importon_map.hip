@@ -133,7 +133,7 @@ __global__ void __launch_bounds__(kEmitBlock)
       float3 pos = lv.pos;
       bool specular = false;
       for (uint32_t depth = 0; depth < c.kmax; ++depth) {
-        HitRec const h = trace_closest<false>(s, pos, dir, stk, blockDim.x, nullptr, nullptr);
+        HitRec const h = trace_lane(s, pos, dir, stk, blockDim.x);
         if (h.prim < 0) break;
         uint32_t bsdf_id;
         Basis B = shading_basis(s, h.prim, h.f, h.g, h.back, &bsdf_id);
