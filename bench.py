@@ -364,7 +364,7 @@ def bench_bdpt_production(hip, capi, scenes):
         films[name] = film.numpy()
     out = dict(msamples_per_s=res["queued"], per_thread_msamples_per_s=res["per_thread"],
                n_tris=int(g.info()["n_tris"]), width=W, height=H, spp=spp, max_depth=D,
-               default_engine="queued" if g.info()["n_leaf_refs"] >= 200000 else "per_thread",
+               default_engine="queued" if g.info()["n_leaf_refs"] >= 300000 else "per_thread",
                own_films_identical=bool(np.array_equal(films["queued"].view(np.uint32),
                                                        films["per_thread"].view(np.uint32))))
     return out

@@ -243,8 +243,9 @@ typedef struct {
   uint32_t rr_start;
   float rr_threshold;
   uint32_t engine;        /* xyz_render_bdpt only: 0 = by the scene's size, 1 = a thread
-                             per sample traces its queries, 2 = the connections' queries
-                             queued and traced together; the other renderers ignore it */
+                             per sample traces its queries, 2 = the lanes of a wave trace
+                             their subpaths' rays together and the connections' queries
+                             are queued and traced together; the other renderers ignore it */
   uint64_t seed;
 } xyz_render_params;
 

@@ -89,13 +89,13 @@ def test_mirror_wall_matches_oracle(oracle_mod, capi_mod, device, camera, engine
 
 def test_large_scene_takes_the_queued_form(oracle_mod, capi_mod, device):
     """A 221,114-triangle scene of the batch workload: engine 0 takes the
-    queued form (at least 200,000 kd leaf references) and equals engine 2 bit
+    queued form (at least 300,000 kd leaf references) and equals engine 2 bit
     for bit; a band of rows equals the oracle, and the per-thread form gives
     the same own film."""
     sc = S.batch_scene(39)
     assert len(sc["v0"]) == 221114
     o, g = scenes_of(oracle_mod, capi_mod, sc)
-    assert g.info()["n_leaf_refs"] >= 200000
+    assert g.info()["n_leaf_refs"] >= 300000
     W, H, spp, D = 48, 32, 4, 8
     print(f"[gpu bdpt] batch scene 39 {W}x{H} {spp} spp D {D}", flush=True)
     own0, spl0, st0 = gpu_bdpt(g, sc["camera"], W, H, spp, D, seed=3, engine=0)

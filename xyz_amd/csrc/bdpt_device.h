@@ -337,15 +337,25 @@ __device__ __forceinline__ float mis_eye_side(const BVert* E, uint32_t t, float 
   return wst;
 }
 
-// The two subpaths of one BDPT sample of pixel (px, py): the set-up of
-// Scene::EvaluateBidirectionalPathTracing, BuildLightsPath and BuildTheEyePath
+// The answer of a query that finds nothing.
+__device__ __forceinline__ HitRec bdpt_no_hit() {
+  HitRec h;
+  h.prim = -1;
+  h.t = __builtin_inff();
+  h.f = h.g = 0.0f;
+  h.back = false;
+  return h;
+}
+
+// The set-up of one BDPT sample of pixel (px, py): Scene::EvaluateBidirectionalPathTracing,
+// and the starts of BuildLightsPath and BuildTheEyePath
 // (scene_bidirectionalpathtracing.cpp:8-126, 251-347) into L and E (kmax + 2
-// entries each), with per-thread closest-hit queries.
-template <bool kCount>
-__device__ __forceinline__ void bdpt_build(const DevScene& s, const DevCamera& cam,
-                                           const BdptConst& bc, uint32_t px, uint32_t py,
-                                           uint32_t smp, uint32_t* stk, uint32_t stride,
-                                           BVert* L, BVert* E, uint32_t* queries) {
+// entries each) and the working vertices wl, we.  Returns whether the light
+// path has a first ray (from L[1].pos along wl.wout); the eye path always has
+// one (from E[1].pos along we.win).
+__device__ __forceinline__ bool bdpt_begin(const DevScene& s, const DevCamera& cam, const BdptConst& bc,
+                                           uint32_t px, uint32_t py, uint32_t smp, BVert* L, BVert* E,
+                                           BWork& wl, BWork& we) {
   uint32_t const W = cam.width;
   uint32_t const pixel = py * W + px;
   uint32_t const kmax = bc.c.kmax;
@@ -353,15 +363,10 @@ __device__ __forceinline__ void bdpt_build(const DevScene& s, const DevCamera& c
     bvert_clear(L[i]);
     bvert_clear(E[i]);
   }
-  auto find = [&](float3 o, float3 d) {
-    if (kCount) ++*queries;
-    return trace_lane(s, o, d, stk, stride);
-  };
   float4 const de = draw4(bc.c, pixel, smp, 0u, kBdptStreamEye);
   float4 const dl = draw4(bc.c, pixel, smp, 0u, kBdptStreamLight);
   // SampleLightPosition and Triangle::Sample's basis.  wl: the working part of
   // the light path's last vertex.
-  BWork wl;
   {
     LightVertex const lv = sample_light(s, dl.x, dl.y);
     L[1].tri = lv.prim;
@@ -383,8 +388,9 @@ __device__ __forceinline__ void bdpt_build(const DevScene& s, const DevCamera& c
   E[1].ng = E[1].ns = fwd;
   E[1].power = f3(1.f, 1.f, 1.f);
 
-  // BuildLightsPath
-  if (L[1].tri >= 0) {
+  // BuildLightsPath's set-up
+  bool const light = L[1].tri >= 0;
+  if (light) {
     L[0].s_next = bc.inv_area;
     wl.cos_in_sh = 1.0f;
     wl.cos_out_geo = bd_diffused(dl.z, dl.w, wl.t, L[1].ng, wl.b, &wl.wout);
@@ -392,48 +398,9 @@ __device__ __forceinline__ void bdpt_build(const DevScene& s, const DevCamera& c
     L[1].s_next = float(kInvPi);
     L[1].g = 1.0f;
     wl.bsdf_x_ipdf = 1.0f;
-    for (uint32_t k = 1;;) {
-      BVert& nw = L[k + 1];
-      BWork wn;
-      HitRec const h = find(L[k].pos, wl.wout);
-      nw.tri = h.prim;
-      if (h.prim < 0) break;
-      uint32_t bsdf;
-      {
-        float4 const ngb = gp(s.prim_ng)[h.prim];
-        bsdf = __float_as_uint(ngb.w);
-      }
-      DevBsdf const bs = gp(s.bsdfs)[bsdf];
-      if (bs.type == XYZ_LIGHT) {
-        nw.tri = -1;
-        break;
-      }
-      geometric_basis(s, h.prim, h.f, h.g, h.back, nw, wn);
-      wn.back = h.back;
-      wn.win = neg3(wl.wout);
-      wn.cos_in_sh = dot3(wn.win, nw.ns);
-      if (wn.cos_in_sh <= 0.0f) {
-        nw.tri = -1;
-        break;
-      }
-      nw.power = scl3(L[k].power, wl.bsdf_x_ipdf);
-      nw.pos = add3(scl3(wl.wout, h.t), L[k].pos);
-      nw.g = wl.cos_out_geo * wn.cos_in_sh / (h.t * h.t);
-      wl = wn;
-      if (++k >= kmax) {
-        L[k + 1].tri = -1;
-        break;
-      }
-      if (!lights_to_camera(s, bs, h.prim, h.f, h.g, L[k], wl,
-                            draw4(bc.c, pixel, smp, k, kBdptStreamLight))) {
-        L[k + 1].tri = -1;
-        break;
-      }
-    }
   }
 
-  // BuildTheEyePath.  we: the working part of the eye path's last vertex.
-  BWork we;
+  // BuildTheEyePath's set-up.  we: the working part of the eye path's last vertex.
   we.win = primary_dir(cam, px, py, de.x, de.y);
   we.cos_in_sh = dot3(we.win, E[1].ns);
   we.cos_out_geo = 1.0f;
@@ -441,42 +408,168 @@ __device__ __forceinline__ void bdpt_build(const DevScene& s, const DevCamera& c
   E[1].s_next = 1.0f;
   E[1].g = 1.0f;
   we.bsdf_x_ipdf = 1.0f;
+  return light;
+}
+
+// One step of BuildLightsPath: the vertex after L[k] from h, the hit of the
+// ray from L[k].pos along wl.wout.  Returns whether the path goes on (then k
+// has advanced and the next ray leaves L[k].pos along wl.wout).
+__device__ __forceinline__ bool bdpt_light_step(const DevScene& s, const BdptConst& bc, uint32_t pixel,
+                                                uint32_t smp, BVert* L, BWork& wl, uint32_t& k,
+                                                const HitRec& h) {
+  uint32_t const kmax = bc.c.kmax;
+  BVert& nw = L[k + 1];
+  BWork wn;
+  nw.tri = h.prim;
+  if (h.prim < 0) return false;
+  uint32_t bsdf;
+  {
+    float4 const ngb = gp(s.prim_ng)[h.prim];
+    bsdf = __float_as_uint(ngb.w);
+  }
+  DevBsdf const bs = gp(s.bsdfs)[bsdf];
+  if (bs.type == XYZ_LIGHT) {
+    nw.tri = -1;
+    return false;
+  }
+  geometric_basis(s, h.prim, h.f, h.g, h.back, nw, wn);
+  wn.back = h.back;
+  wn.win = neg3(wl.wout);
+  wn.cos_in_sh = dot3(wn.win, nw.ns);
+  if (wn.cos_in_sh <= 0.0f) {
+    nw.tri = -1;
+    return false;
+  }
+  nw.power = scl3(L[k].power, wl.bsdf_x_ipdf);
+  nw.pos = add3(scl3(wl.wout, h.t), L[k].pos);
+  nw.g = wl.cos_out_geo * wn.cos_in_sh / (h.t * h.t);
+  wl = wn;
+  if (++k >= kmax) {
+    L[k + 1].tri = -1;
+    return false;
+  }
+  if (!lights_to_camera(s, bs, h.prim, h.f, h.g, L[k], wl, draw4(bc.c, pixel, smp, k, kBdptStreamLight))) {
+    L[k + 1].tri = -1;
+    return false;
+  }
+  return true;
+}
+
+// One step of BuildTheEyePath: the vertex after E[k] from h, the hit of the
+// ray from E[k].pos along we.win.  Returns whether the path goes on (then k
+// has advanced and the next ray leaves E[k].pos along we.win).
+__device__ __forceinline__ bool bdpt_eye_step(const DevScene& s, const BdptConst& bc, uint32_t pixel,
+                                              uint32_t smp, BVert* E, BWork& we, uint32_t& k,
+                                              const HitRec& h) {
+  uint32_t const kmax = bc.c.kmax;
+  BVert& nw = E[k + 1];
+  BWork wn;
+  nw.tri = h.prim;
+  if (h.prim < 0) return false;
+  uint32_t bsdf;
+  Basis const B = shading_basis(s, h.prim, h.f, h.g, h.back, &bsdf);
+  nw.ns = B.ns;
+  nw.ng = B.ng;
+  wn.t = B.t;
+  wn.b = B.b;
+  wn.back = h.back;
+  wn.wout = neg3(we.win);
+  if (dot3(wn.wout, nw.ns) <= 0.0f) {
+    nw.tri = -1;
+    return false;
+  }
+  nw.power = scl3(E[k].power, we.bsdf_x_ipdf);
+  nw.pos = add3(scl3(we.win, h.t), E[k].pos);
+  wn.cos_out_geo = dot3(wn.wout, nw.ng);
+  nw.g = wn.cos_out_geo * we.cos_in_sh / (h.t * h.t);
+  we = wn;
+  DevBsdf const bs = gp(s.bsdfs)[bsdf];
+  if (++k >= kmax) {
+    E[k + 1].tri = -1;
+    return false;
+  }
+  if (bs.type == XYZ_LIGHT) {
+    E[k + 1].tri = -1;
+    return false;
+  }
+  if (!camera_to_lights(s, bs, h.prim, h.f, h.g, E[k], we, draw4(bc.c, pixel, smp, k, kBdptStreamEye))) {
+    E[k + 1].tri = -1;
+    return false;
+  }
+  return true;
+}
+
+// The two subpaths of one BDPT sample of pixel (px, py) with per-thread
+// closest-hit queries: bdpt_begin, then the light path's and the eye path's
+// steps.
+template <bool kCount>
+__device__ __forceinline__ void bdpt_build(const DevScene& s, const DevCamera& cam,
+                                           const BdptConst& bc, uint32_t px, uint32_t py,
+                                           uint32_t smp, uint32_t* stk, uint32_t stride,
+                                           BVert* L, BVert* E, uint32_t* queries) {
+  uint32_t const pixel = py * cam.width + px;
+  BWork wl, we;
+  bool const light = bdpt_begin(s, cam, bc, px, py, smp, L, E, wl, we);
+  if (light) {
+    for (uint32_t k = 1;;) {
+      if (kCount) ++*queries;
+      HitRec const h = trace_lane(s, L[k].pos, wl.wout, stk, stride);
+      if (!bdpt_light_step(s, bc, pixel, smp, L, wl, k, h)) break;
+    }
+  }
   for (uint32_t k = 1;;) {
-    BVert& nw = E[k + 1];
-    BWork wn;
-    HitRec const h = find(E[k].pos, we.win);
-    nw.tri = h.prim;
-    if (h.prim < 0) break;
-    uint32_t bsdf;
-    Basis const B = shading_basis(s, h.prim, h.f, h.g, h.back, &bsdf);
-    nw.ns = B.ns;
-    nw.ng = B.ng;
-    wn.t = B.t;
-    wn.b = B.b;
-    wn.back = h.back;
-    wn.wout = neg3(we.win);
-    if (dot3(wn.wout, nw.ns) <= 0.0f) {
-      nw.tri = -1;
-      break;
+    if (kCount) ++*queries;
+    HitRec const h = trace_lane(s, E[k].pos, we.win, stk, stride);
+    if (!bdpt_eye_step(s, bc, pixel, smp, E, we, k, h)) break;
+  }
+}
+
+// The two subpaths of the items of one wave, built with the cooperative
+// traversal; all 64 lanes call it together (act: the lane has an item).  Each
+// round, every lane with a pending ray (its light path's, then, once that has
+// ended, its eye path's) traces it -- coop_step's wave-shared triangle and box
+// batches, which lanes without a ray help with -- and takes that subpath's
+// next step.  The steps, their order within each subpath and the hits are the
+// per-thread build's, so L and E are the same.  keys and own: this wave's LDS
+// tables of coop_step.
+template <bool kCount>
+__device__ __forceinline__ void bdpt_build_wave(const DevScene& s, const DevCamera& cam,
+                                                const BdptConst& bc, bool act, uint32_t px,
+                                                uint32_t py, uint32_t smp, uint32_t* stk,
+                                                uint32_t stride, unsigned long long* keys, uint8_t* own,
+                                                BVert* L, BVert* E, uint32_t* queries) {
+  uint32_t const pixel = py * cam.width + px;
+  BWork wl, we;
+  bool light = false;
+  if (act) light = bdpt_begin(s, cam, bc, px, py, smp, L, E, wl, we);
+  uint32_t phase = act ? (light ? 0u : 1u) : 2u;  // 0 light path, 1 eye path, 2 done
+  uint32_t kl = 1, ke = 1;
+  while (__ballot(phase < 2u) != 0) {
+    bool const pend = phase < 2u;
+    float3 o = f3(0.0f, 0.0f, 0.0f), d = f3(0.0f, 0.0f, 1.0f);
+    if (phase == 0u) {
+      o = L[kl].pos;
+      d = wl.wout;
+    } else if (phase == 1u) {
+      o = E[ke].pos;
+      d = we.win;
     }
-    nw.power = scl3(E[k].power, we.bsdf_x_ipdf);
-    nw.pos = add3(scl3(we.win, h.t), E[k].pos);
-    wn.cos_out_geo = dot3(wn.wout, nw.ng);
-    nw.g = wn.cos_out_geo * we.cos_in_sh / (h.t * h.t);
-    we = wn;
-    DevBsdf const bs = gp(s.bsdfs)[bsdf];
-    if (++k >= kmax) {
-      E[k + 1].tri = -1;
-      break;
+    if (kCount && pend) ++*queries;
+    TraceState st;
+    trace_init(st, o, d);
+    HitRec h = bdpt_no_hit();
+    bool q = pend;
+    while (__ballot(q) != 0) {
+      bool const fin = coop_step<false>(s, st, q, stk, stride, keys, own);
+      if (q && fin) {
+        h = trace_result(s, st);
+        q = false;
+      }
     }
-    if (bs.type == XYZ_LIGHT) {
-      E[k + 1].tri = -1;
-      break;
-    }
-    if (!camera_to_lights(s, bs, h.prim, h.f, h.g, E[k], we,
-                          draw4(bc.c, pixel, smp, k, kBdptStreamEye))) {
-      E[k + 1].tri = -1;
-      break;
+    if (phase == 0u) {
+      if (!bdpt_light_step(s, bc, pixel, smp, L, wl, kl, h)) phase = 1u;
+    } else if (phase == 1u) {
+      if (!bdpt_eye_step(s, bc, pixel, smp, E, we, ke, h)) phase = 2u;
     }
   }
 }
@@ -587,14 +680,6 @@ struct BdptTrace {
   }
 };
 
-__device__ __forceinline__ HitRec bdpt_no_hit() {
-  HitRec h;
-  h.prim = -1;
-  h.t = __builtin_inff();
-  h.f = h.g = 0.0f;
-  h.back = false;
-  return h;
-}
 
 // Counts a sample's visibility queries (every answer a miss).
 struct BdptCountQueries {

@@ -2066,10 +2066,12 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // Queued form of the same estimator, in three passes per chunk of items:
-//   1. k_bdpt_paths: a thread per item builds the two subpaths (per-thread
-//      queries), stores their vertices, and writes the visibility rays of all
-//      its connections to a queue (a count pass over the connections first,
-//      then one atomic per wave for the wave's entries);
+//   1. k_bdpt_paths: a thread per item builds the two subpaths (the wave's
+//      lanes step their subpaths together and trace each round's rays with
+//      the cooperative traversal), stores their vertices, and writes the
+//      visibility rays of all its connections to a queue (a count pass over
+//      the connections first, then one atomic per wave for the wave's
+//      entries);
 //   2. the cooperative closest-hit kernel over the queue;
 //   3. k_bdpt_connect: a thread per item reloads its subpaths and evaluates
 //      its connections in the reference's order, reading the queue's hits in
@@ -2077,12 +2079,11 @@ __global__ void __launch_bounds__(kBlock)
 // The connections' queries are the bulk of the estimator's work; traced by one
 // thread each they diverge with the scene's size.  xyz_render_bdpt takes the
 // queued form for scenes of at least XYZ_BDPT_QUEUED_MIN_REFS kd leaf
-// references (measured crossover between 1.4e5 and 2.2e5 on the batch
-// workload's scenes at 256x256, 16 spp; on the 36-triangle Cornell box the
-// per-thread form is 1.4x faster, on the 2M-triangle production scene the
-// queued form 1.7x), unless the caller names the form (params->engine).
+// references (measured crossover between 2.2e5 and 4.6e5 references on the
+// batch workload's scenes at 256x256, 16 spp), unless the caller names the
+// form (params->engine).
 #ifndef XYZ_BDPT_QUEUED_MIN_REFS
-#define XYZ_BDPT_QUEUED_MIN_REFS 200000
+#define XYZ_BDPT_QUEUED_MIN_REFS 300000
 #endif
 
 // Most connection queries of one sample: kmax - 2 for light tracing and
@@ -2101,16 +2102,21 @@ __global__ void __launch_bounds__(kBlock)
                  float* __restrict__ qdir, uint32_t* __restrict__ qcount, uint32_t qcap,
                  uint32_t n, uint32_t sample0, uint32_t items, unsigned long long* queries) {
   extern __shared__ uint32_t lds_stack[];
+  __shared__ unsigned long long keys[2 * kBlock];  // coop_step's tables, per wave
   uint32_t* const stk = lds_stack + threadIdx.x;
   uint32_t const i = blockIdx.x * kBlock + threadIdx.x;
   bool const act = i < items;
   uint32_t const kmax = bc.c.kmax;
   uint32_t nq = 0, q = 0;
   BVert L[kBdptMaxDepth + 3], E[kBdptMaxDepth + 3];
+  {
+    uint32_t const smp = sample0 + (act ? i / n : 0u);
+    uint32_t const p = act ? i % n : 0u;
+    bdpt_build_wave<kCount>(s, cam, bc, act, p % cam.width, p / cam.width, smp, stk, kBlock,
+                            keys + (threadIdx.x & ~63u),
+                            reinterpret_cast<uint8_t*>(keys + kBlock + (threadIdx.x & ~63u)), L, E, &q);
+  }
   if (act) {
-    uint32_t const smp = sample0 + i / n;
-    uint32_t const p = i % n;
-    bdpt_build<kCount>(s, cam, bc, p % cam.width, p / cam.width, smp, stk, kBlock, L, E, &q);
     BdptCountQueries c{0};
     (void)bdpt_connect(s, cam, bc, L, E, nullptr, c);
     nq = c.n;
