@@ -2,7 +2,10 @@
 // run on the host with the production functions: tri_test (the reference's
 // ray/triangle test, xyz/geom/triangle.cpp:127-165) and cluster_may_hit, with
 // clusters built by cluster_bounds.  Whenever tri_test accepts a triangle with
-// t in [t_min, t_max], cluster_may_hit must not rule out its cluster.
+// t in [t_min, t_max], cluster_may_hit must not rule out its cluster, and
+// root_clip, given a root record holding the cluster's (its own record, or a
+// union with a larger box and larger margins), must keep t in the query's
+// narrowed interval and must not declare a miss.
 //
 // The rays aim at the hardest cases for the error bound: grazing rays whose
 // |det| is just above the 1e-6 threshold, on triangles up to the size where
@@ -87,7 +90,7 @@ double box_distance(float4 L, float4 H, V p) {
 }
 
 struct Stats {
-  uint64_t checks = 0, accepted = 0, violations = 0, culled = 0, cull_tries = 0;
+  uint64_t checks = 0, accepted = 0, violations = 0, culled = 0, cull_tries = 0, root_checks = 0;
   double worst = 0.0;  // max distance / margin over accepted hits
 };
 
@@ -113,7 +116,30 @@ void check(const std::vector<float4>& A, const std::vector<float4>& B, const std
     st->accepted++;
     // the tightest interval around the accepted t must keep the cluster too
     bool const keep_t = may_hit(L, H, o, d, t, t);
-    if (!keep || !keep_t) {
+    // the root interval of a query along this ray must keep t: with the
+    // cluster's own record as the root, and with a union of it and a larger
+    // box with larger P and Q
+    bool keep_root = true;
+    if (t >= kTMin) {
+      for (int variant = 0; variant < 2; ++variant) {
+        DevScene sc{};
+        sc.root_lo = L;
+        sc.root_hi = H;
+        if (variant == 1) {
+          float const e = float(loguni(1e-6, 10.0));
+          sc.root_lo = make_float4(L.x - e * float(uni()), L.y - e * float(uni()), L.z - e * float(uni()),
+                                   L.w * float(1.0 + uni()));
+          sc.root_hi = make_float4(H.x + e * float(uni()), H.y + e * float(uni()), H.z + e * float(uni()),
+                                   H.w * float(1.0 + uni()));
+        }
+        TraceState ts;
+        trace_init(ts, o, d);
+        root_clip(sc, ts);
+        st->root_checks++;
+        if (ts.in_leaf || t < ts.t_min || t > ts.t_max) keep_root = false;
+      }
+    }
+    if (!keep || !keep_t || !keep_root) {
       st->violations++;
       if (st->violations <= 5)
         std::printf("VIOLATION: o=(%a,%a,%a) d=(%a,%a,%a) tmin=%a tmax=%a t=%a det=%a\n", o.x, o.y,
@@ -260,8 +286,9 @@ int main(int argc, char** argv) {
     cluster_case(&st);
     patch_case(&st);
   }
-  std::printf("checks %llu accepted %llu violations %llu culled %.4f worst distance/margin %.3e\n",
+  std::printf("checks %llu accepted %llu root checks %llu violations %llu culled %.4f worst distance/margin %.3e\n",
               (unsigned long long)st.checks, (unsigned long long)st.accepted,
-              (unsigned long long)st.violations, double(st.culled) / double(st.cull_tries), st.worst);
+              (unsigned long long)st.root_checks, (unsigned long long)st.violations,
+              double(st.culled) / double(st.cull_tries), st.worst);
   return st.violations ? 1 : 0;
 }

@@ -76,6 +76,9 @@ struct DevScene {
   const uint32_t* rec_rank;
   const uint32_t* rank_pos;
   uint32_t stack_cap;  // kd_max_depth + 1 stack entries (one 32-bit word each) per lane
+  // the root record: the union of the cluster boxes, (lo, largest P) and
+  // (hi, largest Q); P NaN when no clip may be made (root_clip)
+  float4 root_lo, root_hi;
   // shading
   const float4* prim_ng;    // (ng.xyz, bsdf index as bits)
   const float4* prim_n0;    // NULL for flat shading, else n0.xyz
@@ -193,7 +196,7 @@ struct TraceState {
 
 constexpr float kTMin = 1e-5f;  // Intersection::Init, xyz/geom/intersection.hpp:11-14
 
-__device__ __forceinline__ void trace_init(TraceState& st, float3 o, float3 d) {
+__host__ __device__ __forceinline__ void trace_init(TraceState& st, float3 o, float3 d) {
   st.o = o;
   st.d = d;
   st.t_min = kTMin;
@@ -437,6 +440,59 @@ __host__ __device__ __forceinline__ bool cluster_may_hit(float4 L, float4 H, flo
   float const dt = XYZ_CULL_MARGIN * tr *
                    fminf(fminf(fabsf(inv.x), fabsf(inv.y)), fabsf(inv.z)) * 1.00001f;
   return !covered || !((te - dt > t_max) || (tx + dt < t_min) || (te > tx));
+}
+
+// The root interval of a fresh query narrowed to where a hit can lie.  The
+// root record (the union of the cluster boxes with the largest P and Q) makes
+// every quantity of cluster_may_hit at least as large as for any cluster, and
+// each of its steps is monotone in them, so the record's widened slab
+// interval [te - dt, tx + dt] holds every t at which tri_test can accept a
+// triangle of the scene (cluster_may_hit over [t, t] keeps its cluster), and
+// an empty one means the ray meets none.  The reference's traversal (root
+// interval [1e-5, +inf]) run over [max(1e-5, te - dt), min(t_max, tx + dt)]
+// instead visits the same leaves with intervals cut to that range, leaves
+// out leaves wholly outside it, where nothing can be accepted, and so finds
+// the same triangle at the same t (the argument of the capped shadow query,
+// at both ends).  A query that certainly misses is put in an empty leaf with
+// an empty stack, which its next step ends as a miss.  Rays or scenes outside
+// cluster_may_hit's covered range keep their interval.
+#ifndef XYZ_ROOT_CLIP
+#define XYZ_ROOT_CLIP 1
+#endif
+__host__ __device__ __forceinline__ void root_clip(const DevScene& s, TraceState& st) {
+  float dlen;
+  float3 inv;
+  cull_ray(st.d, &dlen, &inv);
+  float4 const L = s.root_lo, H = s.root_hi;
+  float3 const o = st.o;
+  float const u = 5.9604645e-08f;
+  float const a = dlen * L.w;
+  float const rx = fmaxf(fabsf(o.x - L.x), fabsf(H.x - o.x));
+  float const ry = fmaxf(fabsf(o.y - L.y), fabsf(H.y - o.y));
+  float const rz = fmaxf(fabsf(o.z - L.z), fabsf(H.z - o.z));
+  float const R = approx_sqrt(rx * rx + ry * ry + rz * rz) * 1.000002f;
+  float const tr = R * (a + 8.0f * u);
+  float const D = XYZ_CULL_MARGIN * (tr + dlen * H.w) * 1.00001f;
+  bool const covered = (a <= 0.3f) && (R < 1e12f) && (D < 1e30f);
+  if (!covered) return;
+  float const px = ((L.x - D) - o.x) * inv.x, qx = ((H.x + D) - o.x) * inv.x;
+  float const py = ((L.y - D) - o.y) * inv.y, qy = ((H.y + D) - o.y) * inv.y;
+  float const pz = ((L.z - D) - o.z) * inv.z, qz = ((H.z + D) - o.z) * inv.z;
+  float te = fmaxf(fmaxf(fminf(px, qx), fminf(py, qy)), fminf(pz, qz));
+  float tx = fminf(fminf(fmaxf(px, qx), fmaxf(py, qy)), fmaxf(pz, qz));
+  te = te * ((te > 0.0f) ? 0.9999994f : 1.0000006f);
+  tx = tx * ((tx > 0.0f) ? 1.0000006f : 0.9999994f);
+  float const dt = XYZ_CULL_MARGIN * tr *
+                   fminf(fminf(fabsf(inv.x), fabsf(inv.y)), fabsf(inv.z)) * 1.00001f;
+  float const lo = te - dt, hi = tx + dt;
+  if ((te > tx) || (lo > st.t_max) || (hi < st.t_min)) {
+    st.in_leaf = true;  // an empty leaf (lj == lend), nothing pending: a miss
+    st.lc0 = st.ln = st.lj = st.lend = 0;
+    st.sp = 0;
+    return;
+  }
+  if (lo > st.t_min) st.t_min = lo;
+  if (hi < st.t_max) st.t_max = st.t_cap = hi;
 }
 
 // ---------------------------------------------------------------------------

@@ -218,8 +218,11 @@ constexpr uint32_t kHitBlocksPerCu = XYZ_HIT_BLOCKS_PER_CU;
 // each; a block starts on part blockIdx.x % 8, the XCD that round-robin
 // dispatch gives it, so each XCD's L2 serves neighbouring rays, and moves on to
 // the other parts when its own is used up.
+// Waves per SIMD the kernel is compiled for: 7, what its LDS allows (7
+// workgroups of 4 waves per CU), which keeps it within 72 VGPRs (0: the
+// compiler's choice, which took 73 with root_clip and left 6 waves).
 #ifndef XYZ_HIT_WAVES_PER_EU
-#define XYZ_HIT_WAVES_PER_EU 0
+#define XYZ_HIT_WAVES_PER_EU 7
 #endif
 template <bool kCount, bool kFG>
 __global__ void __launch_bounds__(kHitBlock)
@@ -249,6 +252,7 @@ closest_hit_kernel(DevScene s, uint32_t n, const float* __restrict__ org,
       r = got;
       trace_init(st, make_float3(org[3 * size_t(r)], org[3 * size_t(r) + 1], org[3 * size_t(r) + 2]),
                  make_float3(dir[3 * size_t(r)], dir[3 * size_t(r) + 1], dir[3 * size_t(r) + 2]));
+      if (!kCount && XYZ_ROOT_CLIP) root_clip(s, st);
     }
     if (__ballot(r != 0xFFFFFFFFu) == 0) break;
     bool const act = r != 0xFFFFFFFFu;
@@ -440,7 +444,7 @@ uint32_t spread10(uint32_t x) {  // 10 bits -> every third bit
 // Device triangle layout: each leaf's triangles, in Morton order of their
 // centroids within the leaf, grouped in clusters of kClusterSize records.
 struct Clustered {
-  std::vector<float4> lo, hi;           // per cluster: (box lo, N), (box hi, E)
+  std::vector<float4> lo, hi;           // per cluster: (box lo - K, P), (box hi + K, Q)
   std::vector<float4> A, B, C;          // per record
   std::vector<int32_t> prim;            // per record
   std::vector<uint32_t> rank, rank_pos; // per record / per leaf rank
@@ -749,6 +753,22 @@ int scene_create(const xyz_scene_desc* desc, xyz_scene** out) {
   UP(tex, tex);
 #undef UP
   D.stack_cap = T.max_depth_param + 1;
+  {
+    // the root record of root_clip
+    float const inf = std::numeric_limits<float>::infinity();
+    float4 rl = make_float4(inf, inf, inf, 0.0f), rh = make_float4(-inf, -inf, -inf, 0.0f);
+    bool nan = CL.lo.empty();
+    for (size_t c = 0; c < CL.lo.size(); ++c) {
+      float4 const l = CL.lo[c], h = CL.hi[c];
+      nan = nan || std::isnan(l.x) || std::isnan(l.y) || std::isnan(l.z) || std::isnan(l.w) ||
+            std::isnan(h.x) || std::isnan(h.y) || std::isnan(h.z) || std::isnan(h.w);
+      rl = make_float4(std::min(rl.x, l.x), std::min(rl.y, l.y), std::min(rl.z, l.z), std::max(rl.w, l.w));
+      rh = make_float4(std::max(rh.x, h.x), std::max(rh.y, h.y), std::max(rh.z, h.z), std::max(rh.w, h.w));
+    }
+    if (nan) rl.w = std::numeric_limits<float>::quiet_NaN();
+    D.root_lo = rl;
+    D.root_hi = rh;
+  }
   D.n_lights = uint32_t(lprim.size());
   D.light_area = light_area;
   D.n_bsdfs = nb;
