@@ -113,6 +113,58 @@ def test_workspace_reused_across_streams(capi_mod, device, monkeypatch, budget):
             R.hipStreamDestroy(s)
 
 
+def test_render_captured_in_a_graph(capi_mod, device):
+    """xyz_render (no stats) captured on a stream into a hipGraph: its
+    workspaces come from the stream-ordered pool inside the graph.  Each of
+    three replays, with eager renders and synchronisations between them, adds
+    the same film as one eager render, bit for bit."""
+    import ctypes
+    from xyz_amd import hip
+    from xyz_amd.hip import DeviceArray
+    R = hip.rt()
+    vp = ctypes.c_void_p
+    R.hipStreamBeginCapture.argtypes = [vp, ctypes.c_int]
+    R.hipStreamEndCapture.argtypes = [vp, ctypes.POINTER(vp)]
+    R.hipGraphInstantiate.argtypes = [ctypes.POINTER(vp), vp, vp, ctypes.c_char_p, ctypes.c_size_t]
+    R.hipGraphLaunch.argtypes = [vp, vp]
+    R.hipGraphExecDestroy.argtypes = [vp]
+    R.hipGraphDestroy.argtypes = [vp]
+    sc = S.cornell_box()
+    g = capi_mod.Scene.from_dict(sc)
+    cam = sc["camera"]
+    W, H, spp = 256, 256, 4  # 262,144 path slots: wave buffers above 32 MiB
+    one, _ = gpu_render(capi_mod, g, cam, W, H, spp, 8, rr_start=3, seed=5)
+    film = DeviceArray.zeros((H, W, 3), np.float32)
+    st = vp()
+    assert R.hipStreamCreate(ctypes.byref(st)) == 0
+    graph, exe = vp(), vp()
+    try:
+        hip.synchronize()
+        assert R.hipStreamBeginCapture(st, 0) == 0  # global capture mode
+        c = capi_mod.make_camera(cam, W, H)
+        p = capi_mod.make_params(spp, 8, 3, 0.2, 5, 0)
+        rc = capi_mod.lib().xyz_render(g.h, ctypes.byref(c), ctypes.byref(p), ctypes.c_void_p(film.data_ptr()),
+                                       None, st)
+        assert R.hipStreamEndCapture(st, ctypes.byref(graph)) == 0
+        capi_mod.check(rc)
+        assert R.hipGraphInstantiate(ctypes.byref(exe), graph, None, None, 0) == 0
+        for k in range(3):
+            film.zero()
+            hip.synchronize()
+            assert R.hipGraphLaunch(exe, st) == 0
+            assert R.hipStreamSynchronize(st) == 0
+            _assert_same(film.numpy(), one)
+            # eager work between the replays
+            other, _ = gpu_render(capi_mod, g, cam, 64, 48, 2, 8, rr_start=3, seed=k)
+            assert other.sum() > 0
+    finally:
+        if exe:
+            R.hipGraphExecDestroy(exe)
+        if graph:
+            R.hipGraphDestroy(graph)
+        R.hipStreamDestroy(st)
+
+
 def test_textured_cornell_box_bit_exact(oracle_mod, capi_mod, device):
     sc = S.cornell_box(textured=True, tex_res=16)
     o, g = both(oracle_mod, capi_mod, sc)
