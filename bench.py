@@ -463,8 +463,9 @@ def closest_hit_roofline(n, ms_avg, alg_bytes, exec_bytes, w_tris, w_boxes, devi
                         frac=round(traffic / s / 1e9 / HBM_PEAK_GBS, 4))
     if "l2_hit_rate" in pmc:
         r["l2_hit_rate"] = pmc["l2_hit_rate"]
-    # the reference's work at SURVEY-style bytes (32 + 8 nodes + 40 triangle tests
-    # per ray): culling executes 38 % of those tests, so this rate is not a bound
+    # the reference's work in bytes (32 per ray, 8 per node, 40 per triangle
+    # test): culling executes about a third of those tests, so this rate is
+    # not a bound
     r["reference_work"] = dict(bytes_per_launch=int(alg_bytes),
                                gb_per_s=round(alg_bytes / s / 1e9, 1))
     r["executed"] = dict(bytes_per_launch=int(exec_bytes),
@@ -533,9 +534,9 @@ def bench_pt(args, hip, capi, scenes):
             single_thread=dict(value=round(n1 / ost1["seconds"] / 1e6, 4), cores=1,
                                sample=f"rows {rows1[0]}-{rows1[1] - 1} at 1 spp ({n1} samples)"))
         res["vs_cpu"] = round(rate / res["cpu_baseline"]["value"], 1)
-        # the reference's work per sample (SURVEY-style bytes), counted by the
-        # oracle on its band: 32 + 8 nodes + 40 triangle tests per query, 64 per
-        # shaded vertex, the film's 12 bytes per pixel over the spp
+        # the reference's work per sample in bytes, counted by the oracle on its
+        # band: 32 per query, 8 per node, 40 per triangle test, 64 per shaded
+        # vertex, the film's 12 bytes per pixel over the spp
         bs = (32 * ost["queries"] + 8 * ost["nodes"] + 40 * ost["tris"] + 64 * ost["vertices"]) / n
         bs += 12.0 / args.pt_spp
         res["roofline"]["reference_work"] = dict(
