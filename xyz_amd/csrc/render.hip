@@ -124,11 +124,15 @@ struct WaveBufs {
   uint32_t* qb;   // 8 * cap
   WaveCounters* ctr;
   uint32_t cap;
-  // adjoint: per (vertex vj, slot) at [vj * cap + slot]: (throughput before
-  // the albedo, bits(texel key)), (albedo, scale after the NEE step), (NEE
-  // term per unit throughput after the albedo, 0); tape_depth vertices
-  float4* tape_a;
-  float4* tape_b;
+  // adjoint: per (vertex vj, slot), record r = vj * cap + slot, tape_depth
+  // vertices.  tape_ab[2r], tape_ab[2r + 1]: (throughput before the albedo,
+  // bits(texel key)), (albedo, scale after the NEE step, negative when the
+  // vertex made a NEE request), written together once per vertex; tape_c[r]:
+  // (NEE term per unit throughput after the albedo, 0), written by the shadow
+  // kernel for every request (0 when occluded).  The slots a shading wave
+  // holds are spread over the octant queues, so each record is one 32-byte
+  // store: partial cache lines written back to HBM cost a read each.
+  float4* tape_ab;
   float4* tape_c;
   uint32_t tape_depth;
 };
@@ -214,8 +218,9 @@ __device__ __forceinline__ void tex_add(TexLds* L, float* d_tex, uint32_t key, f
 }
 
 // Gradient hooks of the adjoint's replay (see NoGrad).  Each shading vertex
-// writes its record to the slot's tape; a path that ends on a light writes
-// that light's emission per unit throughput to ctot.  The L_e derivative of
+// keeps its record in registers until finish() writes it to the slot's tape;
+// a path that ends on a light writes that light's emission per unit
+// throughput to ctot.  The L_e derivative of
 // an emission term is the term with pi A_lights in place of Phi
 // (Phi = L_e pi A_lights) and is deposited at once.
 struct GradReplay {
@@ -225,23 +230,23 @@ struct GradReplay {
   float* d_emit;
   EmitLds* lds;
   float4* ctot;
-  float4* tape_a;
-  float4* tape_b;
-  float4* tape_c;
+  float4* tape_ab;
   uint32_t cap, tape_depth;
   uint32_t slot;
   uint32_t vj;    // the current vertex
   float sigma;    // its throughput scale after the NEE step
+  bool recorded;  // vertex() ran for this shading call
+  float4 ra;      // the record's first half: throughput, bits(texel key)
+  float3 ralb;    // its albedo
   __device__ __forceinline__ void bind(const WaveBufs& wb, uint32_t s) {
     ctot = wb.ctot;
-    tape_a = wb.tape_a;
-    tape_b = wb.tape_b;
-    tape_c = wb.tape_c;
+    tape_ab = wb.tape_ab;
     cap = wb.cap;
     tape_depth = wb.tape_depth;
     slot = s;
     vj = 0;
     sigma = 1.0f;
+    recorded = false;
   }
   __device__ __forceinline__ void emission(uint32_t bsdf, float3 power, double wfac,
                                            const DevBsdf& b, uint32_t k) {
@@ -259,15 +264,19 @@ struct GradReplay {
     if (j >= tape_depth) return;  // the depth cap keeps j < tape_depth
     uint32_t const key = (texture >= 0 && texel >= 0)
                              ? tex[texture].offset / 3u + uint32_t(texel) : kEmptyKey;
-    size_t const i = size_t(j) * cap + slot;
-    tape_a[i] = make_float4(power.x, power.y, power.z, __uint_as_float(key));
-    tape_b[i] = make_float4(alb.x, alb.y, alb.z, 1.0f);
-    tape_c[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    ctot[slot] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(j + 1u));
+    ra = make_float4(power.x, power.y, power.z, __uint_as_float(key));
+    ralb = alb;
+    recorded = true;
   }
-  __device__ __forceinline__ void scale(float sc) {
-    sigma = sigma * sc;
-    if (vj < tape_depth) reinterpret_cast<float*>(&tape_b[size_t(vj) * cap + slot])[3] = sigma;
+  __device__ __forceinline__ void scale(float sc) { sigma = sigma * sc; }
+  // After the shading call: the vertex's record, flagged with whether it made
+  // a NEE request, and the path's vertex count.
+  __device__ __forceinline__ void finish(bool nee) {
+    if (!recorded) return;
+    size_t const r = size_t(vj) * cap + slot;
+    tape_ab[2 * r] = ra;
+    tape_ab[2 * r + 1] = make_float4(ralb.x, ralb.y, ralb.z, nee ? -sigma : sigma);
+    ctot[slot] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(vj + 1u));
   }
 };
 
@@ -662,6 +671,7 @@ k_shade(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBuf
         gr.lds = glds;
         gr.bind(wb, slot);
         cont = shade_vertex(J.s, J.c, ps, h, pixel, sample, lv, &sr, &has_sr, gr);
+        gr.finish(has_sr);
       } else {
         NoGrad ng;
         cont = shade_vertex(J.s, J.c, ps, h, pixel, sample, lv, &sr, &has_sr, ng);
@@ -766,7 +776,15 @@ k_shadow(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBu
     uint32_t const rr = rq;
     rq = 0xFFFFFFFFu;
     uint4 const meta = wb.smeta[rr];
-    if (h.prim != int32_t(meta.x)) continue;
+    if (h.prim != int32_t(meta.x)) {
+      if constexpr (kReplay) {
+        // occluded: the vertex's NEE term is 0
+        uint32_t const vj = meta.y >> 8;
+        if (vj < wb.tape_depth)
+          wb.tape_c[size_t(vj) * wb.cap + __float_as_uint(wb.spe[rr].w)] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      }
+      continue;
+    }
     float4 const o = wb.so[rr], w = wb.sw[rr];
     float4 const pe = wb.spe[rr], met = wb.smet[rr];
     ShadowReq sr;
@@ -790,7 +808,7 @@ k_shadow(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBu
       uint32_t const vj = meta.y >> 8;
       if (vj < wb.tape_depth) {
         size_t const ti = size_t(vj) * wb.cap + slot;
-        float4 const ta = wb.tape_a[ti], tb = wb.tape_b[ti];
+        float4 const ta = wb.tape_ab[2 * ti], tb = wb.tape_ab[2 * ti + 1];
         float3 const pw = mul3(ld3(ta), ld3(tb));  // the throughput after the albedo
         auto const& lb = gp(s.bsdfs)[meta.z];
         wb.tape_c[ti] = make_float4(float(double(lb.phi[0]) * wfac), float(double(lb.phi[1]) * wfac),
@@ -833,8 +851,11 @@ k_adjoint_sweep(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp,
     float3 R = ld3(e);
     for (uint32_t j = cnt; j-- > 0;) {
       size_t const ti = size_t(j) * wb.cap + i;
-      float4 const a = wb.tape_a[ti], b = wb.tape_b[ti], c = wb.tape_c[ti];
-      float3 const g = add3(ld3(c), scl3(R, b.w));
+      float4 const a = wb.tape_ab[2 * ti], b = wb.tape_ab[2 * ti + 1];
+      // the NEE term only where the vertex made a request (sign of the scale)
+      bool const nee = (__float_as_uint(b.w) >> 31) != 0u;
+      float3 const c = nee ? ld3(wb.tape_c[ti]) : f3(0.0f, 0.0f, 0.0f);
+      float3 const g = add3(c, scl3(R, fabsf(b.w)));
       uint32_t const key = __float_as_uint(a.w);
       if (key != kEmptyKey) tex_add(&lds, ga.d_tex, key, mul3(w, mul3(ld3(a), g)));
       R = mul3(ld3(b), g);
@@ -1030,7 +1051,7 @@ size_t wave_bytes(uint32_t cap, uint32_t tape_depth) {
   size_t const u4 = align_up(size_t(cap) * 16);
   size_t const qsz = align_up(size_t(cap) * 8 * 4);
   size_t const tape = replay ? align_up(size_t(cap) * tape_depth * 16) : 0;
-  return f4 * (replay ? 10 : 9) + u4 + 2 * qsz + align_up(sizeof(WaveCounters)) + 3 * tape;
+  return f4 * (replay ? 10 : 9) + u4 + 2 * qsz + align_up(sizeof(WaveCounters)) + 3 * tape;  // ab, c
 }
 
 // Lays one lane's wave buffers out in `mem` (wave_bytes(cap, tape_depth) bytes).
@@ -1061,8 +1082,7 @@ void carve_wave(void* mem, uint32_t cap, uint32_t tape_depth, WaveBufs* wb) {
   wb->qb = reinterpret_cast<uint32_t*>(take(qsz));
   wb->ctr = reinterpret_cast<WaveCounters*>(take(align_up(sizeof(WaveCounters))));
   wb->cap = cap;
-  wb->tape_a = replay ? reinterpret_cast<float4*>(take(tape)) : nullptr;
-  wb->tape_b = replay ? reinterpret_cast<float4*>(take(tape)) : nullptr;
+  wb->tape_ab = replay ? reinterpret_cast<float4*>(take(2 * tape)) : nullptr;
   wb->tape_c = replay ? reinterpret_cast<float4*>(take(tape)) : nullptr;
   wb->tape_depth = tape_depth;
 }
