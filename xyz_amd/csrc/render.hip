@@ -109,10 +109,12 @@ struct WaveCounters {
 };
 
 struct WaveBufs {
-  float4* pos;    // pos.xyz, f_next
-  float4* dir;    // dir.xyz, bits(k | specular << 31)
-  float4* pw;     // power.xyz
-  float4* col;    // colour.xyz
+  // path state, one 64-byte record per slot at ps[4 * slot]: (pos, f_next),
+  // (dir, bits(k | specular << 31)), (power, 0), (colour, 0).  Shading reads
+  // and rewrites whole records of slots spread over the octant queues; as one
+  // record a slot's state is one whole 64-byte segment instead of four
+  // partial ones.
+  float4* ps;
   float4* hit;    // bits(prim | back << 31), t, f, g
   float4* ctot;   // adjoint: emission per unit throughput of the light the path ends on, bits(vertex count)
   float4* so;     // shadow: origin.xyz, nk2
@@ -283,7 +285,8 @@ struct GradReplay {
 __device__ __forceinline__ uint32_t pack_k(uint32_t k, bool spec) { return k | (spec ? 0x80000000u : 0u); }
 
 __device__ __forceinline__ void load_state(const WaveBufs& wb, uint32_t slot, PathState& ps) {
-  float4 const p = wb.pos[slot], d = wb.dir[slot], w = wb.pw[slot], c = wb.col[slot];
+  float4 const* const r = wb.ps + 4 * size_t(slot);
+  float4 const p = r[0], d = r[1], w = r[2], c = r[3];
   ps.pos = ld3(p);
   ps.f_next = p.w;
   ps.dir = ld3(d);
@@ -295,10 +298,11 @@ __device__ __forceinline__ void load_state(const WaveBufs& wb, uint32_t slot, Pa
 }
 
 __device__ __forceinline__ void store_state(const WaveBufs& wb, uint32_t slot, const PathState& ps) {
-  wb.pos[slot] = make_float4(ps.pos.x, ps.pos.y, ps.pos.z, ps.f_next);
-  wb.dir[slot] = make_float4(ps.dir.x, ps.dir.y, ps.dir.z, __uint_as_float(pack_k(ps.k, ps.specular)));
-  wb.pw[slot] = make_float4(ps.power.x, ps.power.y, ps.power.z, 0.0f);
-  wb.col[slot] = make_float4(ps.color.x, ps.color.y, ps.color.z, 0.0f);
+  float4* const r = wb.ps + 4 * size_t(slot);
+  r[0] = make_float4(ps.pos.x, ps.pos.y, ps.pos.z, ps.f_next);
+  r[1] = make_float4(ps.dir.x, ps.dir.y, ps.dir.z, __uint_as_float(pack_k(ps.k, ps.specular)));
+  r[2] = make_float4(ps.power.x, ps.power.y, ps.power.z, 0.0f);
+  r[3] = make_float4(ps.color.x, ps.color.y, ps.color.z, 0.0f);
 }
 
 __device__ __forceinline__ uint32_t pad_seg(uint32_t c) { return (c + kSegPad - 1u) & ~(kSegPad - 1u); }
@@ -495,10 +499,11 @@ k_camera(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBu
   float4 const u = draw4(J.c, pixel, sample, 0);
   float3 o, d;
   primary_ray(J.cam, J.c, pixel, sample, u.x, u.y, &o, &d);
-  wb.pos[i] = make_float4(o.x, o.y, o.z, 0.0f);
-  wb.dir[i] = make_float4(d.x, d.y, d.z, __uint_as_float(pack_k(1u, true)));
-  wb.pw[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-  wb.col[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  float4* const r = wb.ps + 4 * size_t(i);
+  r[0] = make_float4(o.x, o.y, o.z, 0.0f);
+  r[1] = make_float4(d.x, d.y, d.z, __uint_as_float(pack_k(1u, true)));
+  r[2] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+  r[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   if (wb.ctot) wb.ctot[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // no vertex, no emission
   wb.qa[8u * wj.start[l] + (i - wj.start[l])] = i;
 }
@@ -540,7 +545,7 @@ k_extend(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBu
     DevScene const& s = *sp;
     if (r != 0xFFFFFFFFu) {
       slot = qcur[ext_entry(wj, lseg, r)];
-      trace_init(st, ld3(wb.pos[slot]), ld3(wb.dir[slot]));
+      trace_init(st, ld3(wb.ps[4 * size_t(slot)]), ld3(wb.ps[4 * size_t(slot) + 1]));
     }
     if (__ballot(slot != 0xFFFFFFFFu) == 0) break;
     bool const act = slot != 0xFFFFFFFFu;
@@ -819,8 +824,9 @@ k_shadow(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBu
         emit_add(glds, ga.d_emit, 3 * meta.z + 2, ga.d_film[3 * size_t(pixel) + 2] * float(double(pw.z) * u));
       }
     }
-    float4 const cc = wb.col[slot];
-    wb.col[slot] = make_float4(cc.x + add.x, cc.y + add.y, cc.z + add.z, 0.0f);
+    float4* const cr = wb.ps + 4 * size_t(slot) + 3;
+    float4 const cc = *cr;
+    *cr = make_float4(cc.x + add.x, cc.y + add.y, cc.z + add.z, 0.0f);
   }
   if constexpr (kReplay) emit_lds_flush(*glds, ga.d_emit, 3 * jobs[wj.jlo].s.n_bsdfs);
 }
@@ -912,7 +918,7 @@ __global__ void k_zero_counters(WaveCounters* ctr) {
 // blockIdx.y = local job.
 __global__ void __launch_bounds__(kBlock)
 k_accumulate(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp,
-             const float4* __restrict__ col) {
+             const float4* __restrict__ ps) {
   WaveJobs const& wj = *wjp;
   uint32_t const l = blockIdx.y;
   if (l >= wj.nj) return;
@@ -928,7 +934,7 @@ k_accumulate(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp,
   float* const film = J.film;
   float fx = film[3 * size_t(p)], fy = film[3 * size_t(p) + 1], fz = film[3 * size_t(p) + 2];
   for (; q < q0 + n; q += wh) {
-    float4 const c = col[s0 + (q - q0)];
+    float4 const c = ps[4 * size_t(s0 + (q - q0)) + 3];
     fx = fx + c.x;
     fy = fy + c.y;
     fz = fz + c.z;
@@ -1067,10 +1073,7 @@ void carve_wave(void* mem, uint32_t cap, uint32_t tape_depth, WaveBufs* wb) {
     b += n;
     return r;
   };
-  wb->pos = reinterpret_cast<float4*>(take(f4));
-  wb->dir = reinterpret_cast<float4*>(take(f4));
-  wb->pw = reinterpret_cast<float4*>(take(f4));
-  wb->col = reinterpret_cast<float4*>(take(f4));
+  wb->ps = reinterpret_cast<float4*>(take(4 * f4));
   wb->hit = reinterpret_cast<float4*>(take(f4));
   wb->so = reinterpret_cast<float4*>(take(f4));
   wb->sw = reinterpret_cast<float4*>(take(f4));
@@ -1343,7 +1346,7 @@ int enqueue_render(const RenderPlan& p, void* tables, WaveBufs* wb, hipStream_t 
       if (e != hipSuccess) return hip_fail(e, "render wait");
     }
     hipLaunchKernelGGL(k_accumulate, dim3((p.wmaxwh[k] + kBlock - 1) / kBlock, p.waves[k].nj),
-                       dim3(kBlock), 0, s, jobs_dev, wj, wb[i].col);
+                       dim3(kBlock), 0, s, jobs_dev, wj, wb[i].ps);
     if (lanes > 1) {
       e = hipEventRecord(ev.e[i], s);
       if (e != hipSuccess) return hip_fail(e, "render record");
