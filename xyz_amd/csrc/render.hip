@@ -115,7 +115,10 @@ struct WaveBufs {
   // record a slot's state is one whole 64-byte segment instead of four
   // partial ones.
   float4* ps;
-  float4* hit;    // bits(prim | back << 31), t, f, g
+  // the extension hits of the current bounce, by virtual queue position (the
+  // extension and shading kernels both walk the queues by position, so both
+  // access it in whole segments): bits(prim | back << 31), t, f, g
+  float4* hit;
   float4* ctot;   // adjoint: emission per unit throughput of the light the path ends on, bits(vertex count)
   float4* so;     // shadow: origin.xyz, nk2
   float4* sw;     // shadow: w.xyz, nk3
@@ -535,7 +538,7 @@ k_extend(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBu
   uint32_t job = 0;
   const DevScene* sp = &jobs[wj.jlo].s;  // the wave's job (uniform)
   TraceState st;
-  uint32_t slot = 0xFFFFFFFFu;
+  uint32_t slot = 0xFFFFFFFFu, hv = 0;
   for (;;) {
     bool switched = false;
     uint32_t lseg = 0;
@@ -544,6 +547,7 @@ k_extend(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBu
     if (switched) sp = &jobs[wj.jlo + __builtin_amdgcn_readfirstlane(job)].s;
     DevScene const& s = *sp;
     if (r != 0xFFFFFFFFu) {
+      hv = vst[lseg] + r;  // the query's virtual position
       slot = qcur[ext_entry(wj, lseg, r)];
       trace_init(st, ld3(wb.ps[4 * size_t(slot)]), ld3(wb.ps[4 * size_t(slot) + 1]));
     }
@@ -553,8 +557,8 @@ k_extend(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBu
                                             reinterpret_cast<uint8_t*>(keys + kBlock + (threadIdx.x & ~63u)));
     if (act && fin) {
       HitRec const h = trace_result(s, st);
-      wb.hit[slot] = make_float4(__int_as_float(h.prim < 0 ? -1 : (h.prim | (h.back ? 0x80000000 : 0))),
-                                 h.t, h.f, h.g);
+      wb.hit[hv] = make_float4(__int_as_float(h.prim < 0 ? -1 : (h.prim | (h.back ? 0x80000000 : 0))),
+                               h.t, h.f, h.g);
       slot = 0xFFFFFFFFu;
     }
   }
@@ -643,7 +647,7 @@ k_shade(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBuf
         pixel = uint32_t(item % J.wh);
         sample = J.sample0 + uint32_t(item / J.wh);
         load_state(wb, slot, ps);
-        h = unpack_hit(wb.hit[slot]);
+        h = unpack_hit(wb.hit[vb + threadIdx.x]);
         float4 const u0 = draw4(J.c, pixel, sample, 0);
         lv = sample_light(J.s, u0.z, u0.w);
         go = shade_begin<NoGrad, true>(J.s, J.c, ps, h, pixel, sample, x, ng);
@@ -663,7 +667,7 @@ k_shade(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBuf
       uint32_t const sample = J.sample0 + uint32_t(item / J.wh);
       PathState ps;
       load_state(wb, slot, ps);
-      HitRec const h = unpack_hit(wb.hit[slot]);
+      HitRec const h = unpack_hit(wb.hit[vb + threadIdx.x]);
       float4 const u0 = draw4(J.c, pixel, sample, 0);
       LightVertex const lv = sample_light(J.s, u0.z, u0.w);
       if (kReplay) {
@@ -1051,19 +1055,25 @@ size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
 
 // Bytes of one lane's wave buffers; tape_depth > 0: with the adjoint's
 // buffers (ctot and the vertex tape).
+// Virtual queue positions beyond a wave's slots: each of at most 8 segments
+// per job is padded to a multiple of kSegPad.
+constexpr size_t kHitPad = size_t(8) * kMaxWaveJobs * kSegPad;
+
 size_t wave_bytes(uint32_t cap, uint32_t tape_depth) {
   bool const replay = tape_depth > 0;
   size_t const f4 = align_up(size_t(cap) * 16);
+  size_t const hb = align_up((size_t(cap) + kHitPad) * 16);
   size_t const u4 = align_up(size_t(cap) * 16);
   size_t const qsz = align_up(size_t(cap) * 8 * 4);
   size_t const tape = replay ? align_up(size_t(cap) * tape_depth * 16) : 0;
-  return f4 * (replay ? 10 : 9) + u4 + 2 * qsz + align_up(sizeof(WaveCounters)) + 3 * tape;  // ab, c
+  return f4 * (replay ? 9 : 8) + hb + u4 + 2 * qsz + align_up(sizeof(WaveCounters)) + 3 * tape;  // ab, c
 }
 
 // Lays one lane's wave buffers out in `mem` (wave_bytes(cap, tape_depth) bytes).
 void carve_wave(void* mem, uint32_t cap, uint32_t tape_depth, WaveBufs* wb) {
   bool const replay = tape_depth > 0;
   size_t const f4 = align_up(size_t(cap) * 16);
+  size_t const hb = align_up((size_t(cap) + kHitPad) * 16);
   size_t const u4 = align_up(size_t(cap) * 16);
   size_t const qsz = align_up(size_t(cap) * 8 * 4);
   size_t const tape = replay ? align_up(size_t(cap) * tape_depth * 16) : 0;
@@ -1074,7 +1084,7 @@ void carve_wave(void* mem, uint32_t cap, uint32_t tape_depth, WaveBufs* wb) {
     return r;
   };
   wb->ps = reinterpret_cast<float4*>(take(4 * f4));
-  wb->hit = reinterpret_cast<float4*>(take(f4));
+  wb->hit = reinterpret_cast<float4*>(take(hb));
   wb->so = reinterpret_cast<float4*>(take(f4));
   wb->sw = reinterpret_cast<float4*>(take(f4));
   wb->spe = reinterpret_cast<float4*>(take(f4));
