@@ -120,11 +120,15 @@ struct WaveBufs {
   // access it in whole segments): bits(prim | back << 31), t, f, g
   float4* hit;
   float4* ctot;   // adjoint: emission per unit throughput of the light the path ends on, bits(vertex count)
-  float4* so;     // shadow: origin.xyz, nk2
-  float4* sw;     // shadow: w.xyz, nk3
-  float4* spe;    // shadow: pe.xyz, bits(slot)
-  uint4* smeta;   // shadow: expected prim, type | vertex << 8, light bsdf, t_target bits
-  float4* smet;   // shadow: fsn, fres, mx, histogram density
+  // shadow requests, one 64-byte record per request at rqs[4 * rq]:
+  // (origin, nk2), (w, nk3), (pe, bits(slot)), bits(expected prim,
+  // type | extra << 7 | vertex << 8, light bsdf), t_target.  The shadow
+  // kernel reads a request before and after its query; as one record that
+  // is one segment.  smet[rq]: fsn, fres, mx, histogram density, written and
+  // read only with `extra` (a METAL request or a guided one; otherwise
+  // 0, 0, 0, 1).
+  float4* rqs;
+  float4* smet;
   uint32_t* qa;   // 8 * cap
   uint32_t* qb;   // 8 * cap
   WaveCounters* ctr;
@@ -702,12 +706,15 @@ k_shade(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBuf
     if (active && cont) qnext[ext_entry(wj, 8u * l + oct, bbase[oct] + qoff)] = slot;
     uint32_t const rq = wj.start[l] + bbase[8] + soff;
     if (active && has_sr) {
-      wb.so[rq] = make_float4(sr.o.x, sr.o.y, sr.o.z, sr.nk2);
-      wb.sw[rq] = make_float4(sr.w.x, sr.w.y, sr.w.z, sr.nk3);
-      wb.spe[rq] = make_float4(sr.pe.x, sr.pe.y, sr.pe.z, __uint_as_float(slot));
-      wb.smeta[rq] = make_uint4(uint32_t(sr.expected), sr.type | (sr.vj << 8), sr.light_bsdf,
-                                __float_as_uint(sr.t_target));
-      wb.smet[rq] = make_float4(sr.fsn, sr.fres, sr.mx, sr.gd);
+      float4* const q = wb.rqs + 4 * size_t(rq);
+      bool const extra = (sr.type == XYZ_METAL) || (sr.gd != 1.0f);
+      q[0] = make_float4(sr.o.x, sr.o.y, sr.o.z, sr.nk2);
+      q[1] = make_float4(sr.w.x, sr.w.y, sr.w.z, sr.nk3);
+      q[2] = make_float4(sr.pe.x, sr.pe.y, sr.pe.z, __uint_as_float(slot));
+      q[3] = make_float4(__uint_as_float(uint32_t(sr.expected)),
+                         __uint_as_float(sr.type | (extra ? 0x80u : 0u) | (sr.vj << 8)),
+                         __uint_as_float(sr.light_bsdf), sr.t_target);
+      if (extra) wb.smet[rq] = make_float4(sr.fsn, sr.fres, sr.mx, sr.gd);
     }
   }
   if constexpr (kReplay) emit_lds_flush(*glds, ga.d_emit, 3 * jobs[wj.jlo].s.n_bsdfs);
@@ -763,7 +770,8 @@ k_shadow(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBu
     DevScene const& s = *sp;
     if (r != 0xFFFFFFFFu) {
       rq = wj.start[lseg] + r;
-      trace_init(st, ld3(wb.so[rq]), ld3(wb.sw[rq]));
+      float4 const* const q = wb.rqs + 4 * size_t(rq);
+      trace_init(st, ld3(q[0]), ld3(q[1]));
       // The query only asks whether its closest hit is the shading point's
       // triangle, which it can find only at that triangle's distance T (the
       // request's t_target; tri_test on the same record values gives the same
@@ -774,7 +782,7 @@ k_shadow(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBu
       // kept; a result past T comes from a leaf whose interval reaches past T,
       // and no later leaf can accept T.  So the capped query finds the
       // triangle exactly when the full query does, at t = T.
-      st.t_cap = st.t_max = __uint_as_float(wb.smeta[rq].w);
+      st.t_cap = st.t_max = q[3].w;
     }
     if (__ballot(rq != 0xFFFFFFFFu) == 0) break;
     bool const act = rq != 0xFFFFFFFFu;
@@ -784,20 +792,22 @@ k_shadow(const RJob* __restrict__ jobs, const WaveJobs* __restrict__ wjp, WaveBu
     HitRec const h = trace_result(s, st);
     uint32_t const rr = rq;
     rq = 0xFFFFFFFFu;
-    uint4 const meta = wb.smeta[rr];
+    float4 const* const q = wb.rqs + 4 * size_t(rr);
+    float4 const m4 = q[3];
+    uint3 const meta = make_uint3(__float_as_uint(m4.x), __float_as_uint(m4.y), __float_as_uint(m4.z));
     if (h.prim != int32_t(meta.x)) {
       if constexpr (kReplay) {
         // occluded: the vertex's NEE term is 0
         uint32_t const vj = meta.y >> 8;
         if (vj < wb.tape_depth)
-          wb.tape_c[size_t(vj) * wb.cap + __float_as_uint(wb.spe[rr].w)] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+          wb.tape_c[size_t(vj) * wb.cap + __float_as_uint(q[2].w)] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       }
       continue;
     }
-    float4 const o = wb.so[rr], w = wb.sw[rr];
-    float4 const pe = wb.spe[rr], met = wb.smet[rr];
+    float4 const o = q[0], w = q[1], pe = q[2];
+    float4 const met = (meta.y & 0x80u) ? wb.smet[rr] : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
     ShadowReq sr;
-    sr.type = meta.y & 0xFFu;
+    sr.type = meta.y & 0x7Fu;
     sr.nk2 = o.w;
     sr.nk3 = w.w;
     sr.pe = ld3(pe);
@@ -1063,10 +1073,10 @@ size_t wave_bytes(uint32_t cap, uint32_t tape_depth) {
   bool const replay = tape_depth > 0;
   size_t const f4 = align_up(size_t(cap) * 16);
   size_t const hb = align_up((size_t(cap) + kHitPad) * 16);
-  size_t const u4 = align_up(size_t(cap) * 16);
   size_t const qsz = align_up(size_t(cap) * 8 * 4);
   size_t const tape = replay ? align_up(size_t(cap) * tape_depth * 16) : 0;
-  return f4 * (replay ? 9 : 8) + hb + u4 + 2 * qsz + align_up(sizeof(WaveCounters)) + 3 * tape;  // ab, c
+  // path state 4, requests 4, smet 1, ctot 1 (replay); hits; queues; tape ab, c
+  return f4 * (replay ? 10 : 9) + hb + 2 * qsz + align_up(sizeof(WaveCounters)) + 3 * tape;
 }
 
 // Lays one lane's wave buffers out in `mem` (wave_bytes(cap, tape_depth) bytes).
@@ -1074,7 +1084,6 @@ void carve_wave(void* mem, uint32_t cap, uint32_t tape_depth, WaveBufs* wb) {
   bool const replay = tape_depth > 0;
   size_t const f4 = align_up(size_t(cap) * 16);
   size_t const hb = align_up((size_t(cap) + kHitPad) * 16);
-  size_t const u4 = align_up(size_t(cap) * 16);
   size_t const qsz = align_up(size_t(cap) * 8 * 4);
   size_t const tape = replay ? align_up(size_t(cap) * tape_depth * 16) : 0;
   char* b = static_cast<char*>(mem);
@@ -1085,12 +1094,10 @@ void carve_wave(void* mem, uint32_t cap, uint32_t tape_depth, WaveBufs* wb) {
   };
   wb->ps = reinterpret_cast<float4*>(take(4 * f4));
   wb->hit = reinterpret_cast<float4*>(take(hb));
-  wb->so = reinterpret_cast<float4*>(take(f4));
-  wb->sw = reinterpret_cast<float4*>(take(f4));
-  wb->spe = reinterpret_cast<float4*>(take(f4));
+  wb->rqs = reinterpret_cast<float4*>(take(4 * f4));
   wb->smet = reinterpret_cast<float4*>(take(f4));
   wb->ctot = replay ? reinterpret_cast<float4*>(take(f4)) : nullptr;
-  wb->smeta = reinterpret_cast<uint4*>(take(u4));
+
   wb->qa = reinterpret_cast<uint32_t*>(take(qsz));
   wb->qb = reinterpret_cast<uint32_t*>(take(qsz));
   wb->ctr = reinterpret_cast<WaveCounters*>(take(align_up(sizeof(WaveCounters))));
